@@ -1,0 +1,157 @@
+/*
+ * trex_hip.h -- C ABI of libtrexhip.so, the MI355X (gfx950) automaton-matching
+ * engine.
+ *
+ * Two layers of entry points:
+ *
+ *  1. Reference-compatible entry points with the exact names, argument lists
+ *     and return conventions of the reference's ctypes-bound C ABI, so its
+ *     Python bridges can load this library instead of build/lib*.so. Each
+ *     cites the reference declaration it replaces. Like the reference, they
+ *     keep one process-global engine per family.
+ *
+ *  2. Handle-based trex_* entry points used by the trex_amd Python package:
+ *     many resident patterns, device-pointer dispatch on caller streams,
+ *     int64 CSR offsets, per-string final state + accept bit.
+ *
+ * Conventions: every function returns 0 (TREX_OK) on success or a negative
+ * TREX_ERR_* code; none of them exits the process. trex_last_error() copies
+ * the message of the calling thread's last failure. A string batch is CSR:
+ * bytes[offsets[i] .. offsets[i+1]) is string i, offsets has n_strings + 1
+ * entries. "d_" arguments are device pointers, other pointers are host memory.
+ * `stream` is a hipStream_t (NULL = the default stream).
+ */
+#ifndef TREX_HIP_H
+#define TREX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TREX_OK 0
+#define TREX_ERR_NO_DEVICE (-1)  /* same meaning as the reference's -1 */
+#define TREX_ERR_ARCH (-2)       /* same meaning as the reference's -2 */
+#define TREX_ERR_INVALID (-3)
+#define TREX_ERR_CAPACITY (-4)
+#define TREX_ERR_HIP (-5)
+#define TREX_ERR_NOT_INIT (-6)
+#define TREX_ERR_TIMEOUT (-7)
+
+/* ------------------------------------------------------------------------
+ * Device and errors
+ * ------------------------------------------------------------------------ */
+
+/* 0 if a gfx950 device is present; TREX_ERR_NO_DEVICE / TREX_ERR_ARCH otherwise.
+ * Replaces monoid_batch_engine_device_check (cuda/monoid_batch.cu:601-609). */
+int trex_device_check(void);
+
+/* Copies the calling thread's last error message into buf[0..n); returns its length. */
+int trex_last_error(char* buf, int n);
+
+/* ------------------------------------------------------------------------
+ * Memory and stream plumbing (no reference counterpart: the reference copies
+ * synchronously inside each dispatch, cuda/monoid_batch.cu:504-553).
+ * ------------------------------------------------------------------------ */
+int trex_malloc(void** d_ptr, size_t bytes);
+int trex_free(void* d_ptr);
+int trex_host_alloc(void** h_ptr, size_t bytes); /* pinned host memory */
+int trex_host_free(void* h_ptr);
+int trex_memcpy_h2d(void* d_dst, const void* src, size_t n, void* stream);
+int trex_memcpy_d2h(void* dst, const void* d_src, size_t n, void* stream);
+int trex_memcpy_d2d(void* d_dst, const void* d_src, size_t n, void* stream);
+int trex_stream_create(void** stream);
+int trex_stream_destroy(void* stream);
+int trex_stream_sync(void* stream);
+int trex_device_sync(void);
+int trex_event_create(void** ev);
+int trex_event_destroy(void* ev);
+int trex_event_record(void* ev, void* stream);
+int trex_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* ------------------------------------------------------------------------
+ * DFA byte-table engine (hot path (a)).
+ * Replaces the engine behind monoid_batch_engine_init/dispatch
+ * (cuda/monoid_batch.cu:611-658) and prefix_engine_init/dispatch
+ * (cuda/prefix_compose.cu:425-470) with a handle per resident pattern.
+ * ------------------------------------------------------------------------ */
+typedef struct trex_dfa trex_dfa;
+
+/* delta: [n_states][256] uint8 successor table, accept: [n_states] (0/1),
+ * 1 <= n_states <= 256, start < n_states. The table is taken as given (it
+ * fixes what bytes outside the alphabet do). Tables with <= 16 states use the
+ * DFA-16 kernels; a 16-state table whose live bytes lie in one aligned
+ * 32-byte window uses the two-bytes-per-lookup kernel. */
+int trex_dfa_create(const uint8_t* delta, const uint8_t* accept, int n_states, int start, trex_dfa** out);
+int trex_dfa_destroy(trex_dfa* h);
+
+#define TREX_DFA_AUTO 0
+#define TREX_DFA_K1_GENERIC 1 /* any table, one LDS lookup per byte */
+#define TREX_DFA_K1_16 2      /* <= 16 states, one lookup per byte */
+#define TREX_DFA_K2_WINDOW 3  /* <= 16 states, window bytes, one lookup per byte pair */
+
+/* Reports the kernel variant chosen at create time, the window base byte
+ * (-1 if none) and the dead state (-1 if none). */
+int trex_dfa_info(const trex_dfa* h, int* variant, int* window_base, int* dead_state);
+/* Forces a variant (TREX_DFA_AUTO restores the default); fails if the table
+ * does not allow it. */
+int trex_dfa_set_variant(trex_dfa* h, int variant);
+
+/* Device-pointer dispatch, asynchronous on `stream`: final state and accept
+ * bit per string. d_offsets: int64[n_strings + 1]. */
+int trex_dfa_match(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_offsets, int64_t n_strings,
+                   uint8_t* d_final_state, uint8_t* d_accept, void* stream);
+
+/* Host-pointer dispatch (synchronous): stages the batch through engine-owned
+ * device buffers. kernel_ms / total_ms (nullable) receive the kernel time and
+ * the copy+kernel+copy time, like the reference's dispatch outputs. */
+int trex_dfa_match_host(trex_dfa* h, const uint8_t* bytes, const int64_t* offsets, int64_t n_strings,
+                        uint8_t* final_state, uint8_t* accept, float* kernel_ms, float* total_ms);
+
+/* ------------------------------------------------------------------------
+ * Reference-compatible DFA entry points (process-global engines).
+ * ------------------------------------------------------------------------ */
+
+/* cuda/prefix_compose.cu:425-433 */
+int prefix_engine_device_check(void);
+/* cuda/prefix_compose.cu:435-450. tmap[byte * N + s], unknown bytes as the
+ * caller encodes them (the reference's tables use the identity). K is
+ * ignored, as in the reference. */
+int prefix_engine_init(const uint8_t* tmap, const uint8_t* accept, int start_state, int N, int K,
+                       int max_total_chars, int max_batch);
+/* cuda/prefix_compose.cu:452-463. results[i] = accept bit (0/1). Fails with
+ * TREX_ERR_CAPACITY instead of overrunning when B or total_chars exceed the
+ * init capacities. */
+int prefix_engine_dispatch(const uint8_t* raw_concat, const int* offsets, int* results, int B,
+                           int total_chars, float* kernel_ms, float* total_ms);
+/* cuda/prefix_compose.cu:465-470 */
+void prefix_engine_destroy(void);
+
+/* cuda/monoid_batch.cu:601-609 */
+int monoid_batch_engine_device_check(void);
+/* cuda/monoid_batch.cu:611-628. The monoid index walk is run as a byte-table
+ * walk over M states: delta[m][byte] = char_compose[m * sigma_ext + raw_char_map[byte]]. */
+int monoid_batch_engine_init(int M, int sigma_ext, int identity, const uint8_t* char_compose,
+                             const uint8_t* raw_char_map, const uint8_t* accept, const uint8_t* monoid_compose,
+                             int max_total_chars, int max_batch);
+/* cuda/monoid_batch.cu:630-638. Accepted for compatibility; the byte-table
+ * walk needs no separate K-gram table. */
+int monoid_batch_engine_set_kgram(const uint8_t* fused_compose, const uint8_t* kgram_compose, int kgram_k,
+                                  int sigma_k, int sigma);
+/* cuda/monoid_batch.cu:647-658 */
+int monoid_batch_engine_dispatch(const uint8_t* raw_concat, const int* offsets, int* results, int B,
+                                 int total_chars, float* kernel_ms, float* total_ms);
+/* cuda/monoid_batch.cu:660-671 (long strings; same kernels, the split of long
+ * strings is internal) */
+int monoid_batch_engine_dispatch_prefix(const uint8_t* raw_concat, const int* offsets, int* results, int B,
+                                        int total_chars, float* kernel_ms, float* total_ms);
+/* cuda/monoid_batch.cu:640-645 */
+void monoid_batch_engine_destroy(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TREX_HIP_H */

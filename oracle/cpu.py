@@ -1,0 +1,72 @@
+"""ctypes wrapper of liboracle_cpu.so (test infrastructure only; see oracle/__init__.py)."""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle_cpu.so"
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = ctypes.CDLL(str(LIB))
+        _lib.oracle_dfa_walk.restype = ctypes.c_int
+        _lib.oracle_dfa_walk.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        _lib.oracle_nfa_walk.restype = ctypes.c_int
+        _lib.oracle_nfa_walk.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
+                                         ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    return _lib
+
+
+def default_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def dfa_walk(delta, accept, start, data, offsets, threads: int | None = None):
+    delta = np.ascontiguousarray(delta, dtype=np.uint8)
+    accept = np.ascontiguousarray(accept, dtype=np.uint8)
+    data = np.ascontiguousarray(data, dtype=np.uint8)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    n = len(offsets) - 1
+    final = np.empty(n, np.uint8)
+    acc = np.empty(n, np.uint8)
+    rc = lib().oracle_dfa_walk(delta.ctypes.data, accept.ctypes.data, delta.shape[0], int(start),
+                               data.ctypes.data, offsets.ctypes.data, n, final.ctypes.data, acc.ctypes.data,
+                               threads or default_threads())
+    if rc:
+        raise ValueError(f"oracle_dfa_walk rc={rc}")
+    return final, acc
+
+
+def nfa_walk(succ, byte_to_sym, start, accept, data, offsets, threads: int | None = None):
+    succ = np.ascontiguousarray(succ, dtype=np.uint64)
+    b2s = np.ascontiguousarray(byte_to_sym, dtype=np.int16)
+    data = np.ascontiguousarray(data, dtype=np.uint8)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    n = len(offsets) - 1
+    final = np.empty(n, np.uint64)
+    acc = np.empty(n, np.uint8)
+    rc = lib().oracle_nfa_walk(succ.ctypes.data, succ.shape[0], b2s.ctypes.data, int(start), int(accept),
+                               data.ctypes.data, offsets.ctypes.data, n, final.ctypes.data, acc.ctypes.data,
+                               threads or default_threads())
+    if rc:
+        raise ValueError(f"oracle_nfa_walk rc={rc}")
+    return final, acc

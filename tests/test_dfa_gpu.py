@@ -1,0 +1,201 @@
+"""GPU parity of the DFA byte-walk kernels against the CPU oracle.
+
+Bit-exact: final state and accept bit of every string, through the C ABI.
+"""
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import cpu as ocpu
+from oracle.dfa import dfa_simulate, table_walk
+from trex_amd.engine import DfaEngine, pack_strings, upload_batch
+from trex_amd.regex_to_dfa import compile_regex
+from trex_amd.tables import DfaByteTable, dfa_byte_table
+from trex_amd.workloads import ALPHABET_32, ALPHABET_ABCD, random_dfa_pattern, ragged_batch, uniform_batch
+
+pytestmark = pytest.mark.gpu
+GOLD = json.loads((Path(__file__).parent / "golden" / "reference_vectors.json").read_text())
+
+
+def _strings(entry):
+    if entry.get("strings") is not None:
+        return entry["strings"]
+    return [ch * n for ch, n in entry["repeat"]]
+
+
+def _check(eng: DfaEngine, tab: DfaByteTable, data, offsets):
+    f, a, kms, tms = eng.match_arrays(data, offsets)
+    rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offsets)
+    bad = np.nonzero((f != rf) | (a != ra))[0]
+    assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:5]}: gpu {f[bad[:5]]} cpu {rf[bad[:5]]}"
+    return f, a
+
+
+@pytest.mark.parametrize("entry", GOLD["regex_accept"], ids=lambda e: e["source"])
+def test_golden_regex_vectors(gpu_lib, entry):
+    dfa = compile_regex(entry["pattern"])
+    tab = dfa_byte_table(dfa)
+    eng = DfaEngine(tab)
+    strings = _strings(entry)
+    f, a, _, _ = eng.match_strings(strings)
+    assert [bool(x) for x in a] == entry["expected"]
+    rf, _ = table_walk(tab.delta, tab.accept, tab.start, *pack_strings(strings))
+    assert np.array_equal(f, rf)
+
+
+def test_reference_compat_prefix_engine_golden(gpu_lib):
+    """prefix_engine_* with the reference's (a|b)*a(a|b) tmap (identity for unknown bytes)."""
+    import ctypes
+    from trex_amd import _lib
+    e = GOLD["tmap_tables"][0]
+    n = e["N"]
+    tmap = np.tile(np.arange(n, dtype=np.uint8), 256)
+    for ch, row in e["transitions"].items():
+        tmap[ord(ch) * n: ord(ch) * n + n] = row
+    acc = np.array(e["accept"], np.uint8)
+    _lib.call("prefix_engine_init", tmap.ctypes.data_as(_lib.u8p), acc.ctypes.data_as(_lib.u8p),
+              e["start"], n, 32, 1 << 20, 1 << 16)
+    data, offs = pack_strings(e["strings"])
+    offs32 = offs.astype(np.int32)
+    res = np.zeros(len(e["strings"]), np.int32)
+    km, tm = ctypes.c_float(), ctypes.c_float()
+    _lib.call("prefix_engine_dispatch", data.ctypes.data_as(_lib.u8p), offs32.ctypes.data_as(_lib.intp),
+              res.ctypes.data_as(_lib.intp), len(e["strings"]), int(offs[-1]), ctypes.byref(km), ctypes.byref(tm))
+    assert res.tolist() == e["expected"]
+    # capacity is checked instead of overrun
+    rc = _lib.lib().prefix_engine_dispatch(data.ctypes.data_as(_lib.u8p), offs32.ctypes.data_as(_lib.intp),
+                                           res.ctypes.data_as(_lib.intp), 1 << 17, int(offs[-1]),
+                                           ctypes.byref(km), ctypes.byref(tm))
+    assert rc == -4
+    _lib.lib().prefix_engine_destroy()
+
+
+def test_reference_compat_monoid_engine_golden(gpu_lib):
+    import ctypes
+    from trex_amd import _lib
+    e = GOLD["monoid_tables"][0]
+    cc = np.array(e["char_compose"], np.uint8)
+    cmap = np.full(256, e["unmapped"], np.uint8)
+    for ch, i in e["char_map"].items():
+        cmap[ord(ch)] = i
+    acc = np.array(e["accept"], np.uint8)
+    mc = np.array([0, 1, 1, 0], np.uint8)
+    _lib.call("monoid_batch_engine_init", e["M"], e["sigma_ext"], e["identity"], cc.ctypes.data_as(_lib.u8p),
+              cmap.ctypes.data_as(_lib.u8p), acc.ctypes.data_as(_lib.u8p), mc.ctypes.data_as(_lib.u8p),
+              1 << 20, 1 << 16)
+    data, offs = pack_strings(e["strings"])
+    offs32 = offs.astype(np.int32)
+    res = np.zeros(len(e["strings"]), np.int32)
+    km, tm = ctypes.c_float(), ctypes.c_float()
+    _lib.call("monoid_batch_engine_dispatch", data.ctypes.data_as(_lib.u8p), offs32.ctypes.data_as(_lib.intp),
+              res.ctypes.data_as(_lib.intp), len(e["strings"]), int(offs[-1]), ctypes.byref(km), ctypes.byref(tm))
+    assert res.tolist() == e["expected"]
+    _lib.lib().monoid_batch_engine_destroy()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_patterns_ragged_all_variants(gpu_lib, seed):
+    rng = np.random.default_rng(1000 + seed)
+    alpha = ALPHABET_ABCD if seed % 2 == 0 else ALPHABET_32
+    _, dfa = random_dfa_pattern(rng, alpha)
+    tab = dfa_byte_table(dfa)
+    eng = DfaEngine(tab)
+    lengths = rng.integers(0, 400, size=3000)
+    lengths[:5] = [0, 1, 15, 16, 17]
+    lo, hi = ord(min(alpha)), ord(max(alpha)) + 1
+    data, offs = ragged_batch(rng, lengths, lo, hi)
+    # sprinkle bytes outside the alphabet (and outside the 32-byte window)
+    hits = rng.integers(0, len(data), size=50)
+    data[hits] = rng.integers(0, 256, size=50, dtype=np.uint8)
+    variants = ["k1_generic", "k1_16"] + (["k2_window"] if eng.info()["window_base"] >= 0 else [])
+    results = []
+    for v in variants:
+        eng.set_variant(v)
+        results.append(_check(eng, tab, data, offs))
+    for f, a in results[1:]:
+        assert np.array_equal(f, results[0][0]) and np.array_equal(a, results[0][1])
+
+
+def test_window_variant_out_of_window_bytes(gpu_lib):
+    """A byte outside the 32-byte window anywhere in a string forces the dead state."""
+    rng = np.random.default_rng(5)
+    _, dfa = random_dfa_pattern(rng, ALPHABET_32)
+    tab = dfa_byte_table(dfa)
+    eng = DfaEngine(tab)
+    assert eng.info()["variant"] == "k2_window"
+    data, offs = uniform_batch(rng, 512, 300, 0x40, 0x60)
+    for i in range(512):
+        pos = int(offs[i] + (i * 7) % 300)
+        data[pos] = [0x00, 0x3F, 0x60, 0x7F, 0xC0, 0xFF, 0x20, 0x41][i % 8]
+    f, a = _check(eng, tab, data, offs)
+
+
+def test_unaligned_offsets_and_empty(gpu_lib):
+    rng = np.random.default_rng(9)
+    _, dfa = random_dfa_pattern(rng, ALPHABET_32)
+    tab = dfa_byte_table(dfa)
+    eng = DfaEngine(tab)
+    lengths = np.array([0, 0, 3, 0, 65, 1, 0, 130, 77, 0], np.int64)
+    data, offs = ragged_batch(rng, lengths, 0x40, 0x60)
+    # offsets that start mid-buffer (first string not at 0)
+    data2 = np.concatenate([np.full(5, 0x41, np.uint8), data])
+    _check(eng, tab, data2, offs + 5)
+    f, a, _, _ = eng.match_arrays(np.zeros(0, np.uint8), np.zeros(4, np.int64))
+    assert f.tolist() == [tab.start] * 3 and a.tolist() == [tab.accept[tab.start]] * 3
+    f, a, _, _ = eng.match_arrays(np.zeros(0, np.uint8), np.zeros(1, np.int64))
+    assert len(f) == 0
+
+
+def test_wide_dfa_generic_kernel(gpu_lib):
+    """More than 16 states: the general byte walk (stride > 16)."""
+    dfa = compile_regex("(a|b)*a(a|b)(a|b)(a|b)(a|b)(a|b)")   # 2^6 = 64 states
+    tab = dfa_byte_table(dfa)
+    assert tab.n_states > 16
+    eng = DfaEngine(tab)
+    assert eng.info()["variant"] == "k1_generic"
+    rng = np.random.default_rng(3)
+    data, offs = ragged_batch(rng, rng.integers(0, 200, 2000), ord("a"), ord("c"))
+    f, a = _check(eng, tab, data, offs)
+    strings = [bytes(data[offs[i]:offs[i + 1]]).decode() for i in range(50)]
+    assert [bool(x) for x in a[:50]] == [dfa_simulate(dfa, s) for s in strings]
+
+
+def test_device_resident_dispatch_matches_host(gpu_lib):
+    from trex_amd.device import DeviceBuffer, Stream
+    rng = np.random.default_rng(77)
+    _, dfa = random_dfa_pattern(rng, ALPHABET_32)
+    tab = dfa_byte_table(dfa)
+    eng = DfaEngine(tab)
+    data, offs = uniform_batch(rng, 4096, 1024, 0x40, 0x60)
+    d_bytes, d_offs = upload_batch(data, offs)
+    n = len(offs) - 1
+    d_f, d_a = DeviceBuffer(n), DeviceBuffer(n)
+    s = Stream()
+    eng.match_device(d_bytes.ptr, d_offs.ptr, n, d_f.ptr, d_a.ptr, s)
+    s.synchronize()
+    f = d_f.download(np.uint8, n)
+    a = d_a.download(np.uint8, n)
+    rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offs)
+    assert np.array_equal(f, rf) and np.array_equal(a, ra)
+
+
+def test_config2_shape_slice_bit_exact(gpu_lib):
+    """Eight seeded 32-symbol patterns over 16,384 x 4,096 B (1/16 of the 1 GiB batch)."""
+    rng = np.random.default_rng(2026)
+    data, offs = uniform_batch(rng, 16384, 4096, 0x40, 0x60)
+    d_bytes, d_offs = upload_batch(data, offs)
+    from trex_amd.device import DeviceBuffer
+    n = len(offs) - 1
+    d_f, d_a = DeviceBuffer(n), DeviceBuffer(n)
+    for p in range(8):
+        _, dfa = random_dfa_pattern(rng, ALPHABET_32)
+        tab = dfa_byte_table(dfa)
+        eng = DfaEngine(tab)
+        eng.match_device(d_bytes.ptr, d_offs.ptr, n, d_f.ptr, d_a.ptr)
+        f = d_f.download(np.uint8, n)
+        a = d_a.download(np.uint8, n)
+        rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offs)
+        assert np.array_equal(f, rf) and np.array_equal(a, ra), f"pattern {p}"

@@ -1,0 +1,122 @@
+"""ctypes binding of ``libtrexhip.so`` (declared in ``include/trex_hip.h``).
+
+The library is loaded from this package directory only. If it is missing the
+import fails with an error naming the build command: there is no CPU
+fallback behind any matching entry point.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_char_p, c_float, c_int, c_int64, c_size_t, c_uint8, c_uint64, c_void_p, c_double
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libtrexhip.so"
+KERNEL_DIR = PKG / "kernels"
+
+TREX_OK = 0
+ERR_NAMES = {-1: "TREX_ERR_NO_DEVICE", -2: "TREX_ERR_ARCH", -3: "TREX_ERR_INVALID",
+             -4: "TREX_ERR_CAPACITY", -5: "TREX_ERR_HIP", -6: "TREX_ERR_NOT_INIT",
+             -7: "TREX_ERR_TIMEOUT"}
+
+u8p = POINTER(c_uint8)
+i64p = POINTER(c_int64)
+u64p = POINTER(c_uint64)
+f32p = POINTER(c_float)
+f64p = POINTER(c_double)
+intp = POINTER(c_int)
+vpp = POINTER(c_void_p)
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "trex_device_check": (c_int, []),
+    "trex_last_error": (c_int, [c_char_p, c_int]),
+    "trex_malloc": (c_int, [vpp, c_size_t]),
+    "trex_free": (c_int, [c_void_p]),
+    "trex_host_alloc": (c_int, [vpp, c_size_t]),
+    "trex_host_free": (c_int, [c_void_p]),
+    "trex_memcpy_h2d": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
+    "trex_memcpy_d2h": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
+    "trex_memcpy_d2d": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
+    "trex_stream_create": (c_int, [vpp]),
+    "trex_stream_destroy": (c_int, [c_void_p]),
+    "trex_stream_sync": (c_int, [c_void_p]),
+    "trex_device_sync": (c_int, []),
+    "trex_event_create": (c_int, [vpp]),
+    "trex_event_destroy": (c_int, [c_void_p]),
+    "trex_event_record": (c_int, [c_void_p, c_void_p]),
+    "trex_event_elapsed_ms": (c_int, [c_void_p, c_void_p, f32p]),
+    # DFA
+    "trex_dfa_create": (c_int, [u8p, u8p, c_int, c_int, vpp]),
+    "trex_dfa_destroy": (c_int, [c_void_p]),
+    "trex_dfa_info": (c_int, [c_void_p, intp, intp, intp]),
+    "trex_dfa_set_variant": (c_int, [c_void_p, c_int]),
+    "trex_dfa_match": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    "trex_dfa_match_host": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, f32p, f32p]),
+    # reference-compatible
+    "prefix_engine_device_check": (c_int, []),
+    "prefix_engine_init": (c_int, [u8p, u8p, c_int, c_int, c_int, c_int, c_int]),
+    "prefix_engine_dispatch": (c_int, [u8p, intp, intp, c_int, c_int, f32p, f32p]),
+    "prefix_engine_destroy": (None, []),
+    "monoid_batch_engine_device_check": (c_int, []),
+    "monoid_batch_engine_init": (c_int, [c_int, c_int, c_int, u8p, u8p, u8p, u8p, c_int, c_int]),
+    "monoid_batch_engine_set_kgram": (c_int, [u8p, u8p, c_int, c_int, c_int]),
+    "monoid_batch_engine_dispatch": (c_int, [u8p, intp, intp, c_int, c_int, f32p, f32p]),
+    "monoid_batch_engine_dispatch_prefix": (c_int, [u8p, intp, intp, c_int, c_int, f32p, f32p]),
+    "monoid_batch_engine_destroy": (None, []),
+}
+
+_lib = None
+
+
+class TrexError(RuntimeError):
+    def __init__(self, code: int, what: str, message: str):
+        super().__init__(f"{what} failed: {ERR_NAMES.get(code, code)}: {message}")
+        self.code = code
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library (loaded on first use)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists() or not list(KERNEL_DIR.glob("*.hsaco")):
+            raise ImportError(f"trex_amd native library not built ({LIB_PATH}, {KERNEL_DIR}/*.hsaco); "
+                              "run `python -m trex_amd.build`")
+        handle = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGNATURES)
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(1024)
+    lib().trex_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int, what: str) -> None:
+    if rc != TREX_OK:
+        raise TrexError(rc, what, last_error())
+
+
+def call(name: str, *args) -> int:
+    """Call ``name`` and raise :class:`TrexError` on a non-zero return."""
+    rc = getattr(lib(), name)(*args)
+    check(rc, name)
+    return rc
+
+
+def device_available() -> bool:
+    """True iff the library is built and a gfx950 device is visible."""
+    try:
+        return lib().trex_device_check() == TREX_OK
+    except (ImportError, OSError):
+        return False

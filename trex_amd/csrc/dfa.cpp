@@ -1,0 +1,346 @@
+// DFA byte-table engine: handle creation (table analysis and expansion),
+// dispatch, and the reference-compatible prefix_engine_* / monoid_batch_engine_*
+// entry points.
+
+#include <hip/hip_runtime_api.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "runtime_internal.h"
+#include "trex_hip.h"
+
+using namespace trex;
+
+struct trex_dfa {
+    int n_states = 0;
+    int stride = 16;          // row stride of t1 (states, rounded up to 16)
+    int start = 0;
+    int dead = -1;            // absorbing non-accepting state, -1 if none
+    int window_base = -1;     // aligned 32-byte window holding every live byte, -1 if none
+    int auto_variant = TREX_DFA_K1_GENERIC;
+    int variant = TREX_DFA_K1_GENERIC;
+    uint8_t* d_t1 = nullptr;  // [256][stride]
+    uint8_t* d_acc = nullptr; // [stride]
+    uint8_t* d_t2 = nullptr;  // [1024][16] pair table (window variant only)
+    // staging for trex_dfa_match_host
+    DevBuf in_bytes, in_offsets, out_state, out_accept;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::mutex mu;            // serialises host dispatches on this handle
+};
+
+namespace {
+
+// Finds the absorbing non-accepting state, or -1.
+int find_dead(const uint8_t* delta, const uint8_t* accept, int n) {
+    for (int d = 0; d < n; ++d) {
+        if (accept[d]) continue;
+        bool absorbing = true;
+        for (int b = 0; b < 256 && absorbing; ++b) absorbing = delta[d * 256 + b] == d;
+        if (absorbing) return d;
+    }
+    return -1;
+}
+
+// Aligned 32-byte window containing every byte that does not send all states
+// to `dead`; -1 if the live bytes span more than one window.
+int find_window(const uint8_t* delta, int n, int dead) {
+    if (dead < 0) return -1;
+    int window = -1;
+    for (int b = 0; b < 256; ++b) {
+        bool kills = true;
+        for (int s = 0; s < n && kills; ++s) kills = delta[s * 256 + b] == dead;
+        if (kills) continue;
+        int w = b & ~31;
+        if (window >= 0 && window != w) return -1;
+        window = w;
+    }
+    return window < 0 ? 0 : window;
+}
+
+bool variant_allowed(const trex_dfa* h, int v) {
+    switch (v) {
+        case TREX_DFA_K1_GENERIC: return true;
+        case TREX_DFA_K1_16: return h->n_states <= 16;
+        case TREX_DFA_K2_WINDOW: return h->n_states <= 16 && h->window_base >= 0 && h->d_t2 != nullptr;
+        default: return false;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int trex_dfa_create(const uint8_t* delta, const uint8_t* accept, int n_states, int start, trex_dfa** out) {
+    if (!delta || !accept || !out) return fail(TREX_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (n_states < 1 || n_states > 256) return fail(TREX_ERR_INVALID, "n_states=%d outside 1..256", n_states);
+    if (start < 0 || start >= n_states) return fail(TREX_ERR_INVALID, "start=%d outside 0..%d", start, n_states - 1);
+    for (int i = 0; i < n_states * 256; ++i)
+        if (delta[i] >= n_states) return fail(TREX_ERR_INVALID, "delta[%d]=%d names a state >= %d", i, delta[i], n_states);
+    TREX_TRY(ensure_module());
+
+    trex_dfa* h = new trex_dfa();
+    h->n_states = n_states;
+    h->start = start;
+    h->stride = std::max(16, (n_states + 15) / 16 * 16);
+    h->dead = find_dead(delta, accept, n_states);
+    h->window_base = n_states <= 16 ? find_window(delta, n_states, h->dead) : -1;
+
+    const int stride = h->stride;
+    std::vector<uint8_t> t1((size_t)256 * stride), acc((size_t)stride, 0);
+    for (int b = 0; b < 256; ++b)
+        for (int s = 0; s < stride; ++s)
+            t1[(size_t)b * stride + s] = s < n_states ? delta[s * 256 + b] : (uint8_t)s;
+    for (int s = 0; s < n_states; ++s) acc[s] = accept[s] ? 1 : 0;
+
+    int rc = upload((void**)&h->d_t1, t1.data(), t1.size());
+    if (!rc) rc = upload((void**)&h->d_acc, acc.data(), acc.size());
+    if (!rc && h->window_base >= 0) {
+        std::vector<uint8_t> t2(1024 * 16);
+        const int base = h->window_base;
+        for (int i = 0; i < 1024; ++i) {
+            const int b0 = base + (i & 31), b1 = base + (i >> 5);
+            for (int s = 0; s < 16; ++s)
+                t2[i * 16 + s] = s < n_states ? delta[delta[s * 256 + b0] * 256 + b1] : (uint8_t)s;
+        }
+        rc = upload((void**)&h->d_t2, t2.data(), t2.size());
+    }
+    if (!rc) {
+        hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+        for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&h->ev[i]);
+        if (e != hipSuccess) rc = hip_fail(e, "stream/event create");
+    }
+    if (rc) { trex_dfa_destroy(h); return rc; }
+    if (h->window_base >= 0) h->auto_variant = TREX_DFA_K2_WINDOW;
+    else if (n_states <= 16) h->auto_variant = TREX_DFA_K1_16;
+    else h->auto_variant = TREX_DFA_K1_GENERIC;
+    h->variant = h->auto_variant;
+    *out = h;
+    return TREX_OK;
+}
+
+int trex_dfa_destroy(trex_dfa* h) {
+    if (!h) return TREX_OK;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    hipFree(h->d_t1);
+    hipFree(h->d_acc);
+    hipFree(h->d_t2);
+    h->in_bytes.release();
+    h->in_offsets.release();
+    h->out_state.release();
+    h->out_accept.release();
+    for (auto& e : h->ev) if (e) hipEventDestroy(e);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return TREX_OK;
+}
+
+int trex_dfa_info(const trex_dfa* h, int* variant, int* window_base, int* dead_state) {
+    if (!h) return fail(TREX_ERR_INVALID, "null handle");
+    if (variant) *variant = h->variant;
+    if (window_base) *window_base = h->window_base;
+    if (dead_state) *dead_state = h->dead;
+    return TREX_OK;
+}
+
+int trex_dfa_set_variant(trex_dfa* h, int variant) {
+    if (!h) return fail(TREX_ERR_INVALID, "null handle");
+    if (variant == TREX_DFA_AUTO) { h->variant = h->auto_variant; return TREX_OK; }
+    if (!variant_allowed(h, variant)) return fail(TREX_ERR_INVALID, "variant %d not usable for this table", variant);
+    h->variant = variant;
+    return TREX_OK;
+}
+
+int trex_dfa_match(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_offsets, int64_t n_strings,
+                   uint8_t* d_final_state, uint8_t* d_accept, void* stream) {
+    if (!h) return fail(TREX_ERR_INVALID, "null handle");
+    if (n_strings < 0) return fail(TREX_ERR_INVALID, "n_strings < 0");
+    if (n_strings == 0) return TREX_OK;
+    if (!d_bytes || !d_offsets || !d_final_state || !d_accept) return fail(TREX_ERR_INVALID, "null buffer");
+    const int64_t grid64 = (n_strings + kBlock - 1) / kBlock;
+    if (grid64 > 0x7fffffff) return fail(TREX_ERR_CAPACITY, "too many strings");
+    const unsigned grid = (unsigned)grid64;
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t start = (uint32_t)h->start;
+    if (h->variant == TREX_DFA_K2_WINDOW) {
+        uint32_t win4 = (uint32_t)h->window_base * 0x01010101u;
+        uint32_t dead = (uint32_t)h->dead;
+        void* args[] = {&d_bytes, &d_offsets, &n_strings, &h->d_t2, &h->d_t1, &h->d_acc,
+                        &win4, &start, &dead, &d_final_state, &d_accept};
+        return launch("dfa16_k2w", grid, kBlock, 0, s, args);
+    }
+    uint32_t stride = (uint32_t)h->stride;
+    void* args[] = {&d_bytes, &d_offsets, &n_strings, &h->d_t1, &h->d_acc, &stride, &start,
+                    &d_final_state, &d_accept};
+    const unsigned lds = 256u * stride + stride;
+    if (h->variant == TREX_DFA_K1_16) return launch("dfa16_k1", grid, kBlock, lds, s, args);
+    return launch("dfa_k1", grid, kBlock, lds, s, args);
+}
+
+int trex_dfa_match_host(trex_dfa* h, const uint8_t* bytes, const int64_t* offsets, int64_t n_strings,
+                        uint8_t* final_state, uint8_t* accept, float* kernel_ms, float* total_ms) {
+    if (!h) return fail(TREX_ERR_INVALID, "null handle");
+    if (kernel_ms) *kernel_ms = 0.f;
+    if (total_ms) *total_ms = 0.f;
+    if (n_strings <= 0) return n_strings < 0 ? fail(TREX_ERR_INVALID, "n_strings < 0") : TREX_OK;
+    if (!offsets || !final_state || !accept) return fail(TREX_ERR_INVALID, "null buffer");
+    const int64_t base = offsets[0], total = offsets[n_strings] - offsets[0];
+    if (total < 0) return fail(TREX_ERR_INVALID, "offsets decrease");
+    if (total > 0 && !bytes) return fail(TREX_ERR_INVALID, "null bytes");
+    for (int64_t i = 0; i < n_strings; ++i)
+        if (offsets[i + 1] < offsets[i]) return fail(TREX_ERR_INVALID, "offsets decrease at %lld", (long long)i);
+
+    std::lock_guard<std::mutex> lock(h->mu);
+    TREX_TRY(h->in_bytes.reserve((size_t)total + 16));
+    TREX_TRY(h->in_offsets.reserve(sizeof(int64_t) * (size_t)(n_strings + 1)));
+    TREX_TRY(h->out_state.reserve((size_t)n_strings));
+    TREX_TRY(h->out_accept.reserve((size_t)n_strings));
+    // Offsets are rebased so the device batch starts at byte 0.
+    std::vector<int64_t> rebased((size_t)n_strings + 1);
+    for (int64_t i = 0; i <= n_strings; ++i) rebased[i] = offsets[i] - base;
+
+    hipStream_t s = h->stream;
+    TREX_HIP(hipEventRecord(h->ev[0], s), "hipEventRecord");
+    if (total > 0) TREX_HIP(hipMemcpyAsync(h->in_bytes.ptr, bytes + base, (size_t)total, hipMemcpyHostToDevice, s), "H2D bytes");
+    TREX_HIP(hipMemcpyAsync(h->in_offsets.ptr, rebased.data(), rebased.size() * sizeof(int64_t), hipMemcpyHostToDevice, s), "H2D offsets");
+    TREX_HIP(hipEventRecord(h->ev[1], s), "hipEventRecord");
+    TREX_TRY(trex_dfa_match(h, h->in_bytes.as<uint8_t>(), h->in_offsets.as<int64_t>(), n_strings,
+                            h->out_state.as<uint8_t>(), h->out_accept.as<uint8_t>(), s));
+    TREX_HIP(hipEventRecord(h->ev[2], s), "hipEventRecord");
+    TREX_HIP(hipMemcpyAsync(final_state, h->out_state.ptr, (size_t)n_strings, hipMemcpyDeviceToHost, s), "D2H state");
+    TREX_HIP(hipMemcpyAsync(accept, h->out_accept.ptr, (size_t)n_strings, hipMemcpyDeviceToHost, s), "D2H accept");
+    TREX_HIP(hipEventRecord(h->ev[3], s), "hipEventRecord");
+    TREX_HIP(hipEventSynchronize(h->ev[3]), "hipEventSynchronize");
+    float k = 0.f, t = 0.f;
+    hipEventElapsedTime(&k, h->ev[1], h->ev[2]);
+    hipEventElapsedTime(&t, h->ev[0], h->ev[3]);
+    if (kernel_ms) *kernel_ms = k;
+    if (total_ms) *total_ms = t;
+    return TREX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Reference-compatible singletons
+// ---------------------------------------------------------------------------
+namespace {
+
+struct CompatEngine {
+    trex_dfa* h = nullptr;
+    int max_total_chars = 0;
+    int max_batch = 0;
+    int empty_accept = 0;      // accept bit of the start state, for empty strings
+    std::vector<int64_t> offs;
+    std::vector<uint8_t> st, acc;
+    void reset() { trex_dfa_destroy(h); h = nullptr; }
+};
+
+std::mutex g_compat_mu;
+CompatEngine g_prefix, g_monoid;
+
+int compat_init(CompatEngine& eng, const std::vector<uint8_t>& delta, const std::vector<uint8_t>& accept, int n,
+                int start, int max_total_chars, int max_batch) {
+    eng.reset();
+    TREX_TRY(trex_dfa_create(delta.data(), accept.data(), n, start, &eng.h));
+    eng.max_total_chars = max_total_chars;
+    eng.max_batch = max_batch;
+    eng.empty_accept = accept[start] ? 1 : 0;
+    return TREX_OK;
+}
+
+int compat_dispatch(CompatEngine& eng, const uint8_t* raw_concat, const int* offsets, int* results, int B,
+                    int total_chars, float* kernel_ms, float* total_ms) {
+    if (!eng.h) return TREX_ERR_NO_DEVICE;  // reference: -1 when not initialised
+    if (B < 0 || total_chars < 0) return fail(TREX_ERR_INVALID, "negative size");
+    if (B > eng.max_batch || total_chars > eng.max_total_chars)
+        return fail(TREX_ERR_CAPACITY, "batch of %d strings / %d chars exceeds init capacity %d / %d", B, total_chars,
+                    eng.max_batch, eng.max_total_chars);
+    if (B == 0) { if (kernel_ms) *kernel_ms = 0.f; if (total_ms) *total_ms = 0.f; return TREX_OK; }
+    eng.offs.resize((size_t)B + 1);
+    for (int i = 0; i <= B; ++i) eng.offs[i] = offsets[i];
+    if (eng.offs[B] > total_chars) return fail(TREX_ERR_INVALID, "offsets[B] > total_chars");
+    eng.st.resize((size_t)B);
+    eng.acc.resize((size_t)B);
+    TREX_TRY(trex_dfa_match_host(eng.h, raw_concat, eng.offs.data(), B, eng.st.data(), eng.acc.data(), kernel_ms,
+                                 total_ms));
+    for (int i = 0; i < B; ++i) results[i] = eng.acc[i];
+    return TREX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int prefix_engine_device_check(void) { return trex_device_check(); }
+
+int prefix_engine_init(const uint8_t* tmap, const uint8_t* accept, int start_state, int N, int K,
+                       int max_total_chars, int max_batch) {
+    (void)K;
+    if (!tmap || !accept) return fail(TREX_ERR_INVALID, "null table");
+    if (N < 1 || N > 256) return fail(TREX_ERR_INVALID, "N=%d outside 1..256", N);
+    std::vector<uint8_t> delta((size_t)N * 256), acc(accept, accept + N);
+    for (int b = 0; b < 256; ++b)
+        for (int s = 0; s < N; ++s) delta[(size_t)s * 256 + b] = tmap[b * N + s];
+    std::lock_guard<std::mutex> lock(g_compat_mu);
+    return compat_init(g_prefix, delta, acc, N, start_state, max_total_chars, max_batch);
+}
+
+int prefix_engine_dispatch(const uint8_t* raw_concat, const int* offsets, int* results, int B, int total_chars,
+                           float* kernel_ms, float* total_ms) {
+    std::lock_guard<std::mutex> lock(g_compat_mu);
+    return compat_dispatch(g_prefix, raw_concat, offsets, results, B, total_chars, kernel_ms, total_ms);
+}
+
+void prefix_engine_destroy(void) {
+    std::lock_guard<std::mutex> lock(g_compat_mu);
+    g_prefix.reset();
+}
+
+int monoid_batch_engine_device_check(void) { return trex_device_check(); }
+
+int monoid_batch_engine_init(int M, int sigma_ext, int identity, const uint8_t* char_compose,
+                             const uint8_t* raw_char_map, const uint8_t* accept, const uint8_t* monoid_compose,
+                             int max_total_chars, int max_batch) {
+    (void)monoid_compose;
+    if (!char_compose || !raw_char_map || !accept) return fail(TREX_ERR_INVALID, "null table");
+    if (M < 1 || M > 256 || sigma_ext < 1) return fail(TREX_ERR_INVALID, "M=%d sigma_ext=%d", M, sigma_ext);
+    std::vector<uint8_t> delta((size_t)M * 256), acc(accept, accept + M);
+    for (int m = 0; m < M; ++m)
+        for (int b = 0; b < 256; ++b) {
+            int c = raw_char_map[b];
+            if (c >= sigma_ext) return fail(TREX_ERR_INVALID, "raw_char_map[%d]=%d >= sigma_ext", b, c);
+            delta[(size_t)m * 256 + b] = char_compose[m * sigma_ext + c];
+        }
+    std::lock_guard<std::mutex> lock(g_compat_mu);
+    return compat_init(g_monoid, delta, acc, M, identity, max_total_chars, max_batch);
+}
+
+int monoid_batch_engine_set_kgram(const uint8_t* fused_compose, const uint8_t* kgram_compose, int kgram_k,
+                                  int sigma_k, int sigma) {
+    (void)fused_compose; (void)kgram_compose; (void)kgram_k; (void)sigma_k; (void)sigma;
+    std::lock_guard<std::mutex> lock(g_compat_mu);
+    return g_monoid.h ? TREX_OK : TREX_ERR_NO_DEVICE;
+}
+
+int monoid_batch_engine_dispatch(const uint8_t* raw_concat, const int* offsets, int* results, int B,
+                                 int total_chars, float* kernel_ms, float* total_ms) {
+    std::lock_guard<std::mutex> lock(g_compat_mu);
+    return compat_dispatch(g_monoid, raw_concat, offsets, results, B, total_chars, kernel_ms, total_ms);
+}
+
+int monoid_batch_engine_dispatch_prefix(const uint8_t* raw_concat, const int* offsets, int* results, int B,
+                                        int total_chars, float* kernel_ms, float* total_ms) {
+    return monoid_batch_engine_dispatch(raw_concat, offsets, results, B, total_chars, kernel_ms, total_ms);
+}
+
+void monoid_batch_engine_destroy(void) {
+    std::lock_guard<std::mutex> lock(g_compat_mu);
+    g_monoid.reset();
+}
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+"""Device memory, streams and events over the C ABI (no torch dependency)."""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from trex_amd import _lib
+
+
+class Stream:
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _lib.call("trex_stream_create", ctypes.byref(h))
+        self.handle = h.value
+
+    def synchronize(self) -> None:
+        _lib.call("trex_stream_sync", self.handle)
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.lib().trex_stream_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _lib.call("trex_event_create", ctypes.byref(h))
+        self.handle = h.value
+
+    def record(self, stream: Stream | None = None) -> None:
+        _lib.call("trex_event_record", self.handle, stream.handle if stream else None)
+
+    def elapsed_ms(self, later: "Event") -> float:
+        ms = ctypes.c_float()
+        _lib.call("trex_event_elapsed_ms", self.handle, later.handle, ctypes.byref(ms))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.lib().trex_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """A block of device memory owned by this object."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        h = ctypes.c_void_p()
+        _lib.call("trex_malloc", ctypes.byref(h), max(self.nbytes, 1))
+        self.ptr = h.value
+
+    @classmethod
+    def from_array(cls, arr: np.ndarray, stream: Stream | None = None) -> "DeviceBuffer":
+        arr = np.ascontiguousarray(arr)
+        buf = cls(arr.nbytes)
+        buf.upload(arr, stream)
+        if stream is None:
+            _lib.call("trex_device_sync")
+        return buf
+
+    def upload(self, arr: np.ndarray, stream: Stream | None = None, offset: int = 0) -> None:
+        arr = np.ascontiguousarray(arr)
+        if offset + arr.nbytes > self.nbytes:
+            raise ValueError("upload overruns the buffer")
+        if arr.nbytes:
+            _lib.call("trex_memcpy_h2d", self.ptr + offset, arr.ctypes.data, arr.nbytes,
+                      stream.handle if stream else None)
+            if stream is None:
+                _lib.call("trex_device_sync")
+
+    def download(self, dtype, count: int | None = None, stream: Stream | None = None, offset: int = 0) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = (self.nbytes - offset) // dtype.itemsize
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            _lib.call("trex_memcpy_d2h", out.ctypes.data, self.ptr + offset, out.nbytes,
+                      stream.handle if stream else None)
+        if stream is None:
+            _lib.call("trex_device_sync")
+        else:
+            stream.synchronize()
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            _lib.lib().trex_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize() -> None:
+    _lib.call("trex_device_sync")
