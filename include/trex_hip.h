@@ -90,7 +90,8 @@ int trex_dfa_destroy(trex_dfa* h);
 #define TREX_DFA_AUTO 0
 #define TREX_DFA_K1_GENERIC 1 /* any table, one LDS lookup per byte */
 #define TREX_DFA_K1_16 2      /* <= 16 states, one lookup per byte */
-#define TREX_DFA_K2_WINDOW 3  /* <= 16 states, window bytes, one lookup per byte pair */
+#define TREX_DFA_K2_WINDOW 3   /* <= 16 states, window bytes, one lookup per byte pair */
+#define TREX_DFA_K2_WINDOW_F 5 /* as 3, pair table of 64-bit nibble functions */
 
 /* Reports the kernel variant chosen at create time, the window base byte
  * (-1 if none) and the dead state (-1 if none). */
@@ -99,16 +100,37 @@ int trex_dfa_info(const trex_dfa* h, int* variant, int* window_base, int* dead_s
  * does not allow it. */
 int trex_dfa_set_variant(trex_dfa* h, int variant);
 
+/* 16-byte vectors each lane keeps in flight while streaming a string (8 or 16). */
+int trex_dfa_set_prefetch(trex_dfa* h, int vectors);
+
 /* Device-pointer dispatch, asynchronous on `stream`: final state and accept
  * bit per string. d_offsets: int64[n_strings + 1]. */
 int trex_dfa_match(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_offsets, int64_t n_strings,
                    uint8_t* d_final_state, uint8_t* d_accept, void* stream);
+
+/* Same, with string i = d_bytes[d_begin[i] .. d_end[i]) (ranges may leave
+ * gaps, as in the staggered HBM layout). */
+int trex_dfa_match_ranges(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_begin, const int64_t* d_end,
+                          int64_t n_strings, uint8_t* d_final_state, uint8_t* d_accept, void* stream);
 
 /* Host-pointer dispatch (synchronous): stages the batch through engine-owned
  * device buffers. kernel_ms / total_ms (nullable) receive the kernel time and
  * the copy+kernel+copy time, like the reference's dispatch outputs. */
 int trex_dfa_match_host(trex_dfa* h, const uint8_t* bytes, const int64_t* offsets, int64_t n_strings,
                         uint8_t* final_state, uint8_t* accept, float* kernel_ms, float* total_ms);
+
+/* Plain device copy with the engine's streaming-load kernel (16-byte aligned
+ * buffers, size a multiple of 16): the reference point for HBM traffic
+ * counters and the achievable-bandwidth probe. No reference counterpart. */
+int trex_copy(void* d_dst, const void* d_src, size_t n_bytes, void* stream);
+
+/* HBM read-rate probes (no reference counterpart). mode 0: coalesced
+ * grid-stride read of d_src[0..n_bytes), d_sink holds 8192 * 256 words;
+ * mode 1 / 2: the per-string access pattern of the window kernels (8 / 16
+ * vectors in flight) over 16-byte-aligned ranges [d_begin[i], d_end[i]),
+ * d_sink holds n_strings words. */
+int trex_read_probe(const void* d_src, size_t n_bytes, const int64_t* d_begin, const int64_t* d_end,
+                    int64_t n_strings, int mode, uint32_t* d_sink, void* stream);
 
 /* ------------------------------------------------------------------------
  * Reference-compatible DFA entry points (process-global engines).

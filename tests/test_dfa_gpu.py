@@ -110,11 +110,15 @@ def test_random_patterns_ragged_all_variants(gpu_lib, seed):
     # sprinkle bytes outside the alphabet (and outside the 32-byte window)
     hits = rng.integers(0, len(data), size=50)
     data[hits] = rng.integers(0, 256, size=50, dtype=np.uint8)
-    variants = ["k1_generic", "k1_16"] + (["k2_window"] if eng.info()["window_base"] >= 0 else [])
+    variants = ["k1_generic", "k1_16"]
+    if eng.info()["window_base"] >= 0:
+        variants += ["k2_window", "k2_window_f"]
     results = []
     for v in variants:
         eng.set_variant(v)
-        results.append(_check(eng, tab, data, offs))
+        for pf in (8, 16):
+            eng.set_prefetch(pf)
+            results.append(_check(eng, tab, data, offs))
     for f, a in results[1:]:
         assert np.array_equal(f, results[0][0]) and np.array_equal(a, results[0][1])
 
@@ -130,7 +134,36 @@ def test_window_variant_out_of_window_bytes(gpu_lib):
     for i in range(512):
         pos = int(offs[i] + (i * 7) % 300)
         data[pos] = [0x00, 0x3F, 0x60, 0x7F, 0xC0, 0xFF, 0x20, 0x41][i % 8]
-    f, a = _check(eng, tab, data, offs)
+    for v in ["k2_window", "k2_window_f"]:
+        eng.set_variant(v)
+        f, a = _check(eng, tab, data, offs)
+        assert np.all(f[np.arange(512) % 8 != 7] == tab.dead)
+
+
+@pytest.mark.parametrize("layout", ["csr", "staggered"])
+def test_device_batch_layouts_bit_exact(gpu_lib, layout):
+    """Ragged batch in both HBM layouts, every variant, against the oracle."""
+    from trex_amd.device import DeviceBuffer
+    from trex_amd.engine import DeviceBatch
+    rng = np.random.default_rng(31)
+    _, dfa = random_dfa_pattern(rng, ALPHABET_32)
+    tab = dfa_byte_table(dfa)
+    eng = DfaEngine(tab)
+    lengths = rng.integers(0, 3000, size=1000)
+    lengths[::97] = 4096
+    data, offs = ragged_batch(rng, lengths, 0x40, 0x60)
+    data[rng.integers(0, len(data), 20)] = 0x21
+    batch = DeviceBatch(data, offs, layout=layout)
+    n = len(offs) - 1
+    d_f, d_a = DeviceBuffer(n), DeviceBuffer(n)
+    rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offs)
+    for v in ["k1_16", "k2_window", "k2_window_f"]:
+        eng.set_variant(v)
+        for pf in (8, 16):
+            eng.set_prefetch(pf)
+            eng.match_batch_device(batch, d_f.ptr, d_a.ptr)
+            assert np.array_equal(d_f.download(np.uint8, n), rf), (v, pf)
+            assert np.array_equal(d_a.download(np.uint8, n), ra), (v, pf)
 
 
 def test_unaligned_offsets_and_empty(gpu_lib):

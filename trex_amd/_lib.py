@@ -52,6 +52,11 @@ _SIGNATURES = {
     "trex_dfa_destroy": (c_int, [c_void_p]),
     "trex_dfa_info": (c_int, [c_void_p, intp, intp, intp]),
     "trex_dfa_set_variant": (c_int, [c_void_p, c_int]),
+    "trex_dfa_set_prefetch": (c_int, [c_void_p, c_int]),
+    "trex_dfa_match_ranges": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
+                                      c_void_p]),
+    "trex_copy": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
+    "trex_read_probe": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p]),
     "trex_dfa_match": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
     "trex_dfa_match_host": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, f32p, f32p]),
     # reference-compatible

@@ -24,7 +24,9 @@ struct trex_dfa {
     int variant = TREX_DFA_K1_GENERIC;
     uint8_t* d_t1 = nullptr;  // [256][stride]
     uint8_t* d_acc = nullptr; // [stride]
-    uint8_t* d_t2 = nullptr;  // [1024][16] pair table (window variant only)
+    uint8_t* d_t2 = nullptr;  // [1024][16] pair table t2[pair * 16 + s] (window variants only)
+    uint8_t* d_f2 = nullptr;  // [1024] uint64 pair functions, nibble s = successor of s
+    int prefetch = 16;        // 16-byte vectors in flight per lane (8 or 16)
     // staging for trex_dfa_match_host
     DevBuf in_bytes, in_offsets, out_state, out_accept;
     hipStream_t stream = nullptr;
@@ -65,7 +67,8 @@ bool variant_allowed(const trex_dfa* h, int v) {
     switch (v) {
         case TREX_DFA_K1_GENERIC: return true;
         case TREX_DFA_K1_16: return h->n_states <= 16;
-        case TREX_DFA_K2_WINDOW: return h->n_states <= 16 && h->window_base >= 0 && h->d_t2 != nullptr;
+        case TREX_DFA_K2_WINDOW:
+        case TREX_DFA_K2_WINDOW_F: return h->n_states <= 16 && h->window_base >= 0 && h->d_t2 != nullptr;
         default: return false;
     }
 }
@@ -101,13 +104,18 @@ int trex_dfa_create(const uint8_t* delta, const uint8_t* accept, int n_states, i
     if (!rc) rc = upload((void**)&h->d_acc, acc.data(), acc.size());
     if (!rc && h->window_base >= 0) {
         std::vector<uint8_t> t2(1024 * 16);
+        std::vector<uint64_t> f2(1024, 0);
         const int base = h->window_base;
         for (int i = 0; i < 1024; ++i) {
             const int b0 = base + (i & 31), b1 = base + (i >> 5);
-            for (int s = 0; s < 16; ++s)
-                t2[i * 16 + s] = s < n_states ? delta[delta[s * 256 + b0] * 256 + b1] : (uint8_t)s;
+            for (int s = 0; s < 16; ++s) {
+                const uint8_t nxt = s < n_states ? delta[delta[s * 256 + b0] * 256 + b1] : (uint8_t)s;
+                t2[i * 16 + s] = nxt;
+                f2[i] |= (uint64_t)nxt << (4 * s);
+            }
         }
         rc = upload((void**)&h->d_t2, t2.data(), t2.size());
+        if (!rc) rc = upload((void**)&h->d_f2, f2.data(), f2.size() * sizeof(uint64_t));
     }
     if (!rc) {
         hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
@@ -129,6 +137,7 @@ int trex_dfa_destroy(trex_dfa* h) {
     hipFree(h->d_t1);
     hipFree(h->d_acc);
     hipFree(h->d_t2);
+    hipFree(h->d_f2);
     h->in_bytes.release();
     h->in_offsets.release();
     h->out_state.release();
@@ -155,30 +164,50 @@ int trex_dfa_set_variant(trex_dfa* h, int variant) {
     return TREX_OK;
 }
 
-int trex_dfa_match(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_offsets, int64_t n_strings,
-                   uint8_t* d_final_state, uint8_t* d_accept, void* stream) {
+int trex_dfa_set_prefetch(trex_dfa* h, int vectors) {
+    if (!h) return fail(TREX_ERR_INVALID, "null handle");
+    if (vectors != 8 && vectors != 16) return fail(TREX_ERR_INVALID, "prefetch must be 8 or 16 vectors");
+    h->prefetch = vectors;
+    return TREX_OK;
+}
+
+int trex_dfa_match_ranges(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_begin, const int64_t* d_end,
+                          int64_t n_strings, uint8_t* d_final_state, uint8_t* d_accept, void* stream) {
     if (!h) return fail(TREX_ERR_INVALID, "null handle");
     if (n_strings < 0) return fail(TREX_ERR_INVALID, "n_strings < 0");
     if (n_strings == 0) return TREX_OK;
-    if (!d_bytes || !d_offsets || !d_final_state || !d_accept) return fail(TREX_ERR_INVALID, "null buffer");
+    if (!d_bytes || !d_begin || !d_end || !d_final_state || !d_accept) return fail(TREX_ERR_INVALID, "null buffer");
     const int64_t grid64 = (n_strings + kBlock - 1) / kBlock;
     if (grid64 > 0x7fffffff) return fail(TREX_ERR_CAPACITY, "too many strings");
     const unsigned grid = (unsigned)grid64;
     hipStream_t s = (hipStream_t)stream;
     uint32_t start = (uint32_t)h->start;
-    if (h->variant == TREX_DFA_K2_WINDOW) {
+    if (h->variant == TREX_DFA_K2_WINDOW || h->variant == TREX_DFA_K2_WINDOW_F) {
         uint32_t win4 = (uint32_t)h->window_base * 0x01010101u;
         uint32_t dead = (uint32_t)h->dead;
-        void* args[] = {&d_bytes, &d_offsets, &n_strings, &h->d_t2, &h->d_t1, &h->d_acc,
+        const bool pf16 = h->prefetch == 16;
+        uint8_t* table = h->d_t2;
+        const char* name = pf16 ? "dfa16_k2w_pf16" : "dfa16_k2w_pf8";
+        if (h->variant == TREX_DFA_K2_WINDOW_F) {
+            table = h->d_f2;
+            name = pf16 ? "dfa16_k2f_pf16" : "dfa16_k2f_pf8";
+        }
+        void* args[] = {&d_bytes, &d_begin, &d_end, &n_strings, &table, &h->d_t1, &h->d_acc,
                         &win4, &start, &dead, &d_final_state, &d_accept};
-        return launch("dfa16_k2w", grid, kBlock, 0, s, args);
+        return launch(name, grid, kBlock, 0, s, args);
     }
     uint32_t stride = (uint32_t)h->stride;
-    void* args[] = {&d_bytes, &d_offsets, &n_strings, &h->d_t1, &h->d_acc, &stride, &start,
+    void* args[] = {&d_bytes, &d_begin, &d_end, &n_strings, &h->d_t1, &h->d_acc, &stride, &start,
                     &d_final_state, &d_accept};
     const unsigned lds = 256u * stride + stride;
     if (h->variant == TREX_DFA_K1_16) return launch("dfa16_k1", grid, kBlock, lds, s, args);
     return launch("dfa_k1", grid, kBlock, lds, s, args);
+}
+
+int trex_dfa_match(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_offsets, int64_t n_strings,
+                   uint8_t* d_final_state, uint8_t* d_accept, void* stream) {
+    if (!d_offsets) return fail(TREX_ERR_INVALID, "null offsets");
+    return trex_dfa_match_ranges(h, d_bytes, d_offsets, d_offsets + 1, n_strings, d_final_state, d_accept, stream);
 }
 
 int trex_dfa_match_host(trex_dfa* h, const uint8_t* bytes, const int64_t* offsets, int64_t n_strings,
@@ -221,6 +250,32 @@ int trex_dfa_match_host(trex_dfa* h, const uint8_t* bytes, const int64_t* offset
     if (kernel_ms) *kernel_ms = k;
     if (total_ms) *total_ms = t;
     return TREX_OK;
+}
+
+int trex_read_probe(const void* d_src, size_t n_bytes, const int64_t* d_begin, const int64_t* d_end,
+                    int64_t n_strings, int mode, uint32_t* d_sink, void* stream) {
+    if (((uintptr_t)d_src & 15) || (n_bytes & 15)) return fail(TREX_ERR_INVALID, "probe buffer must be 16-byte aligned");
+    hipStream_t s = (hipStream_t)stream;
+    if (mode == 0) {
+        int64_t n16 = (int64_t)(n_bytes / 16);
+        const unsigned grid = 8192;   // d_sink must hold grid * 256 words
+        void* args[] = {&d_src, &n16, &d_sink};
+        return launch("read_probe_stream", grid, kBlock, 0, s, args);
+    }
+    if (!d_begin || !d_end || n_strings <= 0) return fail(TREX_ERR_INVALID, "string probe needs ranges");
+    const unsigned grid = (unsigned)((n_strings + kBlock - 1) / kBlock);
+    void* args[] = {&d_src, &d_begin, &d_end, &n_strings, &d_sink};
+    return launch(mode == 2 ? "read_probe_strings_pf16" : "read_probe_strings_pf8", grid, kBlock, 0, s, args);
+}
+
+int trex_copy(void* d_dst, const void* d_src, size_t n_bytes, void* stream) {
+    if (n_bytes % 16 != 0) return fail(TREX_ERR_INVALID, "copy size must be a multiple of 16");
+    if (((uintptr_t)d_dst | (uintptr_t)d_src) & 15) return fail(TREX_ERR_INVALID, "copy buffers must be 16-byte aligned");
+    int64_t n16 = (int64_t)(n_bytes / 16);
+    const int64_t want = (n16 + kBlock - 1) / kBlock;
+    const unsigned grid = (unsigned)std::min<int64_t>(want, 8192);
+    void* args[] = {&d_src, &d_dst, &n16};
+    return launch("copy_u4", grid, kBlock, 0, (hipStream_t)stream, args);
 }
 
 }  // extern "C"

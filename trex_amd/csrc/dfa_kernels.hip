@@ -1,40 +1,44 @@
 // DFA byte-walk kernels for gfx950 (CDNA4, wave64).
 //
-// One lane walks one string of a CSR batch (bytes + int64 offsets) through a
-// byte transition table held in LDS and writes the final state and its
-// accept bit. This is the work of the reference's monoid_batch_kernel /
-// prefix_fused_kernel (cuda/monoid_batch.cu:58-95, cuda/prefix_compose.cu:217-258),
-// reorganised for HBM streaming:
+// One lane walks one string through a byte transition table held in LDS and
+// writes the final state and its accept bit. This is the work of the
+// reference's monoid_batch_kernel / prefix_fused_kernel
+// (cuda/monoid_batch.cu:58-95, cuda/prefix_compose.cu:217-258), reorganised
+// for HBM streaming:
 //
-//  * the body of each string is read with 16-byte global loads, four vectors
-//    in flight per lane while the previous four are consumed;
-//  * dfa16_k2w folds two bytes per LDS lookup ("K=4": one 32-bit word = two
-//    pair lookups) when every byte that does not lead straight to the dead
-//    state lies in one aligned 32-byte window [base, base+32). Its table is
-//    t2[(sym0 | sym1 << 5) * 16 + s] = delta(delta(s, base+sym0), base+sym1).
-//    A byte outside the window sends every state to the absorbing dead
-//    state, so the kernel only records whether one was seen (two bit ops per
-//    word) and forces the dead state at the end;
-//  * dfa_k1 is the general walk over any table of up to 256 states, one
-//    lookup per byte: t1[byte * stride + s].
+//  * string i is bytes[begin[i] .. end[i]); a CSR batch passes
+//    begin = offsets, end = offsets + 1, and the staggered HBM layout (strings
+//    grouped per wave, groups separated by a gap so that concurrent waves read
+//    different HBM channels) passes its own begin/end arrays;
+//  * the body of each string is read with 16-byte global loads through a
+//    rotating buffer of PF vectors per lane: every consumed slot is refilled
+//    at once with the vector PF positions ahead, so PF loads stay in flight;
+//  * the "window" kernels fold two bytes per table step ("K=4": one 32-bit
+//    word = two pair steps) when every byte that does not lead straight to
+//    the dead state lies in one aligned 32-byte window [base, base+32); the
+//    pair index is sym0 | sym1 << 5 with sym = byte & 31. A byte outside the
+//    window sends every state to the absorbing dead state, so the kernels
+//    only record whether one was seen (two bit operations per word) and force
+//    the dead state at the end. Two pair-table forms:
+//      k2w  state walk, t2[pair * 16 + s] (16 KB, uint8)
+//      k2f  function table, f2[pair] = 16 nibbles (8 KB, uint64): the LDS
+//           read does not depend on the state, so it leaves the dependent
+//           chain, which becomes s = (f >> 4s) & 15
+//  * dfa_k1 / dfa16_k1 are the general walk, one lookup per byte:
+//    t1[byte * stride + s], for any table of up to 256 states.
 //
 // Bytes before the first 16-byte boundary and after the last one go through
-// the exact per-byte table t1 in both kernels.
+// the exact per-byte table t1 in every kernel.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #define TREX_BLOCK 256
+// The string kernels run one wave per 64 strings; four waves per SIMD must stay
+// resident, which caps them at 128 VGPRs.
+#define TREX_STRING_KERNEL __global__ void __launch_bounds__(TREX_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 
 namespace {
-
-__device__ __forceinline__ uint32_t k2w_word(const uint8_t* __restrict__ t2, uint32_t s, uint32_t x) {
-    uint32_t i0 = (x & 31u) | (((x >> 8) & 31u) << 5);
-    uint32_t i1 = ((x >> 16) & 31u) | (((x >> 24) & 31u) << 5);
-    s = t2[(i0 << 4) | s];
-    s = t2[(i1 << 4) | s];
-    return s;
-}
 
 struct WinCheck {
     uint32_t any_hi;   // OR of the top three bits of every body byte
@@ -47,19 +51,43 @@ struct WinCheck {
     }
 };
 
-__device__ __forceinline__ uint32_t k2w_vec(const uint8_t* __restrict__ t2, uint32_t s, uint4 v, WinCheck& w) {
+__device__ __forceinline__ uint32_t pair_lo(uint32_t x) { return (x & 31u) | (((x >> 8) & 31u) << 5); }
+__device__ __forceinline__ uint32_t pair_hi(uint32_t x) { return ((x >> 16) & 31u) | (((x >> 24) & 31u) << 5); }
+
+// Pair-step policies over the LDS pair table `t`.
+struct StepK2W {
+    static constexpr int kTableBytes = 1024 * 16;
+    __device__ __forceinline__ static uint32_t word(const uint8_t* __restrict__ t, uint32_t s, uint32_t x) {
+        s = t[(pair_lo(x) << 4) | s];
+        return t[(pair_hi(x) << 4) | s];
+    }
+};
+
+struct StepK2F {
+    static constexpr int kTableBytes = 1024 * 8;
+    __device__ __forceinline__ static uint32_t word(const uint8_t* __restrict__ t, uint32_t s, uint32_t x) {
+        const uint64_t* f = reinterpret_cast<const uint64_t*>(t);
+        const uint64_t f0 = f[pair_lo(x)];
+        const uint64_t f1 = f[pair_hi(x)];
+        s = (uint32_t)(f0 >> (s << 2)) & 15u;
+        return (uint32_t)(f1 >> (s << 2)) & 15u;
+    }
+};
+
+template <class Step>
+__device__ __forceinline__ uint32_t window_vec(const uint8_t* __restrict__ t, uint32_t s, uint4 v, WinCheck& w) {
     w.add(v.x); w.add(v.y); w.add(v.z); w.add(v.w);
-    s = k2w_word(t2, s, v.x);
-    s = k2w_word(t2, s, v.y);
-    s = k2w_word(t2, s, v.z);
-    s = k2w_word(t2, s, v.w);
+    s = Step::word(t, s, v.x);
+    s = Step::word(t, s, v.y);
+    s = Step::word(t, s, v.z);
+    s = Step::word(t, s, v.w);
     return s;
 }
 
 template <int STRIDE_LOG2>
 __device__ __forceinline__ uint32_t k1_byte(const uint8_t* __restrict__ t1, uint32_t s, uint32_t b, uint32_t stride) {
-    if (STRIDE_LOG2 >= 0) return t1[(b << STRIDE_LOG2) | s];
-    return t1[b * stride + s];
+    if constexpr (STRIDE_LOG2 >= 0) return t1[(b << STRIDE_LOG2) | s];
+    else return t1[b * stride + s];
 }
 
 template <int STRIDE_LOG2>
@@ -87,52 +115,65 @@ __device__ __forceinline__ void stage_table(uint8_t* __restrict__ dst, const uin
     for (int i = threadIdx.x; i < (n >> 4); i += blockDim.x) d4[i] = s4[i];
 }
 
-}  // namespace
+// Number of bytes from `p` to the next 16-byte boundary of the address.
+__device__ __forceinline__ int64_t head_len(const uint8_t* bytes, int64_t p) {
+    return (int64_t)((16u - ((uint32_t)(uintptr_t)(bytes + p) & 15u)) & 15u);
+}
 
-// ---------------------------------------------------------------------------
-// dfa16_k2w: 16-state table, window fast path, two bytes per lookup.
-//   t2g  [1024 * 16] pair table, t1g [256 * 16] byte table, accg [16]
-//   win4 = window base byte replicated in all four bytes (base % 32 == 0)
-// ---------------------------------------------------------------------------
-extern "C" __global__ void __launch_bounds__(TREX_BLOCK)
-dfa16_k2w(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ offsets, int64_t n_strings,
-          const uint8_t* __restrict__ t2g, const uint8_t* __restrict__ t1g, const uint8_t* __restrict__ accg,
-          uint32_t win4, uint32_t start, uint32_t dead,
-          uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
+// Streams v[0 .. nvec) through `consume` in order with two blocks of PF
+// 16-byte vectors: the loads of block i + 1 are issued before block i is
+// consumed, so PF loads are in flight while the table walk runs.
+template <int PF, class F>
+__device__ __forceinline__ void stream_body(const uint4* __restrict__ v, int64_t nvec, F&& consume) {
+    int64_t i = 0;
+    if (nvec >= PF) {
+        uint4 a[PF];
+#pragma unroll
+        for (int k = 0; k < PF; ++k) a[k] = v[k];
+        for (i = PF; i + PF <= nvec; i += PF) {
+            uint4 b[PF];
+#pragma unroll
+            for (int k = 0; k < PF; ++k) b[k] = v[i + k];
+#pragma unroll
+            for (int k = 0; k < PF; ++k) consume(a[k]);
+#pragma unroll
+            for (int k = 0; k < PF; ++k) a[k] = b[k];
+        }
+#pragma unroll
+        for (int k = 0; k < PF; ++k) consume(a[k]);
+    }
+    for (; i < nvec; ++i) consume(v[i]);
+}
+
+template <class Step, int PF>
+__device__ __forceinline__ void window_kernel(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
+                                              const int64_t* __restrict__ endp, int64_t n_strings,
+                                              const uint8_t* __restrict__ tpg, const uint8_t* __restrict__ t1g,
+                                              const uint8_t* __restrict__ accg, uint32_t win4, uint32_t start,
+                                              uint32_t dead, uint8_t* __restrict__ out_state,
+                                              uint8_t* __restrict__ out_accept)
 {
-    __shared__ __attribute__((aligned(16))) uint8_t t2[1024 * 16];
+    __shared__ __attribute__((aligned(16))) uint8_t tp[Step::kTableBytes];
     __shared__ __attribute__((aligned(16))) uint8_t t1[256 * 16];
     __shared__ __attribute__((aligned(16))) uint8_t acc[16];
-    stage_table(t2, t2g, 1024 * 16);
+    stage_table(tp, tpg, Step::kTableBytes);
     stage_table(t1, t1g, 256 * 16);
     if (threadIdx.x < 16) acc[threadIdx.x] = accg[threadIdx.x];
     __syncthreads();
 
     const int64_t sid = (int64_t)blockIdx.x * TREX_BLOCK + threadIdx.x;
     if (sid >= n_strings) return;
-    int64_t p = offsets[sid];
-    const int64_t end = offsets[sid + 1];
+    int64_t p = begin[sid];
+    const int64_t end = endp[sid];
     uint32_t s = start;
 
-    const int64_t head_end = min(end, p + (int64_t)((16u - ((uint32_t)(uintptr_t)(bytes + p) & 15u)) & 15u));
+    const int64_t head_end = min(end, p + head_len(bytes, p));
     for (; p < head_end; ++p) s = t1[((uint32_t)bytes[p] << 4) | s];
 
     WinCheck w; w.init();
-    const uint4* v = reinterpret_cast<const uint4*>(bytes + p);
     const int64_t nvec = (end - p) >> 4;
-    int64_t i = 0;
-    if (nvec >= 4) {
-        uint4 a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
-        for (i = 4; i + 4 <= nvec; i += 4) {
-            const uint4 b0 = v[i], b1 = v[i + 1], b2 = v[i + 2], b3 = v[i + 3];
-            s = k2w_vec(t2, s, a0, w); s = k2w_vec(t2, s, a1, w);
-            s = k2w_vec(t2, s, a2, w); s = k2w_vec(t2, s, a3, w);
-            a0 = b0; a1 = b1; a2 = b2; a3 = b3;
-        }
-        s = k2w_vec(t2, s, a0, w); s = k2w_vec(t2, s, a1, w);
-        s = k2w_vec(t2, s, a2, w); s = k2w_vec(t2, s, a3, w);
-    }
-    for (; i < nvec; ++i) s = k2w_vec(t2, s, v[i], w);
+    stream_body<PF>(reinterpret_cast<const uint4*>(bytes + p), nvec,
+                    [&](uint4 x) { s = window_vec<Step>(tp, s, x, w); });
     p += nvec << 4;
     for (; p < end; ++p) s = t1[((uint32_t)bytes[p] << 4) | s];
     if (w.outside(win4)) s = dead;
@@ -141,16 +182,12 @@ dfa16_k2w(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ offsets
     out_accept[sid] = acc[s];
 }
 
-// ---------------------------------------------------------------------------
-// dfa_k1: general byte walk, up to 256 states.
-//   t1g [256 * stride] with t1[byte * stride + s]; accg [stride]
-//   Dynamic LDS: 256 * stride + stride bytes (stride a multiple of 16).
-// ---------------------------------------------------------------------------
 template <int STRIDE_LOG2>
-__device__ __forceinline__ void dfa_k1_body(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ offsets,
-                                            int64_t n_strings, const uint8_t* __restrict__ t1g,
-                                            const uint8_t* __restrict__ accg, uint32_t stride, uint32_t start,
-                                            uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
+__device__ __forceinline__ void k1_kernel(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
+                                          const int64_t* __restrict__ endp, int64_t n_strings,
+                                          const uint8_t* __restrict__ t1g, const uint8_t* __restrict__ accg,
+                                          uint32_t stride, uint32_t start, uint8_t* __restrict__ out_state,
+                                          uint8_t* __restrict__ out_accept)
 {
     extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
     uint8_t* t1 = lds;
@@ -161,28 +198,15 @@ __device__ __forceinline__ void dfa_k1_body(const uint8_t* __restrict__ bytes, c
 
     const int64_t sid = (int64_t)blockIdx.x * TREX_BLOCK + threadIdx.x;
     if (sid >= n_strings) return;
-    int64_t p = offsets[sid];
-    const int64_t end = offsets[sid + 1];
+    int64_t p = begin[sid];
+    const int64_t end = endp[sid];
     uint32_t s = start;
 
-    const int64_t head_end = min(end, p + (int64_t)((16u - ((uint32_t)(uintptr_t)(bytes + p) & 15u)) & 15u));
+    const int64_t head_end = min(end, p + head_len(bytes, p));
     for (; p < head_end; ++p) s = k1_byte<STRIDE_LOG2>(t1, s, bytes[p], stride);
-
-    const uint4* v = reinterpret_cast<const uint4*>(bytes + p);
     const int64_t nvec = (end - p) >> 4;
-    int64_t i = 0;
-    if (nvec >= 4) {
-        uint4 a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
-        for (i = 4; i + 4 <= nvec; i += 4) {
-            const uint4 b0 = v[i], b1 = v[i + 1], b2 = v[i + 2], b3 = v[i + 3];
-            s = k1_vec<STRIDE_LOG2>(t1, s, a0, stride); s = k1_vec<STRIDE_LOG2>(t1, s, a1, stride);
-            s = k1_vec<STRIDE_LOG2>(t1, s, a2, stride); s = k1_vec<STRIDE_LOG2>(t1, s, a3, stride);
-            a0 = b0; a1 = b1; a2 = b2; a3 = b3;
-        }
-        s = k1_vec<STRIDE_LOG2>(t1, s, a0, stride); s = k1_vec<STRIDE_LOG2>(t1, s, a1, stride);
-        s = k1_vec<STRIDE_LOG2>(t1, s, a2, stride); s = k1_vec<STRIDE_LOG2>(t1, s, a3, stride);
-    }
-    for (; i < nvec; ++i) s = k1_vec<STRIDE_LOG2>(t1, s, v[i], stride);
+    stream_body<8>(reinterpret_cast<const uint4*>(bytes + p), nvec,
+                   [&](uint4 x) { s = k1_vec<STRIDE_LOG2>(t1, s, x, stride); });
     p += nvec << 4;
     for (; p < end; ++p) s = k1_byte<STRIDE_LOG2>(t1, s, bytes[p], stride);
 
@@ -190,18 +214,107 @@ __device__ __forceinline__ void dfa_k1_body(const uint8_t* __restrict__ bytes, c
     out_accept[sid] = acc[s];
 }
 
-extern "C" __global__ void __launch_bounds__(TREX_BLOCK)
-dfa16_k1(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ offsets, int64_t n_strings,
-         const uint8_t* __restrict__ t1g, const uint8_t* __restrict__ accg, uint32_t stride, uint32_t start,
-         uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Window kernels: 16-state table, two bytes per step.
+//   tpg: pair table (k2w: 16 KB uint8, k2f: 8 KB uint64)
+//   t1g [256 * 16] byte table, accg [16]
+//   win4 = window base byte replicated in all four bytes (base % 32 == 0)
+// ---------------------------------------------------------------------------
+#define TREX_WINDOW_KERNEL(NAME, STEP, PF, ATTR)                                                              \
+    extern "C" __global__ void __launch_bounds__(TREX_BLOCK) ATTR                                            \
+    NAME(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin, const int64_t* __restrict__ end, \
+         int64_t n_strings, const uint8_t* __restrict__ tpg, const uint8_t* __restrict__ t1g,                \
+         const uint8_t* __restrict__ accg, uint32_t win4, uint32_t start, uint32_t dead,                     \
+         uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept) {                                \
+        window_kernel<STEP, PF>(bytes, begin, end, n_strings, tpg, t1g, accg, win4, start, dead, out_state,  \
+                                out_accept);                                                                  \
+    }
+
+#define TREX_W4 __attribute__((amdgpu_waves_per_eu(4)))
+#define TREX_NOATTR
+TREX_WINDOW_KERNEL(dfa16_k2w_pf8, StepK2W, 8, TREX_NOATTR)
+TREX_WINDOW_KERNEL(dfa16_k2w_pf16, StepK2W, 16, TREX_NOATTR)
+TREX_WINDOW_KERNEL(dfa16_k2f_pf8, StepK2F, 8, TREX_NOATTR)
+TREX_WINDOW_KERNEL(dfa16_k2f_pf16, StepK2F, 16, TREX_NOATTR)
+
+// ---------------------------------------------------------------------------
+// General byte walk.
+//   t1g [256 * stride] with t1[byte * stride + s]; accg [stride]
+//   Dynamic LDS: 256 * stride + stride bytes (stride a multiple of 16).
+// ---------------------------------------------------------------------------
+extern "C" TREX_STRING_KERNEL
+dfa16_k1(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin, const int64_t* __restrict__ end,
+         int64_t n_strings, const uint8_t* __restrict__ t1g, const uint8_t* __restrict__ accg, uint32_t stride,
+         uint32_t start, uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
 {
-    dfa_k1_body<4>(bytes, offsets, n_strings, t1g, accg, 16u, start, out_state, out_accept);
+    k1_kernel<4>(bytes, begin, end, n_strings, t1g, accg, 16u, start, out_state, out_accept);
+}
+
+extern "C" TREX_STRING_KERNEL
+dfa_k1(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin, const int64_t* __restrict__ end,
+       int64_t n_strings, const uint8_t* __restrict__ t1g, const uint8_t* __restrict__ accg, uint32_t stride,
+       uint32_t start, uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
+{
+    k1_kernel<-1>(bytes, begin, end, n_strings, t1g, accg, stride, start, out_state, out_accept);
+}
+
+// ---------------------------------------------------------------------------
+// Read probes: the HBM read rate of (a) a fully coalesced grid-stride stream
+// and (b) the per-string access pattern of the window kernels with no table
+// work (XOR of every word), over the same begin/end ranges the DFA kernels
+// take. Each lane writes one word so the loads are kept.
+// ---------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(TREX_BLOCK)
+read_probe_stream(const uint4* __restrict__ src, int64_t n16, uint32_t* __restrict__ sink)
+{
+    const int64_t stride = (int64_t)gridDim.x * TREX_BLOCK;
+    uint32_t x = 0;
+    for (int64_t i = (int64_t)blockIdx.x * TREX_BLOCK + threadIdx.x; i < n16; i += stride) {
+        const uint4 v = src[i];
+        x ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    sink[(int64_t)blockIdx.x * TREX_BLOCK + threadIdx.x] = x;
+}
+
+template <int PF>
+__device__ __forceinline__ void read_probe_strings_body(const uint8_t* __restrict__ bytes,
+                                                        const int64_t* __restrict__ begin,
+                                                        const int64_t* __restrict__ endp, int64_t n_strings,
+                                                        uint32_t* __restrict__ sink)
+{
+    const int64_t sid = (int64_t)blockIdx.x * TREX_BLOCK + threadIdx.x;
+    if (sid >= n_strings) return;
+    const int64_t p = begin[sid];
+    const int64_t nvec = (endp[sid] - p) >> 4;   // probe ranges are 16-byte aligned
+    uint32_t x = 0;
+    stream_body<PF>(reinterpret_cast<const uint4*>(bytes + p), nvec,
+                    [&](uint4 v) { x ^= v.x ^ v.y ^ v.z ^ v.w; });
+    sink[sid] = x;
+}
+
+extern "C" TREX_STRING_KERNEL
+read_probe_strings_pf8(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
+                       const int64_t* __restrict__ end, int64_t n_strings, uint32_t* __restrict__ sink)
+{
+    read_probe_strings_body<8>(bytes, begin, end, n_strings, sink);
 }
 
 extern "C" __global__ void __launch_bounds__(TREX_BLOCK)
-dfa_k1(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ offsets, int64_t n_strings,
-       const uint8_t* __restrict__ t1g, const uint8_t* __restrict__ accg, uint32_t stride, uint32_t start,
-       uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
+read_probe_strings_pf16(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
+                        const int64_t* __restrict__ end, int64_t n_strings, uint32_t* __restrict__ sink)
 {
-    dfa_k1_body<-1>(bytes, offsets, n_strings, t1g, accg, stride, start, out_state, out_accept);
+    read_probe_strings_body<16>(bytes, begin, end, n_strings, sink);
+}
+
+// ---------------------------------------------------------------------------
+// Plain 16-byte copy, the reference point for HBM traffic counters.
+// n16 = number of 16-byte vectors.
+// ---------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(TREX_BLOCK)
+copy_u4(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16)
+{
+    const int64_t stride = (int64_t)gridDim.x * TREX_BLOCK;
+    for (int64_t i = (int64_t)blockIdx.x * TREX_BLOCK + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
 }

@@ -21,7 +21,14 @@ from trex_amd import _lib
 from trex_amd.device import DeviceBuffer, Stream
 from trex_amd.tables import DfaByteTable
 
-VARIANTS = {"auto": 0, "k1_generic": 1, "k1_16": 2, "k2_window": 3}
+VARIANTS = {"auto": 0, "k1_generic": 1, "k1_16": 2, "k2_window": 3, "k2_window_f": 5}
+
+# Staggered HBM layout: strings are kept in groups of one wave (64 lanes); group g
+# is moved STAGGER_GAP * g bytes further than in the packed CSR layout, so that
+# waves streaming strings of equal length start at different offsets modulo the
+# HBM channel interleave instead of all hitting the same channels.
+STAGGER_GROUP = 64
+STAGGER_GAP = 256
 VARIANT_NAMES = {v: k for k, v in VARIANTS.items()}
 
 
@@ -68,12 +75,21 @@ class DfaEngine:
     def set_variant(self, name: str) -> None:
         _lib.call("trex_dfa_set_variant", self.handle, VARIANTS[name])
 
+    def set_prefetch(self, vectors: int) -> None:
+        _lib.call("trex_dfa_set_prefetch", self.handle, int(vectors))
+
     # -- matching ------------------------------------------------------------
     def match_device(self, d_bytes: int, d_offsets: int, n_strings: int, d_final: int, d_accept: int,
                      stream: Stream | None = None) -> None:
-        """Asynchronous dispatch on device pointers (inputs resident in HBM)."""
+        """Asynchronous dispatch on a device CSR batch (inputs resident in HBM)."""
         _lib.call("trex_dfa_match", self.handle, d_bytes, d_offsets, int(n_strings), d_final, d_accept,
                   stream.handle if stream else None)
+
+    def match_batch_device(self, batch: "DeviceBatch", d_final: int, d_accept: int,
+                           stream: Stream | None = None) -> None:
+        """Asynchronous dispatch on a :class:`DeviceBatch` (any layout)."""
+        _lib.call("trex_dfa_match_ranges", self.handle, batch.d_bytes.ptr, batch.d_begin.ptr, batch.d_end.ptr,
+                  batch.n_strings, d_final, d_accept, stream.handle if stream else None)
 
     def match_arrays(self, data: np.ndarray, offsets: np.ndarray):
         """Host CSR batch -> (final_state uint8[n], accept uint8[n], kernel_ms, total_ms)."""
@@ -102,6 +118,58 @@ class DfaEngine:
             self.close()
         except Exception:
             pass
+
+
+def staggered_ranges(offsets: np.ndarray) -> tuple[np.ndarray, np.ndarray, int]:
+    """begin/end of every string in the staggered layout, and its total size."""
+    offsets = np.asarray(offsets, dtype=np.int64)
+    n = len(offsets) - 1
+    shift = (np.arange(n, dtype=np.int64) // STAGGER_GROUP) * STAGGER_GAP - offsets[0]
+    begin = offsets[:-1] + shift
+    end = offsets[1:] + shift
+    n_groups = -(-n // STAGGER_GROUP)
+    size = int(offsets[-1] - offsets[0]) + n_groups * STAGGER_GAP
+    return begin, end, size
+
+
+class DeviceBatch:
+    """A string batch resident in HBM: bytes plus per-string [begin, end) ranges.
+
+    ``layout="csr"`` keeps the packed CSR bytes (begin = offsets[:-1],
+    end = offsets[1:]); ``layout="staggered"`` moves each group of 64 strings
+    256 bytes further than the previous one (at most 0.4% more memory for
+    4 KiB strings), see STAGGER_GAP.
+    """
+
+    def __init__(self, data: np.ndarray, offsets: np.ndarray, layout: str = "staggered"):
+        data, offsets = _as_csr(data=data, offsets=offsets)
+        self.n_strings = len(offsets) - 1
+        self.layout = layout
+        self.payload_bytes = int(offsets[-1] - offsets[0])
+        base = int(offsets[0])
+        if layout == "csr":
+            begin = offsets[:-1] - base
+            end = offsets[1:] - base
+            self.d_bytes = DeviceBuffer(max(16, -(-self.payload_bytes // 16) * 16))
+            self.d_bytes.upload(data[base:base + self.payload_bytes])
+        elif layout == "staggered":
+            begin, end, size = staggered_ranges(offsets)
+            self.d_bytes = DeviceBuffer(max(16, -(-size // 16) * 16))
+            for g0 in range(0, self.n_strings, STAGGER_GROUP):
+                g1 = min(g0 + STAGGER_GROUP, self.n_strings)
+                lo, hi = int(offsets[g0]), int(offsets[g1])
+                if hi > lo:
+                    self.d_bytes.upload(data[lo:hi], offset=int(begin[g0]))
+        else:
+            raise ValueError(f"unknown layout {layout!r}")
+        self.begin = np.ascontiguousarray(begin, dtype=np.int64)
+        self.end = np.ascontiguousarray(end, dtype=np.int64)
+        self.d_begin = DeviceBuffer.from_array(self.begin)
+        self.d_end = DeviceBuffer.from_array(self.end)
+
+    def free(self) -> None:
+        for b in (self.d_bytes, self.d_begin, self.d_end):
+            b.free()
 
 
 def upload_batch(data: np.ndarray, offsets: np.ndarray) -> tuple[DeviceBuffer, DeviceBuffer]:
