@@ -119,6 +119,74 @@ int trex_dfa_match_ranges(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_
 int trex_dfa_match_host(trex_dfa* h, const uint8_t* bytes, const int64_t* offsets, int64_t n_strings,
                         uint8_t* final_state, uint8_t* accept, float* kernel_ms, float* total_ms);
 
+/* ------------------------------------------------------------------------
+ * Boolean NFA-64, bit-parallel (hot path (c)).
+ * Replaces nfa_tc_engine_init/dispatch with the V4 bit-parallel kernel
+ * (cuda/nfa_tc_evolution.cu:683-745, 943-1013), which handles two symbols
+ * only; this engine handles up to 62 symbols with nibble-sliced successor
+ * tables in LDS.
+ * ------------------------------------------------------------------------ */
+typedef struct trex_nfa trex_nfa;
+
+#define TREX_NFA_BYTES 1   /* any byte -> class map */
+#define TREX_NFA_WINDOW 2  /* every byte outside one aligned 32-byte window rejects */
+
+/* succ: [n_symbols][64] uint64, bit d of succ[c][s] = edge s -> d on symbol c
+ * (epsilon closure already folded in, as in NFAMatrices). byte_to_sym[256]:
+ * symbol index, -1 = reject (the state set becomes empty), -2 = skip (the byte
+ * leaves the set unchanged). start / accept: state masks. n_symbols <= 62. */
+int trex_nfa_create(const uint64_t* succ, int n_symbols, const int16_t* byte_to_sym, uint64_t start,
+                    uint64_t accept, trex_nfa** out);
+int trex_nfa_destroy(trex_nfa* h);
+int trex_nfa_info(const trex_nfa* h, int* variant, int* window_base);
+int trex_nfa_set_variant(trex_nfa* h, int variant);
+/* Final state mask and accept bit ((mask & accept) != 0) per string. */
+int trex_nfa_match(trex_nfa* h, const uint8_t* d_bytes, const int64_t* d_offsets, int64_t n_strings,
+                   uint64_t* d_final_mask, uint8_t* d_accept, void* stream);
+int trex_nfa_match_ranges(trex_nfa* h, const uint8_t* d_bytes, const int64_t* d_begin, const int64_t* d_end,
+                          int64_t n_strings, uint64_t* d_final_mask, uint8_t* d_accept, void* stream);
+int trex_nfa_match_host(trex_nfa* h, const uint8_t* bytes, const int64_t* offsets, int64_t n_strings,
+                        uint64_t* final_mask, uint8_t* accept, float* kernel_ms, float* total_ms);
+
+/* cuda/nfa_tc_evolution.cu:943-953. T_matrices: [sigma][n][n] row-major
+ * [dst][src], entries > 0.5 are edges; n <= 64. Symbols >= sigma in the
+ * dispatched input are skipped, as in the reference's simulate_nfa_sequential
+ * (:1029-1054). */
+int nfa_tc_engine_init(const float* T_matrices, const float* accept_mask, const float* start_vec, int n, int sigma,
+                       int max_B, int max_L);
+/* cuda/nfa_tc_evolution.cu:955-1009. raw_concat holds symbol indices. */
+int nfa_tc_engine_dispatch(const uint8_t* raw_concat, const int* offsets, int* results, int B, int total_chars,
+                           float* kernel_ms, float* total_ms);
+/* cuda/nfa_tc_evolution.cu:1011-1013 */
+void nfa_tc_engine_destroy(void);
+
+/* ------------------------------------------------------------------------
+ * Long strings: chunk plans and the decoupled look-back scan (hot path (b)).
+ * Replaces monoid_engine_dispatch_batch / dispatch_adaptive with the
+ * monoid_r3_kernel look-back (cuda/monoid_scan.cu:369-451, 468-504) and
+ * monoid_batch_engine_dispatch_prefix (cuda/monoid_batch.cu:660-671): one
+ * launch covers a whole batch, every string split into chunks of chunk_len
+ * bytes whose results are chained by a decoupled look-back over DFA functions.
+ * ------------------------------------------------------------------------ */
+typedef struct trex_plan trex_plan;
+
+/* begin/end: host int64 byte ranges of n_strings strings (in the device byte
+ * buffer the plan will be used with). chunk_len: multiple of 16, >= 64. */
+int trex_plan_create(const int64_t* begin, const int64_t* end, int64_t n_strings, int64_t chunk_len,
+                     trex_plan** out);
+int trex_plan_destroy(trex_plan* plan);
+int trex_plan_info(const trex_plan* plan, int64_t* n_chunks, int64_t* n_strings);
+/* Final state and accept bit of every string of the plan (DFA of <= 16
+ * states). A plan holds the look-back state of one launch: launches that use
+ * the same plan must be ordered on one stream. max_spins bounds every wait on
+ * a predecessor chunk (0 = default); on expiry the launch still ends and
+ * trex_plan_error reports it. */
+int trex_dfa_match_plan(trex_dfa* h, trex_plan* plan, const uint8_t* d_bytes, uint8_t* d_final_state,
+                        uint8_t* d_accept, uint32_t max_spins, void* stream);
+/* Error word of the plan's last launch (0 = none; bit 0 = a wait expired),
+ * read synchronously after the work queued on `stream`. */
+int trex_plan_error(trex_plan* plan, uint32_t* error, void* stream);
+
 /* Plain device copy with the engine's streaming-load kernel (16-byte aligned
  * buffers, size a multiple of 16): the reference point for HBM traffic
  * counters and the achievable-bandwidth probe. No reference counterpart. */

@@ -307,6 +307,19 @@ def test_c_oracle_nfa_matches_python_on_random_nfa():
         assert bool(acc) == simulate_nfa(nm, b.decode("latin-1"))
 
 
+def test_successor_masks_always_64_columns():
+    """The C ABI reads [n_symbols][64]; small NFAs are padded, not truncated."""
+    nm = compile_nfa_matrices("(a|b)*abb")
+    assert nm.n_states < 64
+    succ = nm.successor_masks()
+    assert succ.shape == (len(nm.alphabet), 64) and succ.dtype == np.uint64
+    assert not succ[:, nm.n_states:].any()
+    for c in range(len(nm.alphabet)):
+        for src in range(nm.n_states):
+            for dst in range(nm.n_states):
+                assert bool(int(succ[c, src]) >> dst & 1) == bool(nm.matrix_stack[c, dst, src])
+
+
 def test_mutation_nfa_successor_is_caught():
     e = GOLD["nfa_tables"][0]
     nm = _golden_nfa(e)

@@ -1,0 +1,120 @@
+"""Look-back scan: the oracle pinned on the CPU, then the chunked GPU kernel
+against the oracle (bit-exact final state and accept bit)."""
+
+import numpy as np
+import pytest
+
+from oracle import cpu as ocpu
+from oracle.dfa import table_walk_one
+from oracle.scan import lookback_final_state
+from trex_amd.regex_to_dfa import compile_regex
+from trex_amd.tables import dfa_byte_table
+from trex_amd.workloads import ALPHABET_32, random_dfa_pattern, ragged_batch
+
+EVEN_A = "(b*ab*ab*)*b*"
+
+
+# ---------------------------------------------------------------------------
+# oracle pins (CPU)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("tile", [1, 3, 16, 256])
+def test_lookback_oracle_equals_sequential_walk(tile):
+    """Associativity: any tiling and any publication order gives the walk's state."""
+    rng = np.random.default_rng(tile)
+    _, dfa = random_dfa_pattern(rng, "abcd")
+    tab = dfa_byte_table(dfa)
+    for _ in range(20):
+        n = int(rng.integers(1, 700))
+        data = rng.integers(ord("a"), ord("e"), n, dtype=np.uint8)
+        want = table_walk_one(tab.delta, tab.accept, tab.start, bytes(data))[0]
+        n_tiles = -(-n // tile)
+        for order in (None, list(range(n_tiles)), list(rng.permutation(n_tiles))):
+            assert lookback_final_state(tab.delta, tab.start, data, 0, n, tile, order) == want
+
+
+def test_lookback_oracle_even_a_closed_form():
+    """Even number of a's (cuda/monoid_scan.cu:544-578): parity of the count."""
+    tab = dfa_byte_table(compile_regex(EVEN_A))
+    rng = np.random.default_rng(4)
+    for n in (100, 255, 256, 257, 1000, 5000):
+        data = rng.choice(np.frombuffer(b"ab", np.uint8), n)
+        s = lookback_final_state(tab.delta, tab.start, data, 0, n, 256)
+        assert bool(tab.accept[s]) == (int(np.count_nonzero(data == ord("a"))) % 2 == 0)
+
+
+def test_lookback_oracle_mutation_caught():
+    """Composing aggregates in the wrong order (a planted mistake) is caught."""
+    tab = dfa_byte_table(compile_regex("(a|b)*abb"))
+    rng = np.random.default_rng(8)
+
+    def wrong(delta, start, data, tile):
+        n = len(data)
+        tiles = [(k, min(n, k + tile)) for k in range(0, n, tile)]
+        f = np.arange(delta.shape[0])
+        for a, b in tiles:
+            g = np.arange(delta.shape[0])
+            for x in data[a:b]:
+                g = delta[g, int(x)]
+            f = f[g]                     # applies the new tile first: wrong
+        return int(f[start])
+
+    bad = 0
+    for _ in range(50):
+        data = rng.choice(np.frombuffer(b"ab", np.uint8), int(rng.integers(10, 60)))
+        want = table_walk_one(tab.delta, tab.accept, tab.start, bytes(data))[0]
+        assert lookback_final_state(tab.delta, tab.start, data, 0, len(data), 4) == want
+        bad += wrong(tab.delta, tab.start, data, 4) != want
+    assert bad > 0
+
+
+# ---------------------------------------------------------------------------
+# GPU
+# ---------------------------------------------------------------------------
+def _run_plan(tab, data, offsets, chunk_len, layout="csr"):
+    from trex_amd.device import DeviceBuffer
+    from trex_amd.engine import ChunkPlan, DeviceBatch, DfaEngine
+    eng = DfaEngine(tab)
+    batch = DeviceBatch(data, offsets, layout=layout)
+    plan = ChunkPlan.for_batch(batch, chunk_len)
+    n = len(offsets) - 1
+    d_f, d_a = DeviceBuffer(max(n, 1)), DeviceBuffer(max(n, 1))
+    eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr)
+    assert plan.error() == 0
+    return d_f.download(np.uint8, n), d_a.download(np.uint8, n), plan.n_chunks
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pattern,chunk_len", [(EVEN_A, 64), (EVEN_A, 1024), ("(a|b)*abb", 256),
+                                               ("random32", 128), ("random32", 4096)])
+def test_plan_matches_oracle(gpu_lib, pattern, chunk_len):
+    rng = np.random.default_rng(chunk_len)
+    if pattern == "random32":
+        _, dfa = random_dfa_pattern(rng, ALPHABET_32)
+        lo, hi = 0x40, 0x60
+    else:
+        dfa = compile_regex(pattern)
+        lo, hi = ord("a"), ord("c")
+    tab = dfa_byte_table(dfa)
+    lengths = np.concatenate([rng.integers(0, 300, 400), rng.integers(1000, 20000, 60),
+                              [0, 1, 15, 16, 17, chunk_len - 1, chunk_len, chunk_len + 1, 200_000]])
+    rng.shuffle(lengths)
+    data, offs = ragged_batch(rng, lengths, lo, hi)
+    data[rng.integers(0, len(data), 10)] = 0x7e
+    f, a, n_chunks = _run_plan(tab, data, offs, chunk_len, layout="staggered" if chunk_len > 1000 else "csr")
+    assert n_chunks > len(lengths)
+    rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offs)
+    bad = np.nonzero((f != rf) | (a != ra))[0]
+    assert bad.size == 0, f"{bad.size} mismatches at {bad[:5]}, lengths {lengths[bad[:5]]}"
+
+
+@pytest.mark.gpu
+def test_plan_even_a_long_strings_parity(gpu_lib):
+    """Long strings (1 MiB) across many chunks: accept = even number of a's."""
+    tab = dfa_byte_table(compile_regex(EVEN_A))
+    rng = np.random.default_rng(99)
+    lengths = np.array([1 << 20, (1 << 20) + 3, 777_777, 5], np.int64)
+    data, offs = ragged_batch(rng, lengths, ord("a"), ord("c"))
+    f, a, n_chunks = _run_plan(tab, data, offs, 4096)
+    for i in range(len(lengths)):
+        seg = data[offs[i]:offs[i + 1]]
+        assert bool(a[i]) == (int(np.count_nonzero(seg == ord("a"))) % 2 == 0)

@@ -55,6 +55,24 @@ _SIGNATURES = {
     "trex_dfa_set_prefetch": (c_int, [c_void_p, c_int]),
     "trex_dfa_match_ranges": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
                                       c_void_p]),
+    # NFA
+    "trex_nfa_create": (c_int, [c_void_p, c_int, c_void_p, c_uint64, c_uint64, vpp]),
+    "trex_nfa_destroy": (c_int, [c_void_p]),
+    "trex_nfa_info": (c_int, [c_void_p, intp, intp]),
+    "trex_nfa_set_variant": (c_int, [c_void_p, c_int]),
+    "trex_nfa_match": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    "trex_nfa_match_ranges": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
+                                      c_void_p]),
+    "trex_nfa_match_host": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, f32p, f32p]),
+    "nfa_tc_engine_init": (c_int, [f32p, f32p, f32p, c_int, c_int, c_int, c_int]),
+    "nfa_tc_engine_dispatch": (c_int, [u8p, intp, intp, c_int, c_int, f32p, f32p]),
+    "nfa_tc_engine_destroy": (None, []),
+    # chunk plans
+    "trex_plan_create": (c_int, [c_void_p, c_void_p, c_int64, c_int64, vpp]),
+    "trex_plan_destroy": (c_int, [c_void_p]),
+    "trex_plan_info": (c_int, [c_void_p, i64p, i64p]),
+    "trex_dfa_match_plan": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_uint32, c_void_p]),
+    "trex_plan_error": (c_int, [c_void_p, POINTER(ctypes.c_uint32), c_void_p]),
     "trex_copy": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "trex_read_probe": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p]),
     "trex_dfa_match": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),

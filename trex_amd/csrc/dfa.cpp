@@ -14,26 +14,6 @@
 
 using namespace trex;
 
-struct trex_dfa {
-    int n_states = 0;
-    int stride = 16;          // row stride of t1 (states, rounded up to 16)
-    int start = 0;
-    int dead = -1;            // absorbing non-accepting state, -1 if none
-    int window_base = -1;     // aligned 32-byte window holding every live byte, -1 if none
-    int auto_variant = TREX_DFA_K1_GENERIC;
-    int variant = TREX_DFA_K1_GENERIC;
-    uint8_t* d_t1 = nullptr;  // [256][stride]
-    uint8_t* d_acc = nullptr; // [stride]
-    uint8_t* d_t2 = nullptr;  // [1024][16] pair table t2[pair * 16 + s] (window variants only)
-    uint8_t* d_f2 = nullptr;  // [1024] uint64 pair functions, nibble s = successor of s
-    int prefetch = 16;        // 16-byte vectors in flight per lane (8 or 16)
-    // staging for trex_dfa_match_host
-    DevBuf in_bytes, in_offsets, out_state, out_accept;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    std::mutex mu;            // serialises host dispatches on this handle
-};
-
 namespace {
 
 // Finds the absorbing non-accepting state, or -1.

@@ -1,0 +1,147 @@
+// Bit-parallel Boolean NFA-64 kernels for gfx950 (wave64).
+//
+// One lane walks one string. The state is a 64-bit mask (bit j = NFA state j
+// active); one step is the union of the successor masks of the active states,
+// the bit-parallel step of the reference's V4 kernel
+// (cuda/nfa_tc_evolution.cu:683-745), which tests 64 rows per step and only
+// handles two symbols. Here any symbol alphabet is handled with
+// nibble-sliced successor tables in LDS:
+//
+//     N[c][p][v] = union of succ[c][4p + i] over the bits i of v   (uint64)
+//
+// so a step is 16 independent LDS lookups, one per nibble p of the state:
+//     s' = OR_p N[c][p][(s >> 4p) & 15]
+// The lookups depend on the state only through their index; all 16 are
+// issued together.
+//
+// Byte -> class: `cls[byte]` names the table of that byte. Tables are built
+// on the host for every symbol plus two extra classes, "reject" (all zero:
+// the empty set, absorbing) and "skip" (the identity). In the window kernel,
+// every byte outside one aligned 32-byte window [base, base+32) rejects and
+// every byte inside it is its own class (byte & 31); a byte outside the
+// window is only recorded (two bit operations per word) and forces the
+// empty set at the end, exactly as if it had been stepped.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define TREX_NFA_BLOCK 512
+
+namespace {
+
+struct WinCheck {
+    uint32_t any_hi, all;
+    __device__ __forceinline__ void init() { any_hi = 0u; all = 0xffffffffu; }
+    __device__ __forceinline__ void add(uint32_t x) { any_hi |= x & 0xe0e0e0e0u; all &= x; }
+    __device__ __forceinline__ bool outside(uint32_t win4) const { return ((any_hi & ~win4) | (win4 & ~all)) != 0u; }
+};
+
+// One step through class table `tab` (256 uint64 entries: [p][v]).
+__device__ __forceinline__ uint64_t nfa_step(const uint64_t* __restrict__ tab, uint64_t s) {
+    const uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32);
+    uint64_t r0 = 0, r1 = 0;
+#pragma unroll
+    for (int p = 0; p < 8; p += 2) {
+        r0 |= tab[p * 16 + ((lo >> (4 * p)) & 15u)];
+        r1 |= tab[(p + 1) * 16 + ((lo >> (4 * p + 4)) & 15u)];
+    }
+#pragma unroll
+    for (int p = 0; p < 8; p += 2) {
+        r0 |= tab[(8 + p) * 16 + ((hi >> (4 * p)) & 15u)];
+        r1 |= tab[(9 + p) * 16 + ((hi >> (4 * p + 4)) & 15u)];
+    }
+    return r0 | r1;
+}
+
+template <bool WINDOW>
+__device__ __forceinline__ void nfa_kernel(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
+                                           const int64_t* __restrict__ endp, int64_t n_strings,
+                                           const uint64_t* __restrict__ tabg, int n_classes,
+                                           const uint8_t* __restrict__ clsg, uint32_t win4, uint64_t start,
+                                           uint64_t accept, uint64_t* __restrict__ out_mask,
+                                           uint8_t* __restrict__ out_accept)
+{
+    extern __shared__ __attribute__((aligned(16))) uint64_t lds64[];
+    uint64_t* tab = lds64;                                        // [n_classes][256]
+    uint8_t* cls = reinterpret_cast<uint8_t*>(lds64 + (size_t)n_classes * 256);   // [256]
+    {
+        const uint4* s4 = reinterpret_cast<const uint4*>(tabg);
+        uint4* d4 = reinterpret_cast<uint4*>(tab);
+        for (int i = threadIdx.x; i < n_classes * 128; i += blockDim.x) d4[i] = s4[i];
+        if (threadIdx.x < 256) cls[threadIdx.x] = clsg[threadIdx.x];
+    }
+    __syncthreads();
+
+    const int64_t sid = (int64_t)blockIdx.x * TREX_NFA_BLOCK + threadIdx.x;
+    if (sid >= n_strings) return;
+    int64_t p = begin[sid];
+    const int64_t end = endp[sid];
+    uint64_t s = start;
+
+    const int64_t head_end = min(end, p + (int64_t)((16u - ((uint32_t)(uintptr_t)(bytes + p) & 15u)) & 15u));
+    for (; p < head_end; ++p) s = nfa_step(tab + (uint32_t)cls[bytes[p]] * 256u, s);
+
+    WinCheck w;
+    w.init();
+    const int64_t nvec = (end - p) >> 4;
+    const uint4* v = reinterpret_cast<const uint4*>(bytes + p);
+    auto word = [&](uint32_t x) {
+        if constexpr (WINDOW) {
+            w.add(x);
+            s = nfa_step(tab + (x & 31u) * 256u, s);
+            s = nfa_step(tab + ((x >> 8) & 31u) * 256u, s);
+            s = nfa_step(tab + ((x >> 16) & 31u) * 256u, s);
+            s = nfa_step(tab + ((x >> 24) & 31u) * 256u, s);
+        } else {
+            s = nfa_step(tab + (uint32_t)cls[x & 255u] * 256u, s);
+            s = nfa_step(tab + (uint32_t)cls[(x >> 8) & 255u] * 256u, s);
+            s = nfa_step(tab + (uint32_t)cls[(x >> 16) & 255u] * 256u, s);
+            s = nfa_step(tab + (uint32_t)cls[x >> 24] * 256u, s);
+        }
+    };
+    int64_t i = 0;
+    if (nvec >= 2) {
+        uint4 a = v[0];
+        for (i = 1; i < nvec; ++i) {
+            const uint4 b = v[i];
+            word(a.x); word(a.y); word(a.z); word(a.w);
+            a = b;
+        }
+        word(a.x); word(a.y); word(a.z); word(a.w);
+    } else if (nvec == 1) {
+        const uint4 a = v[0];
+        word(a.x); word(a.y); word(a.z); word(a.w);
+        i = 1;
+    }
+    p += nvec << 4;
+    for (; p < end; ++p) s = nfa_step(tab + (uint32_t)cls[bytes[p]] * 256u, s);
+    if constexpr (WINDOW) {
+        if (w.outside(win4)) s = 0;
+    }
+    out_mask[sid] = s;
+    out_accept[sid] = (s & accept) != 0 ? 1 : 0;
+}
+
+}  // namespace
+
+// tabg: [n_classes][16][16] uint64 nibble tables; clsg: [256] class of every
+// byte. Window kernel: classes 0..31 are the window bytes base + k (n_classes
+// >= 32); the byte map is used for the unaligned head and tail bytes only.
+// Dynamic LDS: n_classes * 2048 + 256 bytes.
+extern "C" __global__ void __launch_bounds__(TREX_NFA_BLOCK)
+nfa64_window(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin, const int64_t* __restrict__ end,
+             int64_t n_strings, const uint64_t* __restrict__ tabg, int n_classes, const uint8_t* __restrict__ clsg,
+             uint32_t win4, uint64_t start, uint64_t accept, uint64_t* __restrict__ out_mask,
+             uint8_t* __restrict__ out_accept)
+{
+    nfa_kernel<true>(bytes, begin, end, n_strings, tabg, n_classes, clsg, win4, start, accept, out_mask, out_accept);
+}
+
+extern "C" __global__ void __launch_bounds__(TREX_NFA_BLOCK)
+nfa64_bytes(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin, const int64_t* __restrict__ end,
+            int64_t n_strings, const uint64_t* __restrict__ tabg, int n_classes, const uint8_t* __restrict__ clsg,
+            uint32_t win4, uint64_t start, uint64_t accept, uint64_t* __restrict__ out_mask,
+            uint8_t* __restrict__ out_accept)
+{
+    nfa_kernel<false>(bytes, begin, end, n_strings, tabg, n_classes, clsg, win4, start, accept, out_mask, out_accept);
+}

@@ -149,6 +149,16 @@ void DevBuf::release() {
     cap = 0;
 }
 
+int device_cu_count() {
+    static int cached = 0;
+    if (cached > 0) return cached;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    cached = cus;
+    return cached;
+}
+
 int upload(void** dptr, const void* src, size_t n) {
     hipError_t e = hipMalloc(dptr, std::max<size_t>(n, 16));
     if (e != hipSuccess) { *dptr = nullptr; return hip_fail(e, "hipMalloc"); }

@@ -91,6 +91,13 @@ class DfaEngine:
         _lib.call("trex_dfa_match_ranges", self.handle, batch.d_bytes.ptr, batch.d_begin.ptr, batch.d_end.ptr,
                   batch.n_strings, d_final, d_accept, stream.handle if stream else None)
 
+    def match_plan(self, plan: "ChunkPlan", d_bytes: int, d_final: int, d_accept: int,
+                   stream: Stream | None = None, max_spins: int = 0) -> None:
+        """Asynchronous chunked dispatch with the decoupled look-back
+        (strings of any length; <= 16-state tables)."""
+        _lib.call("trex_dfa_match_plan", self.handle, plan.handle, d_bytes, d_final, d_accept, int(max_spins),
+                  stream.handle if stream else None)
+
     def match_arrays(self, data: np.ndarray, offsets: np.ndarray):
         """Host CSR batch -> (final_state uint8[n], accept uint8[n], kernel_ms, total_ms)."""
         data, offsets = _as_csr(data=data, offsets=offsets)
@@ -170,6 +177,124 @@ class DeviceBatch:
     def free(self) -> None:
         for b in (self.d_bytes, self.d_begin, self.d_end):
             b.free()
+
+
+NFA_VARIANTS = {"bytes": 1, "window": 2}
+NFA_VARIANT_NAMES = {v: k for k, v in NFA_VARIANTS.items()}
+
+
+def nfa_byte_map(nm, unknown: str = "reject") -> np.ndarray:
+    """byte -> symbol index of an :class:`NFAMatrices`; bytes outside the
+    alphabet reject (``simulate_nfa``, src/nfa_matrices.py:149-152) or, with
+    ``unknown="skip"``, leave the state set unchanged."""
+    fill = -1 if unknown == "reject" else -2
+    b2s = np.full(256, fill, dtype=np.int16)
+    for i, ch in enumerate(nm.alphabet):
+        if ord(ch) < 256:
+            b2s[ord(ch)] = i
+    return b2s
+
+
+class NfaEngine:
+    """One resident Boolean NFA of at most 64 states (``trex_nfa_create``)."""
+
+    def __init__(self, succ: np.ndarray, byte_to_sym: np.ndarray, start: int, accept: int):
+        self.succ = np.ascontiguousarray(succ, dtype=np.uint64)
+        self.byte_to_sym = np.ascontiguousarray(byte_to_sym, dtype=np.int16)
+        if self.succ.ndim != 2 or self.succ.shape[1] != 64:
+            raise ValueError(f"succ must have shape [n_symbols, 64], got {self.succ.shape}")
+        if self.byte_to_sym.shape != (256,):
+            raise ValueError("byte_to_sym must have 256 entries")
+        self.start = int(start)
+        self.accept = int(accept)
+        h = ctypes.c_void_p()
+        _lib.call("trex_nfa_create", self.succ.ctypes.data if self.succ.size else None, self.succ.shape[0],
+                  self.byte_to_sym.ctypes.data, self.start, self.accept, ctypes.byref(h))
+        self.handle = h.value
+
+    @classmethod
+    def from_matrices(cls, nm, unknown: str = "reject") -> "NfaEngine":
+        return cls(nm.successor_masks(), nfa_byte_map(nm, unknown), nm.start_mask(), nm.accept_bits())
+
+    def info(self) -> dict:
+        v, w = ctypes.c_int(), ctypes.c_int()
+        _lib.call("trex_nfa_info", self.handle, ctypes.byref(v), ctypes.byref(w))
+        return {"variant": NFA_VARIANT_NAMES.get(v.value, v.value), "window_base": w.value}
+
+    def set_variant(self, name: str) -> None:
+        _lib.call("trex_nfa_set_variant", self.handle, NFA_VARIANTS[name])
+
+    def match_batch_device(self, batch: "DeviceBatch", d_mask: int, d_accept: int,
+                           stream: Stream | None = None) -> None:
+        _lib.call("trex_nfa_match_ranges", self.handle, batch.d_bytes.ptr, batch.d_begin.ptr, batch.d_end.ptr,
+                  batch.n_strings, d_mask, d_accept, stream.handle if stream else None)
+
+    def match_arrays(self, data: np.ndarray, offsets: np.ndarray):
+        """Host CSR batch -> (final mask uint64[n], accept uint8[n], kernel_ms, total_ms)."""
+        data, offsets = _as_csr(data=data, offsets=offsets)
+        n = len(offsets) - 1
+        mask = np.empty(n, dtype=np.uint64)
+        acc = np.empty(n, dtype=np.uint8)
+        kms, tms = ctypes.c_float(), ctypes.c_float()
+        if n > 0:
+            _lib.call("trex_nfa_match_host", self.handle, data.ctypes.data if len(data) else None,
+                      offsets.ctypes.data, n, mask.ctypes.data, acc.ctypes.data, ctypes.byref(kms), ctypes.byref(tms))
+        return mask, acc, kms.value, tms.value
+
+    def match_strings(self, strings: Sequence):
+        return self.match_arrays(*pack_strings(strings))
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.lib().trex_nfa_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ChunkPlan:
+    """Strings cut into chunks of ``chunk_len`` bytes for the look-back scan
+    (``trex_plan_create``). ``begin``/``end`` are byte ranges in the device
+    buffer the plan is matched against."""
+
+    def __init__(self, begin: np.ndarray, end: np.ndarray, chunk_len: int = 1 << 16):
+        begin = np.ascontiguousarray(begin, dtype=np.int64)
+        end = np.ascontiguousarray(end, dtype=np.int64)
+        if begin.shape != end.shape:
+            raise ValueError("begin and end must have the same length")
+        h = ctypes.c_void_p()
+        _lib.call("trex_plan_create", begin.ctypes.data, end.ctypes.data, len(begin), int(chunk_len),
+                  ctypes.byref(h))
+        self.handle = h.value
+        self.n_strings = len(begin)
+        self.chunk_len = int(chunk_len)
+        nc, ns = ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("trex_plan_info", self.handle, ctypes.byref(nc), ctypes.byref(ns))
+        self.n_chunks = nc.value
+
+    @classmethod
+    def for_batch(cls, batch: "DeviceBatch", chunk_len: int = 1 << 16) -> "ChunkPlan":
+        return cls(batch.begin, batch.end, chunk_len)
+
+    def error(self, stream: Stream | None = None) -> int:
+        err = ctypes.c_uint32()
+        _lib.call("trex_plan_error", self.handle, ctypes.byref(err), stream.handle if stream else None)
+        return err.value
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.lib().trex_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def upload_batch(data: np.ndarray, offsets: np.ndarray) -> tuple[DeviceBuffer, DeviceBuffer]:

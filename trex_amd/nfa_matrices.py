@@ -73,15 +73,18 @@ class NFAMatrices:
 
     # -- packed forms for the GPU --------------------------------------------
     def successor_masks(self) -> np.ndarray:
-        """``succ[c][src]`` = 64-bit mask of destinations (bit d = state d)."""
+        """``succ[c][src]`` = 64-bit mask of destinations (bit d = state d),
+        shape [n_symbols, 64] (sources beyond the NFA's states have no edges)."""
         if self.n_states_raw > 64:
             raise ValueError(f"NFA has {self.n_states_raw} states; the bit-parallel "
                              "engine holds at most 64")
         n = min(self.n_states, 64)
         stack = self.matrix_stack[:, :n, :n].astype(np.uint64)
         weights = (np.uint64(1) << np.arange(n, dtype=np.uint64))
+        out = np.zeros((stack.shape[0], 64), dtype=np.uint64)
         # succ[c, src] = sum_dst T[c, dst, src] << dst
-        return np.ascontiguousarray((stack * weights[None, :, None]).sum(axis=1, dtype=np.uint64))
+        out[:, :n] = (stack * weights[None, :, None]).sum(axis=1, dtype=np.uint64)
+        return out
 
     def start_mask(self) -> int:
         return _bits_to_int(self.start_vec[:64])

@@ -1,0 +1,244 @@
+"""OptimizedEngine -- the reference's Python matching API on the HIP engine.
+
+Same constructor, configuration vocabulary and public methods as the
+reference (``src/optimized_engine.py:29-425``): ``OptimizedEngine(regex,
+config=None, dfa_state_cap=64, monoid_cap=65536)``, ``config_info``,
+``match_batch``, ``match_batch_timed``, plus ``match`` (the north star's name
+for ``match_batch``) and the array form ``match_arrays``.
+
+Every configuration matches on the GPU through the C ABI; none falls back to
+a CPU simulation, and matching raises if the native library or the device is
+missing. Each configuration keeps the reference behaviour for characters
+outside the pattern's alphabet:
+
+* ``None`` (auto), ``baseline``, ``batched+gpu``, ``kgram+gpu``,
+  ``fp16_tc+gpu``: the DFA (``DFA.simulate``, src/regex_to_dfa.py:224-233):
+  such a character rejects (dead state);
+* ``monoid``, ``monoid+kgram``, ``monoid+gpu``, ``monoid_batch+gpu``: the
+  monoid walk (``simulate_monoid``, src/monoid.py:185-190): identity;
+* ``prefix+gpu``: the tmap walk (``precompute_tmap``, src/simulation.py:283-289):
+  identity;
+* ``nfa``: the NFA (``simulate_nfa``, src/nfa_matrices.py:149-152): rejects.
+
+GPU engines are created on first use, so construction and ``config_info``
+work on a host without a GPU.
+"""
+
+from __future__ import annotations
+
+import time
+from typing import Optional, Sequence
+
+import numpy as np
+
+from trex_amd.kgram import auto_k
+from trex_amd.monoid import compute_monoid, monoid_byte_table
+from trex_amd.nfa_matrices import compile_nfa_matrices
+from trex_amd.regex_to_dfa import compile_regex
+from trex_amd.simulation import DFAMatrices, precompute_tmap
+from trex_amd.tables import MAX_TABLE_STATES, dfa_byte_table, tmap_byte_table
+
+CONFIGS = (None, "monoid", "monoid+kgram", "baseline", "nfa", "monoid+gpu", "batched+gpu", "kgram+gpu",
+           "monoid_batch+gpu", "prefix+gpu", "fp16_tc+gpu")
+
+# strings longer than this are matched through the chunked look-back scan
+LONG_STRING = 1 << 20
+CHUNK_LEN = 1 << 16
+
+
+class OptimizedEngine:
+    def __init__(self, regex: str, config: Optional[str] = None, dfa_state_cap: int = 64,
+                 monoid_cap: int = 65536):
+        if config not in CONFIGS:
+            raise ValueError(f"Unknown config: {config!r}. Choose from None, " +
+                             ", ".join(repr(c) for c in CONFIGS[1:]) + ".")
+        self._regex = regex
+        self._config = config
+        self._dfa_state_cap = dfa_state_cap
+        self._monoid_cap = monoid_cap
+        self._dfa = None
+        self._dm = None
+        self._md = None
+        self._nm = None
+        self._table = None          # DfaByteTable walked on the GPU (DFA, monoid or tmap form)
+        self._engine = None         # DfaEngine / NfaEngine, created on first match
+        self._representation = None
+        self._scan_backend = None
+        self._selection_reason = None
+        self._kgram_k = None
+        {
+            None: self._auto_select,
+            "monoid": lambda: self._select_monoid("monoid", "forced config='monoid'"),
+            "monoid+kgram": self._select_monoid_kgram,
+            "baseline": lambda: self._select_dfa("sequential", "forced config='baseline'"),
+            "nfa": lambda: self._select_nfa("forced config='nfa'"),
+            "monoid+gpu": lambda: self._select_monoid("monoid+gpu", "GPU monoid scan"),
+            "batched+gpu": lambda: self._select_dfa("batched+gpu", "GPU batched DFA walk"),
+            "kgram+gpu": self._select_kgram_gpu,
+            "monoid_batch+gpu": lambda: self._select_monoid("monoid_batch+gpu", None, limit=255),
+            "prefix+gpu": self._select_prefix,
+            "fp16_tc+gpu": lambda: self._select_dfa("fp16_tc+gpu", "GPU DFA walk (fp16_tc+gpu name kept)"),
+        }[config]()
+
+    # -- selection (host only) ------------------------------------------------
+    def _build_dfa(self):
+        if self._dfa is None:
+            self._dfa = compile_regex(self._regex)
+            self._dm = DFAMatrices(self._dfa)
+
+    def _select_dfa(self, backend: str, reason: str):
+        self._build_dfa()
+        self._table = dfa_byte_table(self._dfa)
+        self._representation = "dfa"
+        self._scan_backend = backend
+        self._selection_reason = reason
+
+    def _select_nfa(self, reason: str):
+        self._nm = compile_nfa_matrices(self._regex)
+        if self._nm.n_states_raw > 64:
+            raise RuntimeError(f"NFA has {self._nm.n_states_raw} states; the bit-parallel engine holds 64")
+        self._representation = "nfa"
+        self._scan_backend = "nfa"
+        self._selection_reason = reason
+
+    def _select_monoid(self, backend: str, reason: Optional[str], limit: int = MAX_TABLE_STATES):
+        self._build_dfa()
+        self._md = compute_monoid(self._dm, max_size=self._monoid_cap)
+        if self._md is None:
+            raise RuntimeError("compute_monoid returned None (monoid too large). "
+                               "Increase monoid_cap or use a different config.")
+        if self._md.size > limit:
+            raise RuntimeError(f"Monoid size {self._md.size} > {limit}; use another config")
+        self._table = monoid_byte_table(self._md, self._dm)
+        self._representation = "dfa"
+        self._scan_backend = backend
+        self._selection_reason = reason or (f"GPU monoid batch (M={self._md.size}, "
+                                            f"sigma={len(self._dm.alphabet)})")
+
+    def _select_monoid_kgram(self):
+        self._select_monoid("monoid+kgram", "forced config='monoid+kgram'")
+        self._kgram_k = auto_k(len(self._dfa.alphabet))
+
+    def _select_kgram_gpu(self):
+        self._select_dfa("kgram+gpu", None)
+        self._kgram_k = 2
+        self._selection_reason = f"GPU K-gram DFA walk (two bytes per table step, N={self._table.n_states})"
+
+    def _select_prefix(self):
+        self._build_dfa()
+        dm = self._dm
+        accept = np.zeros(dm.n_states, np.uint8)
+        for s in self._dfa.accept_states:
+            accept[s] = 1
+        self._table = tmap_byte_table(precompute_tmap(dm), accept, self._dfa.start, dm.n_states)
+        self._representation = "dfa"
+        self._scan_backend = "prefix+gpu"
+        self._selection_reason = f"GPU prefix compose (N={dm.n_states})"
+
+    def _auto_select(self):
+        """DFA size decides: a byte table of at most 256 states (dead state
+        included) runs as a DFA, the K-gram kernels when it has at most 16;
+        larger DFAs, or DFAs over ``dfa_state_cap``, run as a bit-parallel
+        NFA (at most 64 NFA states)."""
+        self._build_dfa()
+        n_states = self._dfa.n_states
+        try:
+            table = dfa_byte_table(self._dfa)
+        except ValueError:          # more states than a byte table holds
+            table = None
+        if n_states > self._dfa_state_cap or table is None:
+            self._select_nfa(f"DFA has {n_states} states > cap {self._dfa_state_cap}; using NFA")
+            return
+        self._table = table
+        self._representation = "dfa"
+        if table.n_states <= 16:
+            self._scan_backend = "dfa16+gpu"
+            self._kgram_k = 2
+            self._selection_reason = (f"DFA has {n_states} states ({table.n_states} with its dead state) "
+                                      f"<= 16; auto-selected dfa16+gpu (two bytes per table step)")
+        else:
+            self._scan_backend = "dfa+gpu"
+            self._selection_reason = (f"DFA has {n_states} states ({table.n_states} with its dead state) "
+                                      f"> 16; auto-selected dfa+gpu (byte table walk)")
+
+    # -- public API -------------------------------------------------------------
+    @property
+    def config_info(self) -> dict:
+        info = {"representation": self._representation, "scan_backend": self._scan_backend,
+                "kgram_k": self._kgram_k, "selection_reason": self._selection_reason}
+        if self._dfa is not None:
+            info["alphabet_size"] = len(self._dfa.alphabet)
+            info["dfa_states"] = self._dfa.n_states
+        elif self._nm is not None:
+            info["alphabet_size"] = len(self._nm.alphabet)
+            info["nfa_states"] = self._nm.n_states_raw
+        if self._md is not None:
+            info["monoid_size"] = self._md.size
+        if self._nm is not None and "nfa_states" not in info:
+            info["nfa_states"] = self._nm.n_states_raw
+        return info
+
+    def _gpu(self):
+        if self._engine is None:
+            from trex_amd.engine import DfaEngine, NfaEngine
+            if self._representation == "nfa":
+                self._engine = NfaEngine.from_matrices(self._nm)
+            else:
+                self._engine = DfaEngine(self._table)
+                if self._scan_backend == "sequential":
+                    self._engine.set_variant("k1_generic" if self._table.n_states > 16 else "k1_16")
+        return self._engine
+
+    def match_arrays(self, data: np.ndarray, offsets: np.ndarray):
+        """CSR batch (uint8 bytes, int64 offsets) -> (final state, accept) arrays.
+        Final states are DFA table states, or 64-bit state masks for NFAs."""
+        from trex_amd.engine import DfaEngine, _as_csr
+        data, offsets = _as_csr(data=data, offsets=offsets)
+        eng = self._gpu()
+        lengths = np.diff(offsets)
+        if (isinstance(eng, DfaEngine) and self._table.n_states <= 16 and len(lengths)
+                and int(lengths.max()) > LONG_STRING):
+            return self._match_long(eng, data, offsets)
+        final, acc, _, _ = eng.match_arrays(data, offsets)
+        return final, acc
+
+    def _match_long(self, eng, data, offsets):
+        from trex_amd.device import DeviceBuffer
+        from trex_amd.engine import ChunkPlan, DeviceBatch
+        batch = DeviceBatch(data, offsets, layout="csr")
+        plan = ChunkPlan.for_batch(batch, CHUNK_LEN)
+        n = batch.n_strings
+        d_f, d_a = DeviceBuffer(n), DeviceBuffer(n)
+        eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr)
+        if plan.error():
+            raise RuntimeError("look-back scan reported a wait timeout")
+        return d_f.download(np.uint8, n), d_a.download(np.uint8, n)
+
+    def match_batch(self, strings: Sequence) -> list:
+        """Match a list of strings (``str`` as latin-1, or bytes); returns list[bool]."""
+        if not len(strings):
+            return []
+        from trex_amd.engine import pack_strings
+        _, acc = self.match_arrays(*pack_strings(strings))
+        return [bool(x) for x in acc]
+
+    def match(self, strings: Sequence) -> list:
+        return self.match_batch(strings)
+
+    def match_batch_timed(self, strings: Sequence) -> tuple:
+        """(results, {'kernel_ms', 'total_ms', ...}) like the reference's GPU configs."""
+        from trex_amd.engine import pack_strings
+        if not len(strings):
+            return [], {"kernel_ms": 0.0, "total_ms": 0.0, "n_strings": 0}
+        data, offsets = pack_strings(strings)
+        t0 = time.perf_counter()
+        eng = self._gpu()
+        lengths = np.diff(offsets)
+        from trex_amd.engine import DfaEngine
+        if isinstance(eng, DfaEngine) and self._table.n_states <= 16 and int(lengths.max()) > LONG_STRING:
+            _, acc = self._match_long(eng, data, offsets)
+            kms = tms = (time.perf_counter() - t0) * 1e3
+        else:
+            _, acc, kms, tms = eng.match_arrays(data, offsets)
+        return [bool(x) for x in acc], {"kernel_ms": kms, "total_ms": tms,
+                                        "total_seconds": time.perf_counter() - t0, "n_strings": len(strings)}
