@@ -78,7 +78,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--variant", default="auto")
-    ap.add_argument("--prefetch", type=int, default=16, help="16-byte vectors in flight per lane (8 or 16)")
+    ap.add_argument("--prefetch", type=int, default=8, help="16-byte vectors in flight per lane (8 or 16)")
     ap.add_argument("--layout", default="staggered", choices=["staggered", "csr"])
     ap.add_argument("--sweep", action="store_true", help="time every kernel variant and layout, then exit")
     ap.add_argument("--copy-probe", action="store_true",

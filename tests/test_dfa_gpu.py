@@ -129,7 +129,7 @@ def test_window_variant_out_of_window_bytes(gpu_lib):
     _, dfa = random_dfa_pattern(rng, ALPHABET_32)
     tab = dfa_byte_table(dfa)
     eng = DfaEngine(tab)
-    assert eng.info()["variant"] == "k2_window"
+    assert eng.info()["variant"] == "k2_window_f"
     data, offs = uniform_batch(rng, 512, 300, 0x40, 0x60)
     for i in range(512):
         pos = int(offs[i] + (i * 7) % 300)

@@ -103,7 +103,7 @@ int trex_dfa_create(const uint8_t* delta, const uint8_t* accept, int n_states, i
         if (e != hipSuccess) rc = hip_fail(e, "stream/event create");
     }
     if (rc) { trex_dfa_destroy(h); return rc; }
-    if (h->window_base >= 0) h->auto_variant = TREX_DFA_K2_WINDOW;
+    if (h->window_base >= 0) h->auto_variant = TREX_DFA_K2_WINDOW_F;
     else if (n_states <= 16) h->auto_variant = TREX_DFA_K1_16;
     else h->auto_variant = TREX_DFA_K1_GENERIC;
     h->variant = h->auto_variant;

@@ -52,7 +52,7 @@ struct trex_dfa {
     uint8_t* d_acc = nullptr; // [stride]
     uint8_t* d_t2 = nullptr;  // [1024][16] pair table t2[pair * 16 + s] (window variants only)
     uint8_t* d_f2 = nullptr;  // [1024] uint64 pair functions, nibble s = successor of s
-    int prefetch = 16;        // 16-byte vectors in flight per lane (8 or 16)
+    int prefetch = 8;         // 16-byte vectors in flight per lane (8 or 16)
     // staging for trex_dfa_match_host
     trex::DevBuf in_bytes, in_offsets, out_state, out_accept;
     hipStream_t stream = nullptr;
