@@ -161,6 +161,45 @@ int nfa_tc_engine_dispatch(const uint8_t* raw_concat, const int* offsets, int* r
 void nfa_tc_engine_destroy(void);
 
 /* ------------------------------------------------------------------------
+ * Probabilistic NFA-64 on MFMA.
+ * Replaces the V5 probabilistic evolution nfa_prob_v5_tc_kernel and its
+ * launch through nfa_tc_engine_init/dispatch (cuda/nfa_tc_evolution.cu:451-561,
+ * 880-895), which runs s' = T[c] s in FP16 with an FP16 accumulator, no
+ * rescaling, and returns the bit (a . s_L) > 0.5. Here every sequence gets
+ *   LL      = log(a . T[c_L] ... T[c_1] s_0)
+ *   logmass = log(1 . T[c_L] ... T[c_1] s_0)
+ * in FP64, from FP16/BF16 MFMA products with FP32 accumulation, a power-of-two
+ * rescale per step into an integer log2-scale, and the last step recomputed in
+ * FP32 from the unrounded T. A symbol >= n_symbols empties the state
+ * (LL = -inf).
+ * ------------------------------------------------------------------------ */
+typedef struct trex_pnfa trex_pnfa;
+
+#define TREX_PNFA_F16X2 1   /* FP16 T, state split into two FP16 halves (two MFMAs) */
+#define TREX_PNFA_BF16X2 2  /* BF16 T, state split into two BF16 halves */
+#define TREX_PNFA_F16 3     /* FP16 T and state (one MFMA) */
+#define TREX_PNFA_BF16 4    /* BF16 T and state */
+
+/* T: [n_symbols][64][64] FP32 row-major [dst][src] (the reference's layout);
+ * start, accept: [64]. n_symbols in 1..32. The default mode is F16X2, or
+ * BF16X2 when some |T| exceeds the FP16 range. */
+int trex_pnfa_create(const float* T, int n_symbols, const float* start, const float* accept, trex_pnfa** out);
+int trex_pnfa_destroy(trex_pnfa* h);
+int trex_pnfa_info(const trex_pnfa* h, int* mode, int* n_symbols, int* f16_ok);
+int trex_pnfa_set_mode(trex_pnfa* h, int mode);
+/* The operator a mode applies on every step but the last: the rounded T as FP32
+ * [n_symbols][64][64] and the column mass factors [n_symbols][64], so that
+ * s' = T_rounded (col_scale[c] * s). Either output may be null. */
+int trex_pnfa_rounded(const trex_pnfa* h, int mode, float* t_rounded, float* col_scale);
+/* d_symbols: [n_seq][length] uint8 symbol indices; d_ll, d_logmass: [n_seq]
+ * FP64. Asynchronous on `stream`. length <= 2^31 - 1. */
+int trex_pnfa_loglik(trex_pnfa* h, const uint8_t* d_symbols, int64_t n_seq, int64_t length, double* d_ll,
+                     double* d_logmass, void* stream);
+/* Host-pointer form (synchronous), with kernel and end-to-end times. */
+int trex_pnfa_loglik_host(trex_pnfa* h, const uint8_t* symbols, int64_t n_seq, int64_t length, double* ll,
+                          double* logmass, float* kernel_ms, float* total_ms);
+
+/* ------------------------------------------------------------------------
  * Long strings: chunk plans and the decoupled look-back scan (hot path (b)).
  * Replaces monoid_engine_dispatch_batch / dispatch_adaptive with the
  * monoid_r3_kernel look-back (cuda/monoid_scan.cu:369-451, 468-504) and

@@ -70,3 +70,32 @@ def nfa_walk(succ, byte_to_sym, start, accept, data, offsets, threads: int | Non
     if rc:
         raise ValueError(f"oracle_nfa_walk rc={rc}")
     return final, acc
+
+
+def pnfa_forward(T, start, accept, symbols, col_scale=None, T_last=None, threads: int | None = None):
+    """C restatement of ``oracle.prob.forward_loglik`` (FP64, threaded)."""
+    T = np.asarray(T, dtype=np.float64)
+    sigma = T.shape[0]
+    Tt = np.ascontiguousarray(np.swapaxes(T, 1, 2))
+    Tt_last = Tt if T_last is None else np.ascontiguousarray(np.swapaxes(np.asarray(T_last, np.float64), 1, 2))
+    scale = None if col_scale is None else np.ascontiguousarray(col_scale, dtype=np.float64)
+    s0 = np.asarray(start, dtype=np.float64)
+    m = np.abs(s0).max()
+    e0 = int(np.frexp(m)[1]) if m > 0 else 0
+    s0 = np.ascontiguousarray(np.ldexp(s0, -e0))
+    a = np.ascontiguousarray(accept, dtype=np.float64)
+    sym = np.ascontiguousarray(symbols, dtype=np.uint8)
+    n, L = sym.shape
+    ll = np.empty(n, np.float64)
+    lm = np.empty(n, np.float64)
+    f = lib().oracle_pnfa_forward
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_int]
+    rc = f(Tt.ctypes.data, Tt_last.ctypes.data, scale.ctypes.data if scale is not None else None, sigma,
+           s0.ctypes.data, e0, a.ctypes.data, sym.ctypes.data if sym.size else None, n, L, ll.ctypes.data,
+           lm.ctypes.data, threads or default_threads())
+    if rc:
+        raise ValueError(f"oracle_pnfa_forward rc={rc}")
+    return ll, lm

@@ -15,9 +15,11 @@
  *
  * Strings are split over `threads` pthreads in contiguous ranges.
  */
+#include <math.h>
 #include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
+#include <string.h>
 
 typedef struct {
     const uint8_t* delta;
@@ -122,6 +124,94 @@ int oracle_nfa_walk(const uint64_t* succ, int n_sym, const int16_t* byte_to_sym,
         nfa_job jb = {succ, byte_to_sym, start, accept, bytes, offsets, lo, hi, final_mask, acc};
         jobs[t] = jb;
         if (pthread_create(&tid[t], NULL, nfa_worker, &jobs[t]) != 0) { nfa_worker(&jobs[t]); tid[t] = 0; }
+        launched = t + 1;
+    }
+    for (int t = 0; t < launched; ++t)
+        if (tid[t]) pthread_join(tid[t], NULL);
+    return 0;
+}
+
+/*
+ *  oracle_pnfa_forward: FP64 forward pass of the probabilistic NFA-64, the
+ *    restatement of oracle/prob.py:forward_loglik (s <- T[c] (scale[c] * s)
+ *    per step, the last step with T_last and no scale; exact power-of-two
+ *    renormalisation into an integer exponent; symbol >= sigma empties the
+ *    state). Tt / Tt_last are the matrices TRANSPOSED, [sigma][src][dst], so
+ *    the product is a sum of scaled columns (s'[i] += T[i][j] s[j], j outer).
+ */
+typedef struct {
+    const double* Tt;
+    const double* Tt_last;
+    const double* scale;
+    int sigma;
+    const double* s0;
+    int e0;
+    const double* a;
+    const uint8_t* sym;
+    int64_t L;
+    int64_t lo, hi;
+    double* ll;
+    double* lm;
+} pnfa_job;
+
+static void* pnfa_worker(void* arg) {
+    pnfa_job* j = (pnfa_job*)arg;
+    double s[64], x[64], nx[64];
+    for (int64_t q = j->lo; q < j->hi; ++q) {
+        memcpy(s, j->s0, sizeof(s));
+        long long expo = j->e0;
+        const uint8_t* sy = j->sym + q * j->L;
+        for (int64_t t = 0; t < j->L; ++t) {
+            const int c = sy[t];
+            const int last = t == j->L - 1;
+            if (c >= j->sigma) {
+                memset(s, 0, sizeof(s));
+                continue;
+            }
+            const double* M = (last ? j->Tt_last : j->Tt) + (size_t)c * 4096;
+            for (int k = 0; k < 64; ++k) x[k] = (j->scale && !last) ? s[k] * j->scale[c * 64 + k] : s[k];
+            for (int i = 0; i < 64; ++i) nx[i] = 0.0;
+            for (int k = 0; k < 64; ++k) {
+                const double xk = x[k];
+                const double* col = M + k * 64;
+                for (int i = 0; i < 64; ++i) nx[i] += col[i] * xk;
+            }
+            double m = 0.0;
+            for (int i = 0; i < 64; ++i) m = fmax(m, fabs(nx[i]));
+            int e = 0;
+            if (m > 0.0) frexp(m, &e);
+            for (int i = 0; i < 64; ++i) s[i] = ldexp(nx[i], -e);
+            expo += e;
+        }
+        double da = 0.0, dm = 0.0;
+        for (int i = 0; i < 64; ++i) {
+            da += s[i] * j->a[i];
+            dm += s[i];
+        }
+        const double sc = (double)expo * 0.69314718055994530942;
+        j->ll[q] = (da > 0.0 ? log(da) : (da == 0.0 ? -INFINITY : NAN)) + sc;
+        j->lm[q] = (dm > 0.0 ? log(dm) : (dm == 0.0 ? -INFINITY : NAN)) + sc;
+    }
+    return NULL;
+}
+
+int oracle_pnfa_forward(const double* Tt, const double* Tt_last, const double* scale, int sigma, const double* s0,
+                        int e0, const double* a, const uint8_t* sym, int64_t n_seq, int64_t L, double* ll,
+                        double* lm, int threads) {
+    if (sigma < 1 || n_seq < 0 || L < 0) return -3;
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t tid[256];
+    pnfa_job jobs[256];
+    int64_t per = (n_seq + threads - 1) / threads;
+    int launched = 0;
+    for (int t = 0; t < threads; ++t) {
+        int64_t lo = t * per, hi = lo + per;
+        if (hi > n_seq) hi = n_seq;
+        if (lo >= hi) break;
+        pnfa_job jb = {Tt, Tt_last, scale, sigma, s0, e0, a, sym, L, lo, hi, ll, lm};
+        jobs[t] = jb;
+        if (pthread_create(&tid[t], NULL, pnfa_worker, &jobs[t]) != 0) { pnfa_worker(&jobs[t]); tid[t] = 0; }
         launched = t + 1;
     }
     for (int t = 0; t < launched; ++t)

@@ -51,7 +51,8 @@ def test_kernel_module_built_for_gfx950():
         assert data[:4] == b"\x7fELF", o
         assert b"gfx950" in data, o
         blob += data
-    for kernel in [b"dfa16_k2w_pf16", b"dfa16_k2f_pf16", b"dfa16_k1", b"dfa_k1", b"copy_u4"]:
+    for kernel in [b"dfa16_k2w_pf16", b"dfa16_k2f_pf16", b"dfa16_k1", b"dfa_k1", b"copy_u4", b"pnfa64_f16x2",
+                   b"pnfa64_bf16x2"]:
         assert kernel in blob
 
 

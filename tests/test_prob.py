@@ -1,0 +1,344 @@
+"""Probabilistic NFA-64: FP64 oracle pins (CPU) and the MFMA kernels against
+the oracle (GPU).
+
+The quantity is LL = log(a . T[c_L] ... T[c_1] s_0) and logmass = log(1 . s_L)
+(the reference's V5 kernel only thresholds a . s_L, cuda/nfa_tc_evolution.cu:552-559,
+so the oracle is pinned by closed forms, not by reference vectors).
+
+Tolerances (relative LL error |LL_gpu - LL_fp64| / |LL_fp64|):
+  * f16x2 / bf16x2 (split state) against FP64 on the FP32 T: <= 1e-3 for
+    every sequence of the Dirichlet(0.1) model (the stated precision bound);
+  * f16 / bf16 (one rounding of the state): <= 5e-3 / 5e-2 (measured, not a
+    precision claim);
+  * exact cases (permutation matrices, identity): 1e-13.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import prob as oprob
+from trex_amd.workloads import dirichlet_pnfa
+
+N = 64
+SPLIT_TOL = 1e-3
+SINGLE_TOL = {"f16": 5e-3, "bf16": 5e-2}
+
+
+def _rel(x, ref):
+    return np.abs(x - ref) / np.abs(ref)
+
+
+# ---------------------------------------------------------------------------
+# closed forms (shared by the oracle pins and the mutation checks)
+# ---------------------------------------------------------------------------
+
+def _perm_model(rng, sigma):
+    perms = np.stack([rng.permutation(N) for _ in range(sigma)])        # state j -> perms[c][j]
+    T = np.zeros((sigma, N, N), dtype=np.float32)
+    for c in range(sigma):
+        T[c, perms[c], np.arange(N)] = 1.0
+    start = (rng.permutation(N) + 1).astype(np.float32) / 64.0          # distinct, exact in FP16/BF16
+    accept = (rng.random(N) < 0.3).astype(np.float32)
+    accept[0] = 1.0
+    return T, perms, start, accept
+
+
+def _perm_closed_form(perms, start, accept, symbols):
+    """s_L[pos[j]] = s0[j] with pos the composed permutation of each sequence."""
+    n_seq, L = symbols.shape
+    pos = np.tile(np.arange(N), (n_seq, 1))
+    for t in range(L):
+        pos = perms[symbols[:, t]][np.arange(n_seq)[:, None], pos]
+    s = np.zeros((n_seq, N))
+    s[np.arange(n_seq)[:, None], pos] = start.astype(np.float64)[None, :]
+    return np.log(s @ accept.astype(np.float64)), np.log(s.sum(axis=1))
+
+
+def _rank_one_model(rng, sigma):
+    u = rng.uniform(0.1, 2.0, size=(sigma, N))                           # T[c] = u_c 1^T
+    T = np.repeat(u[:, :, None], N, axis=2).astype(np.float32)
+    return T, u.astype(np.float32).astype(np.float64)
+
+
+def _rank_one_closed_form(u, start, accept, symbols):
+    """s_t = u_{c_t} (1 . s_{t-1}):  LL = log(a . u_{c_L}) + sum_{t<L} log(1 . u_{c_t}) + log(1 . s0)."""
+    L = symbols.shape[1]
+    mass = np.log(u.sum(axis=1))
+    ll = np.log(u[symbols[:, -1]] @ accept.astype(np.float64)) + np.log(np.sum(start, dtype=np.float64))
+    if L > 1:
+        ll = ll + mass[symbols[:, :-1]].sum(axis=1)
+    lm = ll - np.log(u[symbols[:, -1]] @ accept.astype(np.float64)) + np.log(u[symbols[:, -1]].sum(axis=1))
+    return ll, lm
+
+
+def check_permutation(fwd, seed=1):
+    rng = np.random.default_rng(seed)
+    T, perms, start, accept = _perm_model(rng, 5)
+    sy = rng.integers(0, 5, size=(40, 37))
+    ll, lm = fwd(T, start, accept, sy)
+    ref_ll, ref_lm = _perm_closed_form(perms, start, accept, sy)
+    return np.allclose(ll, ref_ll, rtol=1e-12, atol=1e-12) and np.allclose(lm, ref_lm, rtol=1e-12, atol=1e-12)
+
+
+def check_rank_one(fwd, seed=2):
+    rng = np.random.default_rng(seed)
+    T, u = _rank_one_model(rng, 4)
+    start = rng.uniform(0.0, 1.0, N).astype(np.float32)
+    accept = (rng.random(N) < 0.5).astype(np.float32)
+    accept[3] = 1.0
+    sy = rng.integers(0, 4, size=(30, 400))          # products far outside FP64 range without rescaling
+    ll, lm = fwd(T, start, accept, sy)
+    ref_ll, ref_lm = _rank_one_closed_form(u, start, accept, sy)
+    return np.allclose(ll, ref_ll, rtol=1e-11) and np.allclose(lm, ref_lm, rtol=1e-11)
+
+
+def check_scaling(fwd, seed=3):
+    rng = np.random.default_rng(seed)
+    T, start, accept = dirichlet_pnfa(rng, 3)
+    sy = rng.integers(0, 3, size=(10, 300))
+    ll1, _ = fwd(T, start, accept, sy)
+    ll2, _ = fwd(T * np.float32(4.0), start, accept, sy)
+    return np.allclose(ll2 - ll1, 300 * np.log(4.0), rtol=1e-12, atol=1e-9)
+
+
+# ---------------------------------------------------------------------------
+# oracle pins (CPU)
+# ---------------------------------------------------------------------------
+
+def test_oracle_matches_bruteforce_product():
+    rng = np.random.default_rng(0)
+    T, start, accept = dirichlet_pnfa(rng, 3)
+    T = T * rng.uniform(0.5, 1.5, size=T.shape).astype(np.float32)   # not stochastic
+    sy = rng.integers(0, 3, size=(3, 4))
+    ll, lm = oprob.forward_loglik(T, start, accept, sy)
+    for i in range(len(sy)):
+        bl, bm = oprob.loglik_bruteforce(T, start, accept, list(sy[i]))
+        assert ll[i] == pytest.approx(bl, rel=1e-12)
+        assert lm[i] == pytest.approx(bm, rel=1e-12)
+
+
+def test_oracle_identity():
+    rng = np.random.default_rng(5)
+    T = np.tile(np.eye(N, dtype=np.float32), (4, 1, 1))
+    start = rng.random(N).astype(np.float32)
+    accept = (rng.random(N) < 0.4).astype(np.float32)
+    sy = rng.integers(0, 4, size=(6, 50))
+    ll, lm = oprob.forward_loglik(T, start, accept, sy)
+    assert np.allclose(ll, np.log(start.astype(np.float64) @ accept), rtol=1e-14)
+    assert np.allclose(lm, np.log(start.astype(np.float64).sum()), rtol=1e-14)
+
+
+def test_oracle_closed_forms():
+    assert check_permutation(oprob.forward_loglik)
+    assert check_rank_one(oprob.forward_loglik)
+    assert check_scaling(oprob.forward_loglik)
+
+
+def test_oracle_stationary_limit():
+    """One symbol repeated: s_L -> lambda^L c pi with (lambda, pi) the Perron pair
+    of T and c = (w . s0) / (w . pi) for the left Perron vector w, so
+    LL -> log(a . pi) + L log(lambda) + log(c) (lambda = 1 up to the FP32
+    rounding of a column-stochastic T)."""
+    rng = np.random.default_rng(7)
+    T, start, accept = dirichlet_pnfa(rng, 1, alpha=1.0)
+    Td = T[0].astype(np.float64)
+    w, v = np.linalg.eig(Td)
+    k = np.argmax(np.real(w))
+    lam, pi = np.real(w[k]), np.real(v[:, k])
+    pi = pi / pi.sum()
+    wl, vl = np.linalg.eig(Td.T)
+    left = np.real(vl[:, np.argmax(np.real(wl))])
+    c = (left @ start.astype(np.float64)) / (left @ pi)
+    L = 3000
+    ll, lm = oprob.forward_loglik(T, start, accept, np.zeros((1, L), dtype=np.int64))
+    assert ll[0] == pytest.approx(np.log(pi @ accept) + L * np.log(lam) + np.log(c), rel=1e-10)
+    assert lm[0] == pytest.approx(L * np.log(lam) + np.log(c), abs=1e-10)
+    assert abs(lm[0]) < 1e-3
+
+
+def test_oracle_edge_cases():
+    rng = np.random.default_rng(8)
+    T, start, accept = dirichlet_pnfa(rng, 2)
+    ll, lm = oprob.forward_loglik(T, start, accept, np.zeros((2, 0), dtype=np.int64))
+    assert np.allclose(ll, np.log(start.astype(np.float64) @ accept))
+    ll, lm = oprob.forward_loglik(T, start, accept, np.array([[0, 1, 2, 0], [0, 1, 1, 0]]))
+    assert ll[0] == -np.inf and lm[0] == -np.inf and np.isfinite(ll[1])
+
+
+def test_c_oracle_matches_numpy_oracle():
+    from oracle import cpu as ocpu
+    rng = np.random.default_rng(9)
+    T, start, accept = dirichlet_pnfa(rng, 5)
+    start = rng.uniform(0, 3e-5, N).astype(np.float32)
+    sy = rng.integers(0, 5, size=(37, 300)).astype(np.uint8)
+    sy[3, 100] = 5
+    sy[4, 299] = 9
+    scale = rng.uniform(0.99, 1.01, size=(5, N))
+    T_last = T * np.float32(1.5)
+    for kw in ({}, {"col_scale": scale, "T_last": T_last}):
+        ref = oprob.forward_loglik(T, start, accept, sy, **kw)
+        for threads in (1, 4):
+            got = ocpu.pnfa_forward(T, start, accept, sy, threads=threads, **kw)
+            for g, r in zip(got, ref):
+                fin = np.isfinite(r)
+                assert np.array_equal(fin, np.isfinite(g))
+                assert np.allclose(g[fin], r[fin], rtol=1e-13)
+    assert check_rank_one(ocpu.pnfa_forward) and check_permutation(ocpu.pnfa_forward)
+
+
+def test_planted_oracle_mistakes_are_caught():
+    """Each closed form rejects a plausible mistake planted in the oracle."""
+    transposed = lambda T, s, a, sy: oprob.forward_loglik(np.swapaxes(T, 1, 2), s, a, sy)
+    assert not check_rank_one(transposed)
+    assert not check_permutation(transposed)
+
+    def dropped_scale(T, s, a, sy):      # forget the first step's exponent
+        ll, lm = oprob.forward_loglik(T, s, a, sy[:, 1:])
+        return ll, lm
+    assert not check_rank_one(dropped_scale)
+    assert not check_scaling(dropped_scale) or not check_rank_one(dropped_scale)
+
+    def off_by_one(T, s, a, sy):         # symbols read one step late
+        return oprob.forward_loglik(T, s, a, np.roll(sy, 1, axis=1))
+    assert not check_permutation(off_by_one)
+
+
+# ---------------------------------------------------------------------------
+# GPU: MFMA kernels through the C ABI
+# ---------------------------------------------------------------------------
+
+def _engine(T, start, accept, mode=None):
+    from trex_amd.engine import ProbNfaEngine
+    return ProbNfaEngine(T, start, accept, mode)
+
+
+MODES = ["f16x2", "bf16x2", "f16", "bf16"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", MODES)
+def test_gpu_permutation_layout_probe(gpu_lib, mode):
+    """Permutation matrices are exact in every mode: any mistake in the MFMA
+    operand or result layout (a transposed fragment, a wrong row block, a lane
+    group writing the wrong rows) moves mass to the wrong state and changes LL."""
+    rng = np.random.default_rng(11)
+    T, perms, start, accept = _perm_model(rng, 32)
+    sy = rng.integers(0, 32, size=(300, 100)).astype(np.uint8)   # 4 full groups + 44, crosses a 64-symbol chunk
+    eng = _engine(T, start, accept, mode)
+    ll, lm, _, _ = eng.loglik(sy)
+    ref_ll, ref_lm = _perm_closed_form(perms, start, accept, sy)
+    assert np.allclose(ll, ref_ll, rtol=1e-13, atol=1e-13), np.flatnonzero(~np.isclose(ll, ref_ll))[:10]
+    assert np.allclose(lm, ref_lm, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.gpu
+def test_gpu_rounding_is_nearest_even(gpu_lib):
+    rng = np.random.default_rng(12)
+    T, start, accept = dirichlet_pnfa(rng, 4)
+    eng = _engine(T, start, accept)
+    t16, sc16 = eng.rounded("f16x2")
+    assert np.array_equal(t16, T.astype(np.float16).astype(np.float32))
+    bits = T.view(np.uint32).astype(np.uint64)
+    bf = (((bits + 0x7FFF + ((bits >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(np.float32)
+    tb, scb = eng.rounded("bf16")
+    assert np.array_equal(tb, bf)
+    for tq, sc in ((t16, sc16), (tb, scb)):   # the column factors restore every column's mass
+        mass = (tq.astype(np.float64) * sc[:, None, :].astype(np.float64)).sum(axis=1)
+        assert np.allclose(mass, T.astype(np.float64).sum(axis=1), rtol=1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", MODES)
+def test_gpu_dirichlet_precision(gpu_lib, mode):
+    """The precision workload's model at 1,024 sequences x 2,048 symbols."""
+    rng = np.random.default_rng(13)
+    T, start, accept = dirichlet_pnfa(rng, 32)
+    sy = rng.integers(0, 32, size=(1024, 2048)).astype(np.uint8)
+    eng = _engine(T, start, accept, mode)
+    ll, lm, _, _ = eng.loglik(sy)
+    ref_ll, ref_lm = oprob.forward_loglik(T, start, accept, sy)
+    err = _rel(ll, ref_ll)
+    tol = SPLIT_TOL if mode.endswith("x2") else SINGLE_TOL[mode]
+    print(f"{mode}: rel LL err median {np.median(err):.2e} max {err.max():.2e}")
+    assert err.max() <= tol
+    assert np.abs(lm - ref_lm).max() <= tol * np.abs(ref_ll).max()
+    if mode.endswith("x2"):
+        # against FP64 on the operator the mode applies (rounded T, column factors):
+        # what is left is the split state's rounding
+        tq, sc = eng.rounded(mode)
+        q_ll, _ = oprob.forward_loglik(tq, start, accept, sy, col_scale=sc, T_last=T)
+        assert _rel(ll, q_ll).max() <= 2e-4
+
+
+@pytest.mark.gpu
+def test_gpu_skewed_symbols_many_tiles(gpu_lib):
+    """90% of the sequences on one symbol: up to 4 tiles of 16 columns per step."""
+    rng = np.random.default_rng(14)
+    T, start, accept = dirichlet_pnfa(rng, 8)
+    sy = np.where(rng.random((512, 300)) < 0.9, 3, rng.integers(0, 8, size=(512, 300))).astype(np.uint8)
+    ll, lm, _, _ = _engine(T, start, accept).loglik(sy)
+    ref_ll, _ = oprob.forward_loglik(T, start, accept, sy)
+    assert _rel(ll, ref_ll).max() <= SPLIT_TOL
+
+
+@pytest.mark.gpu
+def test_gpu_nonstochastic_and_edges(gpu_lib):
+    rng = np.random.default_rng(15)
+    T, start, accept = dirichlet_pnfa(rng, 6)
+    T = (T * rng.uniform(0.2, 3.0, size=T.shape)).astype(np.float32)
+    start = rng.uniform(0, 1e-3, N).astype(np.float32)
+    eng = _engine(T, start, accept)
+    for L in (0, 1, 2, 63, 64, 65, 129):
+        sy = rng.integers(0, 6, size=(70, L)).astype(np.uint8)
+        ll, lm, _, _ = eng.loglik(sy)
+        ref_ll, ref_lm = oprob.forward_loglik(T, start, accept, sy)
+        assert _rel(ll, ref_ll).max() <= SPLIT_TOL, L
+        assert np.allclose(lm, ref_lm, rtol=SPLIT_TOL, atol=1e-5), L
+    sy = rng.integers(0, 6, size=(70, 40)).astype(np.uint8)
+    sy[5, 17] = 6          # outside the alphabet: the state empties
+    sy[9, 39] = 200
+    ll, lm, _, _ = eng.loglik(sy)
+    assert ll[5] == -np.inf and ll[9] == -np.inf and lm[5] == -np.inf
+    assert np.isfinite(np.delete(ll, [5, 9])).all()
+    ll, lm, _, _ = eng.loglik(np.zeros((0, 10), dtype=np.uint8))
+    assert ll.shape == (0,)
+
+
+@pytest.mark.gpu
+def test_gpu_device_pointer_dispatch(gpu_lib):
+    from trex_amd.device import DeviceBuffer, Stream
+    rng = np.random.default_rng(16)
+    T, start, accept = dirichlet_pnfa(rng, 32)
+    sy = rng.integers(0, 32, size=(200, 333)).astype(np.uint8)
+    eng = _engine(T, start, accept)
+    host_ll, host_lm, _, _ = eng.loglik(sy)
+    stream = Stream()
+    d_sy = DeviceBuffer.from_array(sy)
+    d_ll = DeviceBuffer(200 * 8)
+    d_lm = DeviceBuffer(200 * 8)
+    eng.loglik_device(d_sy.ptr, 200, 333, d_ll.ptr, d_lm.ptr, stream)
+    ll = d_ll.download(np.float64, 200, stream)
+    lm = d_lm.download(np.float64, 200, stream)
+    assert np.array_equal(ll, host_ll) and np.array_equal(lm, host_lm)
+
+
+@pytest.mark.gpu
+def test_gpu_argument_errors(gpu_lib):
+    from trex_amd._lib import TrexError
+    rng = np.random.default_rng(17)
+    T, start, accept = dirichlet_pnfa(rng, 33)
+    with pytest.raises(TrexError):
+        _engine(T, start, accept)
+    T = T[:4].copy()
+    T[1, 2, 3] = np.inf
+    with pytest.raises(TrexError):
+        _engine(T, start, accept)
+    T[1, 2, 3] = 1e6        # beyond FP16: BF16 only
+    eng = _engine(T, start, accept)
+    assert eng.info()["mode"] == "bf16x2" and not eng.info()["f16_ok"]
+    with pytest.raises(TrexError):
+        eng.set_mode("f16")
+    with pytest.raises(ValueError):
+        _engine(T[:, :32, :], start, accept)

@@ -67,6 +67,14 @@ _SIGNATURES = {
     "nfa_tc_engine_init": (c_int, [f32p, f32p, f32p, c_int, c_int, c_int, c_int]),
     "nfa_tc_engine_dispatch": (c_int, [u8p, intp, intp, c_int, c_int, f32p, f32p]),
     "nfa_tc_engine_destroy": (None, []),
+    # probabilistic NFA
+    "trex_pnfa_create": (c_int, [c_void_p, c_int, c_void_p, c_void_p, vpp]),
+    "trex_pnfa_destroy": (c_int, [c_void_p]),
+    "trex_pnfa_info": (c_int, [c_void_p, intp, intp, intp]),
+    "trex_pnfa_set_mode": (c_int, [c_void_p, c_int]),
+    "trex_pnfa_rounded": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "trex_pnfa_loglik": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
+    "trex_pnfa_loglik_host": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, f32p, f32p]),
     # chunk plans
     "trex_plan_create": (c_int, [c_void_p, c_void_p, c_int64, c_int64, vpp]),
     "trex_plan_destroy": (c_int, [c_void_p]),

@@ -256,6 +256,85 @@ class NfaEngine:
             pass
 
 
+PNFA_MODES = {"f16x2": 1, "bf16x2": 2, "f16": 3, "bf16": 4}
+PNFA_MODE_NAMES = {v: k for k, v in PNFA_MODES.items()}
+
+
+class ProbNfaEngine:
+    """One resident probabilistic NFA of 64 states (``trex_pnfa_create``).
+
+    ``T``: [n_symbols, 64, 64] float32, row = destination, column = source (the
+    layout of the reference's ``T_matrices``, cuda/nfa_tc_evolution.cu:943-953);
+    ``start``/``accept``: [64]. ``loglik`` returns per sequence
+    ``LL = log(accept . T[c_L] ... T[c_1] start)`` and ``log(1 . T[c_L] ... start)``.
+    """
+
+    N = 64
+
+    def __init__(self, T: np.ndarray, start: np.ndarray, accept: np.ndarray, mode: str | None = None):
+        self.T = np.ascontiguousarray(T, dtype=np.float32)
+        self.start = np.ascontiguousarray(start, dtype=np.float32)
+        self.accept = np.ascontiguousarray(accept, dtype=np.float32)
+        if self.T.ndim != 3 or self.T.shape[1:] != (self.N, self.N):
+            raise ValueError(f"T must have shape [n_symbols, 64, 64], got {self.T.shape}")
+        if self.start.shape != (self.N,) or self.accept.shape != (self.N,):
+            raise ValueError("start and accept must have 64 entries")
+        h = ctypes.c_void_p()
+        _lib.call("trex_pnfa_create", self.T.ctypes.data, self.T.shape[0], self.start.ctypes.data,
+                  self.accept.ctypes.data, ctypes.byref(h))
+        self.handle = h.value
+        self.n_symbols = self.T.shape[0]
+        if mode is not None:
+            self.set_mode(mode)
+
+    def info(self) -> dict:
+        m, n, ok = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.call("trex_pnfa_info", self.handle, ctypes.byref(m), ctypes.byref(n), ctypes.byref(ok))
+        return {"mode": PNFA_MODE_NAMES[m.value], "n_symbols": n.value, "f16_ok": bool(ok.value)}
+
+    def set_mode(self, mode: str) -> None:
+        if mode not in PNFA_MODES:
+            raise ValueError(f"unknown mode {mode!r}; choose from {sorted(PNFA_MODES)}")
+        _lib.call("trex_pnfa_set_mode", self.handle, PNFA_MODES[mode])
+
+    def rounded(self, mode: str):
+        """(T_rounded [σ,64,64] float32, col_scale [σ,64] float32) of ``mode``."""
+        t = np.empty_like(self.T)
+        sc = np.empty((self.n_symbols, self.N), dtype=np.float32)
+        _lib.call("trex_pnfa_rounded", self.handle, PNFA_MODES[mode], t.ctypes.data, sc.ctypes.data)
+        return t, sc
+
+    def loglik_device(self, d_symbols: int, n_seq: int, length: int, d_ll: int, d_logmass: int,
+                      stream: Stream | None = None) -> None:
+        _lib.call("trex_pnfa_loglik", self.handle, d_symbols, n_seq, length, d_ll, d_logmass,
+                  stream.handle if stream else None)
+
+    def loglik(self, symbols: np.ndarray):
+        """symbols [n_seq, L] uint8 -> (ll float64[n], logmass float64[n], kernel_ms, total_ms)."""
+        symbols = np.ascontiguousarray(symbols, dtype=np.uint8)
+        if symbols.ndim != 2:
+            raise ValueError("symbols must be [n_seq, length]")
+        n, L = symbols.shape
+        ll = np.empty(n, dtype=np.float64)
+        lm = np.empty(n, dtype=np.float64)
+        kms, tms = ctypes.c_float(), ctypes.c_float()
+        if n > 0:
+            _lib.call("trex_pnfa_loglik_host", self.handle, symbols.ctypes.data if symbols.size else None, n, L,
+                      ll.ctypes.data, lm.ctypes.data, ctypes.byref(kms), ctypes.byref(tms))
+        return ll, lm, kms.value, tms.value
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.lib().trex_pnfa_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class ChunkPlan:
     """Strings cut into chunks of ``chunk_len`` bytes for the look-back scan
     (``trex_plan_create``). ``begin``/``end`` are byte ranges in the device

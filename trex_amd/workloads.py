@@ -68,6 +68,18 @@ def random_nfa64(rng: np.random.Generator, n_symbols: int = 32, n_states: int = 
     return NFAMatrices.from_matrices(stack, start, accept, list(alphabet[:n_symbols]))
 
 
+def dirichlet_pnfa(rng: np.random.Generator, n_symbols: int = 32, n_states: int = 64, alpha: float = 0.1,
+                   n_accept: int = 16):
+    """Probabilistic NFA: column-stochastic T[c] ([dst][src]) whose columns are
+    drawn from Dirichlet(alpha), a uniform start vector and a 0/1 accept vector
+    with ``n_accept`` ones. Returns (T [σ, n, n] float32, start, accept)."""
+    T = np.stack([rng.dirichlet(np.full(n_states, alpha), size=n_states).T for _ in range(n_symbols)])
+    start = np.full(n_states, 1.0 / n_states, dtype=np.float32)
+    accept = np.zeros(n_states, dtype=np.float32)
+    accept[rng.choice(n_states, size=n_accept, replace=False)] = 1.0
+    return T.astype(np.float32), start, accept
+
+
 def uniform_batch(rng: np.random.Generator, n_strings: int, length: int, lo: int, hi: int):
     """``n_strings`` strings of ``length`` bytes uniform over [lo, hi)."""
     data = rng.integers(lo, hi, size=n_strings * length, dtype=np.uint8)
