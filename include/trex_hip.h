@@ -185,8 +185,11 @@ typedef struct trex_pnfa trex_pnfa;
  * BF16X2 when some |T| exceeds the FP16 range. */
 int trex_pnfa_create(const float* T, int n_symbols, const float* start, const float* accept, trex_pnfa** out);
 int trex_pnfa_destroy(trex_pnfa* h);
-int trex_pnfa_info(const trex_pnfa* h, int* mode, int* n_symbols, int* f16_ok);
+int trex_pnfa_info(const trex_pnfa* h, int* mode, int* n_symbols, int* f16_ok, int* waves);
 int trex_pnfa_set_mode(trex_pnfa* h, int mode);
+/* Waves per 64-sequence workgroup: 4 (eight symbols' matrices per wave) or 8
+ * (four per wave, two waves per SIMD). */
+int trex_pnfa_set_waves(trex_pnfa* h, int waves);
 /* The operator a mode applies on every step but the last: the rounded T as FP32
  * [n_symbols][64][64] and the column mass factors [n_symbols][64], so that
  * s' = T_rounded (col_scale[c] * s). Either output may be null. */

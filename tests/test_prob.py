@@ -218,8 +218,9 @@ MODES = ["f16x2", "bf16x2", "f16", "bf16"]
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("waves", [4, 8])
 @pytest.mark.parametrize("mode", MODES)
-def test_gpu_permutation_layout_probe(gpu_lib, mode):
+def test_gpu_permutation_layout_probe(gpu_lib, mode, waves):
     """Permutation matrices are exact in every mode: any mistake in the MFMA
     operand or result layout (a transposed fragment, a wrong row block, a lane
     group writing the wrong rows) moves mass to the wrong state and changes LL."""
@@ -227,10 +228,37 @@ def test_gpu_permutation_layout_probe(gpu_lib, mode):
     T, perms, start, accept = _perm_model(rng, 32)
     sy = rng.integers(0, 32, size=(300, 100)).astype(np.uint8)   # 4 full groups + 44, crosses a 64-symbol chunk
     eng = _engine(T, start, accept, mode)
+    eng.set_waves(waves)
     ll, lm, _, _ = eng.loglik(sy)
     ref_ll, ref_lm = _perm_closed_form(perms, start, accept, sy)
     assert np.allclose(ll, ref_ll, rtol=1e-13, atol=1e-13), np.flatnonzero(~np.isclose(ll, ref_ll))[:10]
     assert np.allclose(lm, ref_lm, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("waves", [4, 8])
+@pytest.mark.parametrize("mode", MODES)
+def test_gpu_rescale_probe(gpu_lib, mode, waves):
+    """T[c] = 4 P_c and a one-hot start: the single non-zero entry moves between
+    the lanes of a result column and grows by 4 per step. LL = log(a[pos]) +
+    L log 4 holds exactly only if every lane of the column agrees on the
+    power-of-two rescale (the cross-lane maximum) and the exponent is kept."""
+    rng = np.random.default_rng(18)
+    T, perms, _, _ = _perm_model(rng, 32)
+    T = T * np.float32(4.0)
+    start = np.zeros(N, dtype=np.float32)
+    start[5] = 1.0
+    accept = (np.arange(N, dtype=np.float32) + 1) / 64
+    sy = rng.integers(0, 32, size=(130, 150)).astype(np.uint8)
+    eng = _engine(T, start, accept, mode)
+    eng.set_waves(waves)
+    ll, lm, _, _ = eng.loglik(sy)
+    pos = np.full(len(sy), 5)
+    for t in range(sy.shape[1]):
+        pos = perms[sy[:, t], pos]
+    ref_ll = np.log(accept.astype(np.float64)[pos]) + sy.shape[1] * np.log(4.0)
+    assert np.allclose(ll, ref_ll, rtol=1e-13, atol=1e-12)
+    assert np.allclose(lm, sy.shape[1] * np.log(4.0), rtol=1e-13)
 
 
 @pytest.mark.gpu
@@ -273,12 +301,15 @@ def test_gpu_dirichlet_precision(gpu_lib, mode):
 
 
 @pytest.mark.gpu
-def test_gpu_skewed_symbols_many_tiles(gpu_lib):
+@pytest.mark.parametrize("waves", [4, 8])
+def test_gpu_skewed_symbols_many_tiles(gpu_lib, waves):
     """90% of the sequences on one symbol: up to 4 tiles of 16 columns per step."""
     rng = np.random.default_rng(14)
     T, start, accept = dirichlet_pnfa(rng, 8)
     sy = np.where(rng.random((512, 300)) < 0.9, 3, rng.integers(0, 8, size=(512, 300))).astype(np.uint8)
-    ll, lm, _, _ = _engine(T, start, accept).loglik(sy)
+    eng = _engine(T, start, accept)
+    eng.set_waves(waves)
+    ll, lm, _, _ = eng.loglik(sy)
     ref_ll, _ = oprob.forward_loglik(T, start, accept, sy)
     assert _rel(ll, ref_ll).max() <= SPLIT_TOL
 
@@ -340,5 +371,7 @@ def test_gpu_argument_errors(gpu_lib):
     assert eng.info()["mode"] == "bf16x2" and not eng.info()["f16_ok"]
     with pytest.raises(TrexError):
         eng.set_mode("f16")
+    with pytest.raises(TrexError):
+        eng.set_waves(16)
     with pytest.raises(ValueError):
         _engine(T[:, :32, :], start, accept)

@@ -70,7 +70,8 @@ _SIGNATURES = {
     # probabilistic NFA
     "trex_pnfa_create": (c_int, [c_void_p, c_int, c_void_p, c_void_p, vpp]),
     "trex_pnfa_destroy": (c_int, [c_void_p]),
-    "trex_pnfa_info": (c_int, [c_void_p, intp, intp, intp]),
+    "trex_pnfa_info": (c_int, [c_void_p, intp, intp, intp, intp]),
+    "trex_pnfa_set_waves": (c_int, [c_void_p, c_int]),
     "trex_pnfa_set_mode": (c_int, [c_void_p, c_int]),
     "trex_pnfa_rounded": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "trex_pnfa_loglik": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p]),

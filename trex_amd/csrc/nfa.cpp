@@ -46,6 +46,21 @@ void nibble_tables(const uint64_t* succ64, uint64_t* out /*[16][16]*/) {
         }
 }
 
+// LDS placement of the tables: class k stores entry (p, v) at row p ^ h and
+// slot v ^ x with h = (k >> 4) & 1 and x = k & 15, so lanes that look up the
+// same nibble value in different classes hit different banks (a dense state
+// set makes v = 15 far more frequent than the other values).
+std::vector<uint64_t> swizzled(const std::vector<uint64_t>& tab) {
+    std::vector<uint64_t> out(tab.size());
+    const size_t n_classes = tab.size() / 256;
+    for (size_t k = 0; k < n_classes; ++k) {
+        const unsigned h = (unsigned)(k >> 4) & 1u, x = (unsigned)k & 15u;
+        for (unsigned p = 0; p < 16; ++p)
+            for (unsigned v = 0; v < 16; ++v) out[k * 256 + (p ^ h) * 16 + (v ^ x)] = tab[k * 256 + p * 16 + v];
+    }
+    return out;
+}
+
 }  // namespace
 
 extern "C" {
@@ -88,7 +103,8 @@ int trex_nfa_create(const uint64_t* succ, int n_symbols, const int16_t* byte_to_
     }
     if (ok) h->window_base = window < 0 ? 0 : window;
 
-    int rc = upload((void**)&h->d_tab, tab.data(), tab.size() * 8);
+    const std::vector<uint64_t> tab_lds = swizzled(tab);
+    int rc = upload((void**)&h->d_tab, tab_lds.data(), tab_lds.size() * 8);
     if (!rc) rc = upload((void**)&h->d_cls, cls.data(), cls.size());
     if (!rc && h->window_base >= 0) {
         h->n_classes_win = 34;
@@ -101,7 +117,8 @@ int trex_nfa_create(const uint64_t* succ, int n_symbols, const int16_t* byte_to_
         std::copy(tab.begin() + (size_t)reject * 256, tab.begin() + (size_t)(reject + 1) * 256, tw.begin() + 32 * 256);
         std::copy(tab.begin() + (size_t)skip * 256, tab.begin() + (size_t)(skip + 1) * 256, tw.begin() + 33 * 256);
         for (int b = 0; b < 256; ++b) cw[b] = (uint8_t)(((b & ~31) == h->window_base) ? (b & 31) : 32);
-        rc = upload((void**)&h->d_tab_win, tw.data(), tw.size() * 8);
+        const std::vector<uint64_t> tw_lds = swizzled(tw);
+        rc = upload((void**)&h->d_tab_win, tw_lds.data(), tw_lds.size() * 8);
         if (!rc) rc = upload((void**)&h->d_cls_win, cw.data(), cw.size());
     }
     if (!rc) {

@@ -36,19 +36,26 @@ struct WinCheck {
     __device__ __forceinline__ bool outside(uint32_t win4) const { return ((any_hi & ~win4) | (win4 & ~all)) != 0u; }
 };
 
-// One step through class table `tab` (256 uint64 entries: [p][v]).
-__device__ __forceinline__ uint64_t nfa_step(const uint64_t* __restrict__ tab, uint64_t s) {
-    const uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32);
+// One step through the table of class k (256 uint64 entries, placed as the
+// host lays them out: entry (p, v) at row p ^ h, slot v ^ x with
+// h = (k >> 4) & 1, x = k & 15). XOR-ing every nibble of the state with x
+// gives the slot directly; rows of even and odd p move by +h and -h.
+__device__ __forceinline__ uint64_t nfa_step(const uint64_t* __restrict__ tab, uint32_t k, uint64_t s) {
+    const uint32_t x = (k & 15u) * 0x11111111u;
+    const uint32_t lo = (uint32_t)s ^ x, hi = (uint32_t)(s >> 32) ^ x;
+    const int h16 = (int)((k >> 4) & 1u) * 16;
+    const uint64_t* te = tab + k * 256u + h16;   // even p: row p + h
+    const uint64_t* to = tab + k * 256u - h16;   // odd p: row p - h
     uint64_t r0 = 0, r1 = 0;
 #pragma unroll
     for (int p = 0; p < 8; p += 2) {
-        r0 |= tab[p * 16 + ((lo >> (4 * p)) & 15u)];
-        r1 |= tab[(p + 1) * 16 + ((lo >> (4 * p + 4)) & 15u)];
+        r0 |= te[p * 16 + ((lo >> (4 * p)) & 15u)];
+        r1 |= to[(p + 1) * 16 + ((lo >> (4 * p + 4)) & 15u)];
     }
 #pragma unroll
     for (int p = 0; p < 8; p += 2) {
-        r0 |= tab[(8 + p) * 16 + ((hi >> (4 * p)) & 15u)];
-        r1 |= tab[(9 + p) * 16 + ((hi >> (4 * p + 4)) & 15u)];
+        r0 |= te[(8 + p) * 16 + ((hi >> (4 * p)) & 15u)];
+        r1 |= to[(9 + p) * 16 + ((hi >> (4 * p + 4)) & 15u)];
     }
     return r0 | r1;
 }
@@ -79,7 +86,7 @@ __device__ __forceinline__ void nfa_kernel(const uint8_t* __restrict__ bytes, co
     uint64_t s = start;
 
     const int64_t head_end = min(end, p + (int64_t)((16u - ((uint32_t)(uintptr_t)(bytes + p) & 15u)) & 15u));
-    for (; p < head_end; ++p) s = nfa_step(tab + (uint32_t)cls[bytes[p]] * 256u, s);
+    for (; p < head_end; ++p) s = nfa_step(tab, cls[bytes[p]], s);
 
     WinCheck w;
     w.init();
@@ -88,15 +95,15 @@ __device__ __forceinline__ void nfa_kernel(const uint8_t* __restrict__ bytes, co
     auto word = [&](uint32_t x) {
         if constexpr (WINDOW) {
             w.add(x);
-            s = nfa_step(tab + (x & 31u) * 256u, s);
-            s = nfa_step(tab + ((x >> 8) & 31u) * 256u, s);
-            s = nfa_step(tab + ((x >> 16) & 31u) * 256u, s);
-            s = nfa_step(tab + ((x >> 24) & 31u) * 256u, s);
+            s = nfa_step(tab, x & 31u, s);
+            s = nfa_step(tab, (x >> 8) & 31u, s);
+            s = nfa_step(tab, (x >> 16) & 31u, s);
+            s = nfa_step(tab, (x >> 24) & 31u, s);
         } else {
-            s = nfa_step(tab + (uint32_t)cls[x & 255u] * 256u, s);
-            s = nfa_step(tab + (uint32_t)cls[(x >> 8) & 255u] * 256u, s);
-            s = nfa_step(tab + (uint32_t)cls[(x >> 16) & 255u] * 256u, s);
-            s = nfa_step(tab + (uint32_t)cls[x >> 24] * 256u, s);
+            s = nfa_step(tab, cls[x & 255u], s);
+            s = nfa_step(tab, cls[(x >> 8) & 255u], s);
+            s = nfa_step(tab, cls[(x >> 16) & 255u], s);
+            s = nfa_step(tab, cls[x >> 24], s);
         }
     };
     int64_t i = 0;
@@ -114,7 +121,7 @@ __device__ __forceinline__ void nfa_kernel(const uint8_t* __restrict__ bytes, co
         i = 1;
     }
     p += nvec << 4;
-    for (; p < end; ++p) s = nfa_step(tab + (uint32_t)cls[bytes[p]] * 256u, s);
+    for (; p < end; ++p) s = nfa_step(tab, cls[bytes[p]], s);
     if constexpr (WINDOW) {
         if (w.outside(win4)) s = 0;
     }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -18,7 +19,6 @@ namespace {
 constexpr int kStates = 64;
 constexpr int kMaxSymbols = 32;   // four waves x eight symbols of register-resident fragments
 constexpr int kSeqPerGroup = 64;
-constexpr unsigned kProbBlock = 256;
 
 uint32_t f32_bits(float f) { uint32_t x; std::memcpy(&x, &f, 4); return x; }
 float bits_f32(uint32_t x) { float f; std::memcpy(&f, &x, 4); return f; }
@@ -84,12 +84,12 @@ void round_tables(const float* T, int n_sym, bool bf16, std::vector<uint16_t>& q
 
 bool uses_f16(int mode) { return mode == TREX_PNFA_F16X2 || mode == TREX_PNFA_F16; }
 
-const char* kernel_name(int mode) {
+const char* mode_name(int mode) {
     switch (mode) {
-        case TREX_PNFA_F16X2: return "pnfa64_f16x2";
-        case TREX_PNFA_BF16X2: return "pnfa64_bf16x2";
-        case TREX_PNFA_F16: return "pnfa64_f16";
-        case TREX_PNFA_BF16: return "pnfa64_bf16";
+        case TREX_PNFA_F16X2: return "f16x2";
+        case TREX_PNFA_BF16X2: return "bf16x2";
+        case TREX_PNFA_F16: return "f16";
+        case TREX_PNFA_BF16: return "bf16";
         default: return nullptr;
     }
 }
@@ -99,6 +99,7 @@ const char* kernel_name(int mode) {
 struct trex_pnfa {
     int n_symbols = 0;
     int mode = TREX_PNFA_F16X2;
+    int waves = 8;           // waves per workgroup (4: eight symbols per wave, 8: four)
     bool f16_ok = true;      // every |T| fits binary16
     int e0 = 0;              // s0 = s0n * 2^e0 with max |s0n| in [0.5, 1)
     std::vector<uint16_t> q16, qb16;
@@ -185,9 +186,10 @@ int trex_pnfa_destroy(trex_pnfa* h) {
     return TREX_OK;
 }
 
-int trex_pnfa_info(const trex_pnfa* h, int* mode, int* n_symbols, int* f16_ok) {
+int trex_pnfa_info(const trex_pnfa* h, int* mode, int* n_symbols, int* f16_ok, int* waves) {
     if (!h) return fail(TREX_ERR_INVALID, "null handle");
     if (mode) *mode = h->mode;
+    if (waves) *waves = h->waves;
     if (n_symbols) *n_symbols = h->n_symbols;
     if (f16_ok) *f16_ok = h->f16_ok ? 1 : 0;
     return TREX_OK;
@@ -195,15 +197,22 @@ int trex_pnfa_info(const trex_pnfa* h, int* mode, int* n_symbols, int* f16_ok) {
 
 int trex_pnfa_set_mode(trex_pnfa* h, int mode) {
     if (!h) return fail(TREX_ERR_INVALID, "null handle");
-    if (!kernel_name(mode)) return fail(TREX_ERR_INVALID, "unknown mode %d", mode);
+    if (!mode_name(mode)) return fail(TREX_ERR_INVALID, "unknown mode %d", mode);
     if (uses_f16(mode) && !h->f16_ok) return fail(TREX_ERR_INVALID, "T has entries beyond the FP16 range");
     h->mode = mode;
     return TREX_OK;
 }
 
+int trex_pnfa_set_waves(trex_pnfa* h, int waves) {
+    if (!h) return fail(TREX_ERR_INVALID, "null handle");
+    if (waves != 4 && waves != 8) return fail(TREX_ERR_INVALID, "waves must be 4 or 8, got %d", waves);
+    h->waves = waves;
+    return TREX_OK;
+}
+
 int trex_pnfa_rounded(const trex_pnfa* h, int mode, float* t_rounded, float* col_scale) {
     if (!h) return fail(TREX_ERR_INVALID, "null handle");
-    if (!kernel_name(mode)) return fail(TREX_ERR_INVALID, "unknown mode %d", mode);
+    if (!mode_name(mode)) return fail(TREX_ERR_INVALID, "unknown mode %d", mode);
     const bool f16 = uses_f16(mode);
     const auto& q = f16 ? h->q16 : h->qb16;
     const auto& sc = f16 ? h->scale16 : h->scaleb16;
@@ -227,7 +236,9 @@ int trex_pnfa_loglik(trex_pnfa* h, const uint8_t* d_symbols, int64_t n_seq, int6
     const float* scale = f16 ? h->d_scale16 : h->d_scaleb16;
     int n_sym = h->n_symbols, e0 = h->e0, L = (int)length;
     void* args[] = {&tlp, &scale, &h->d_t32, &n_sym, &h->d_s0, &e0, &h->d_acc, &d_symbols, &n_seq, &L, &d_ll, &d_logmass};
-    return launch(kernel_name(h->mode), (unsigned)groups, kProbBlock, 0, (hipStream_t)stream, args);
+    char name[48];
+    std::snprintf(name, sizeof(name), "pnfa64_%s_w%d", mode_name(h->mode), h->waves);
+    return launch(name, (unsigned)groups, 64u * (unsigned)h->waves, 0, (hipStream_t)stream, args);
 }
 
 int trex_pnfa_loglik_host(trex_pnfa* h, const uint8_t* symbols, int64_t n_seq, int64_t length, double* ll,

@@ -9,16 +9,18 @@
 // (cuda/nfa_tc_evolution.cu:451-561). This kernel:
 //
 //  * holds T rounded to FP16 (or BF16) as MFMA A-fragments in registers: a
-//    workgroup has four waves and wave w owns symbols 8w .. 8w+7 (64 fragments
-//    of 16 x 32 per wave);
+//    workgroup of W waves covers 32 symbols, wave w owns the SPW = 32 / W
+//    symbols SPW*w .. SPW*w + SPW-1 (8 fragments of 16 x 32 per symbol);
 //  * keeps the 64 sequences' states in LDS in FP32; before the product each
 //    state is multiplied by the symbol's column factors
 //    colsum(T[c]) / colsum(round(T[c])), so the rounded matrix moves exactly
 //    the mass the FP32 one does, then split into a high and a low half (two
 //    MFMAs per tile, about 22 bits of state) or rounded once;
 //  * each step, every wave gathers the sequences whose current symbol it owns
-//    into 16-column tiles (the n-th set bit of a ballot mask -> column n),
-//    runs v_mfma_f32_16x16x32_{f16,bf16} with FP32 accumulation, rescales each
+//    into 16-column tiles (the n-th set bit of a ballot mask -> column n) whose
+//    k dimension runs over all of the wave's matrices, each column's state
+//    placed in its own symbol's k block and zeros elsewhere; it runs
+//    v_mfma_f32_16x16x32_{f16,bf16} with FP32 accumulation, rescales each
 //    new state by an exact power of two (largest entry into [0.5, 1)) and adds
 //    the exponent to the sequence's integer log2-scale;
 //  * recomputes the last step in FP32 from the unrounded T and reduces
@@ -36,14 +38,11 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-#define PNFA_BLOCK 256
-
 namespace {
 
 constexpr int kStates = 64;
 constexpr int kRow = kStates + 4;   // padded LDS row (floats) against bank conflicts
 constexpr int kSeqPerGroup = 64;
-constexpr int kSymPerWave = 8;
 constexpr int kMaxSymbols = 32;
 constexpr int kChunk = 64;          // symbols staged per sequence at a time
 
@@ -93,17 +92,49 @@ __device__ __forceinline__ int nth_set_bit(uint64_t m, int n) {
     return base;
 }
 
+// Largest value over the four lanes l, l^16, l^32, l^48 (the lanes that hold
+// one column of an MFMA result), with the gfx950 lane-swap instructions.
+__device__ __forceinline__ float column_max(float v) {
+    unsigned u = __float_as_uint(v);
+    const auto a = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    v = fmaxf(__uint_as_float(a[0]), __uint_as_float(a[1]));
+    u = __float_as_uint(v);
+    const auto b = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    return fmaxf(__uint_as_float(b[0]), __uint_as_float(b[1]));
+}
+
+// Eight FP32 values -> MFMA operand: the high half (FP32 truncated to the
+// operand's mantissa, converted exactly) and, when split, the rounded rest.
 template <bool BF16, bool SPLIT>
+__device__ __forceinline__ void split8(const float (&x)[8], typename Lp<BF16>::vec& hi, typename Lp<BF16>::vec& lo) {
+    typedef typename Lp<BF16>::scalar scalar;
+    constexpr unsigned kMask = BF16 ? 0xffff0000u : 0xffffe000u;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        if constexpr (SPLIT) {
+            const float h = __uint_as_float(__float_as_uint(x[e]) & kMask);
+            hi[e] = (scalar)h;
+            lo[e] = (scalar)(x[e] - h);
+        } else {
+            hi[e] = (scalar)x[e];
+        }
+    }
+}
+
+template <bool BF16, bool SPLIT, int SPW>
 __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __restrict__ colscale,
                              const float* __restrict__ t32, int n_sym, const float* __restrict__ s0, int e0,
                              const float* __restrict__ accw, const uint8_t* __restrict__ symbols, int64_t n_seq,
                              int L, double* __restrict__ out_ll, double* __restrict__ out_logmass)
 {
     typedef typename Lp<BF16>::vec vec;
-    typedef typename Lp<BF16>::scalar scalar;
+    constexpr int kBlock = 64 * (kMaxSymbols / SPW);
+    constexpr int kTps = kBlock / kSeqPerGroup;   // threads per sequence in the last step
+    constexpr int kStage = kChunk + 1;            // staged symbols per sequence: a chunk plus the next one
     __shared__ __attribute__((aligned(16))) float state[2][kSeqPerGroup][kRow];
     __shared__ __attribute__((aligned(16))) float dscale[kMaxSymbols][kStates];
-    __shared__ uint8_t syms[kSeqPerGroup][kChunk];
+    __shared__ uint8_t syms[kSeqPerGroup][kChunk + 4];
+    __shared__ int32_t estep[2][kSeqPerGroup];   // exponent taken out at a step, by step parity
     __shared__ int32_t expo[kSeqPerGroup];
 
     const int tid = threadIdx.x;
@@ -112,13 +143,14 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
     const int64_t seq0 = (int64_t)blockIdx.x * kSeqPerGroup;
     const int nseq = (int)min<int64_t>(kSeqPerGroup, n_seq - seq0);
     const int koff = 8 * (lane >> 4);
+    const int rsub = 4 * (lane >> 4);   // first of the four result rows of this lane in a row block
 
     // A fragments of this wave's symbols: A[q][rb][kb] holds rows rb*16 + (lane & 15),
     // columns kb*32 + koff + e
-    vec A[kSymPerWave][4][2];
+    vec A[SPW][4][2];
 #pragma unroll
-    for (int q = 0; q < kSymPerWave; ++q) {
-        const int c = wave * kSymPerWave + q;
+    for (int q = 0; q < SPW; ++q) {
+        const int c = wave * SPW + q;
 #pragma unroll
         for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
@@ -132,96 +164,128 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
                 }
             }
     }
-    for (int i = tid; i < kSeqPerGroup * kStates; i += PNFA_BLOCK) state[0][i / kStates][i % kStates] = s0[i % kStates];
-    for (int i = tid; i < kMaxSymbols * kStates; i += PNFA_BLOCK)
+    auto stage = [&](int t) {
+        for (int i = tid; i < kSeqPerGroup * kStage; i += kBlock) {
+            const int s = i / kStage, k = i % kStage;
+            syms[s][k] = (s < nseq && t + k < L) ? symbols[(seq0 + s) * (int64_t)L + t + k] : (uint8_t)255;
+        }
+    };
+    for (int i = tid; i < kMaxSymbols * kStates; i += kBlock)
         dscale[i / kStates][i % kStates] = i < n_sym * kStates ? colscale[i] : 1.f;
-    if (tid < kSeqPerGroup) expo[tid] = e0;
+    if (tid < 2 * kSeqPerGroup) estep[tid / kSeqPerGroup][tid % kSeqPerGroup] = 0;
+    stage(0);
+    __syncthreads();
+    // The state kept in LDS is already multiplied by the column factors of the
+    // symbol it meets next (when that step runs on the rounded matrices).
+    for (int i = tid; i < kSeqPerGroup * kStates; i += kBlock) {
+        const int s = i / kStates, k = i % kStates;
+        const int c0 = syms[s][0];
+        state[0][s][k] = s0[k] * ((L > 1 && c0 < n_sym) ? dscale[c0][k] : 1.f);
+    }
+    int expo_home = e0;   // wave 0, lane j: log2-scale of sequence j
     __syncthreads();
 
     int cur = 0;
     for (int t = 0; t < L - 1; ++t) {
-        if (t % kChunk == 0) {
-            for (int i = tid; i < kSeqPerGroup * kChunk; i += PNFA_BLOCK) {
-                const int s = i / kChunk, k = i % kChunk;
-                syms[s][k] = (s < nseq && t + k < L) ? symbols[(seq0 + s) * (int64_t)L + t + k] : (uint8_t)255;
-            }
+        if (t % kChunk == 0 && t > 0) {
+            stage(t);
             __syncthreads();
         }
-        const int cj = lane < nseq ? (int)syms[lane][t % kChunk] : 255;
+        const int par = t & 1;
+        const int tc = t % kChunk;
+        if (wave == 0 && t > 0) expo_home += estep[par ^ 1][lane];
+        const int cj = lane < nseq ? (int)syms[lane][tc] : 255;
         // a symbol outside the alphabet empties the state
         if (wave == 0 && lane < nseq && cj >= n_sym) {
             float4* row = reinterpret_cast<float4*>(&state[cur ^ 1][lane][0]);
             for (int k = 0; k < kStates / 4; ++k) row[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            estep[par][lane] = 0;
         }
+        // One tile holds up to 16 sequences on any of this wave's symbols: the
+        // k dimension runs over the SPW symbols' matrices (SPW x 64), and each
+        // column's state sits only in its own symbol's k block.
+        const uint64_t m = __ballot(cj < n_sym && cj / SPW == wave);
+        const int cnt = __popcll(m);
+        const bool scale_next = t + 1 < L - 1;   // the next step also runs on the rounded matrices
+        for (int g = 0; g * 16 < cnt; ++g) {
+            const int n = g * 16 + (lane & 15);
+            const bool valid = n < cnt;
+            const int seq = nth_set_bit(m, valid ? n : g * 16);
+            const int c = syms[seq][tc];
+            const int cn = syms[seq][tc + 1];
+            const int myq = valid ? c - wave * SPW : -1;
+            vec bhi[2], blo[2];
 #pragma unroll
-        for (int q = 0; q < kSymPerWave; ++q) {
-            const int c = wave * kSymPerWave + q;
-            if (c >= n_sym) break;
-            const uint64_t m = __ballot(cj == c);
-            if (m == 0) continue;
-            const int cnt = __popcll(m);
-            for (int g = 0; g * 16 < cnt; ++g) {
-                const int n = g * 16 + (lane & 15);
-                const bool valid = n < cnt;
-                const int seq = valid ? nth_set_bit(m, n) : 0;
-                vec bhi[2], blo[2];
+            for (int kb = 0; kb < 2; ++kb) {
+                const float4* src = reinterpret_cast<const float4*>(&state[cur][seq][kb * 32 + koff]);
+                const float4 x0 = src[0], x1 = src[1];
+                const float x[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+                split8<BF16, SPLIT>(x, bhi[kb], blo[kb]);
+            }
+            // column factors of the next symbol for this lane's 16 result rows
+            float4 f[4];
+            const bool use_f = scale_next && cn < n_sym;
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+                f[rb] = use_f ? *reinterpret_cast<const float4*>(&dscale[cn][rb * 16 + rsub])
+                              : make_float4(1.f, 1.f, 1.f, 1.f);
+            f32x4 d[4];
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) d[rb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int q = 0; q < SPW; ++q) {
+                const bool mine = myq == q;
+                if (__ballot(mine) == 0) continue;   // no column of this tile on symbol q
+                const vec zero{};
+                vec bh[2], bl[2];
 #pragma unroll
                 for (int kb = 0; kb < 2; ++kb) {
-                    const float4* src = reinterpret_cast<const float4*>(&state[cur][seq][kb * 32 + koff]);
-                    const float4* dsc = reinterpret_cast<const float4*>(&dscale[c][kb * 32 + koff]);
-                    const float4 x0 = src[0], x1 = src[1], d0 = dsc[0], d1 = dsc[1];
-                    const float x[8] = {x0.x * d0.x, x0.y * d0.y, x0.z * d0.z, x0.w * d0.w,
-                                        x1.x * d1.x, x1.y * d1.y, x1.z * d1.z, x1.w * d1.w};
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        const float v = valid ? x[e] : 0.f;
-                        const scalar h = (scalar)v;
-                        bhi[kb][e] = h;
-                        if constexpr (SPLIT) blo[kb][e] = (scalar)(v - (float)h);
-                    }
+                    bh[kb] = mine ? bhi[kb] : zero;
+                    if constexpr (SPLIT) bl[kb] = mine ? blo[kb] : zero;
                 }
-                f32x4 d[4];
 #pragma unroll
-                for (int rb = 0; rb < 4; ++rb) {
-                    d[rb] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                    for (int kb = 0; kb < 2; ++kb) {
-                        d[rb] = Lp<BF16>::mfma(A[q][rb][kb], bhi[kb], d[rb]);
-                        if constexpr (SPLIT) d[rb] = Lp<BF16>::mfma(A[q][rb][kb], blo[kb], d[rb]);
-                    }
-                }
-                float mx = 0.f;
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb)
-                    mx = fmaxf(mx, fmaxf(fmaxf(fabsf(d[rb][0]), fabsf(d[rb][1])), fmaxf(fabsf(d[rb][2]), fabsf(d[rb][3]))));
-                mx = fmaxf(mx, __shfl_xor(mx, 16, 64));
-                mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
-                int e2 = 0;
-                if (mx > 0.f) (void)frexpf(mx, &e2);     // mx = f * 2^e2 with f in [0.5, 1)
-                if (valid) {
+                for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
                     for (int rb = 0; rb < 4; ++rb) {
-                        const float4 out = make_float4(ldexpf(d[rb][0], -e2), ldexpf(d[rb][1], -e2),
-                                                       ldexpf(d[rb][2], -e2), ldexpf(d[rb][3], -e2));
-                        *reinterpret_cast<float4*>(&state[cur ^ 1][seq][rb * 16 + 4 * (lane >> 4)]) = out;
+                        d[rb] = Lp<BF16>::mfma(A[q][rb][kb], bh[kb], d[rb]);
+                        if constexpr (SPLIT) d[rb] = Lp<BF16>::mfma(A[q][rb][kb], bl[kb], d[rb]);
                     }
-                    if ((lane >> 4) == 0) expo[seq] += e2;
+            }
+            float mx = 0.f;
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+                mx = fmaxf(mx, fmaxf(fmaxf(fabsf(d[rb][0]), fabsf(d[rb][1])), fmaxf(fabsf(d[rb][2]), fabsf(d[rb][3]))));
+            mx = column_max(mx);
+            const int e2 = mx > 0.f ? __builtin_amdgcn_frexp_expf(mx) : 0;   // mx = f * 2^e2, f in [0.5, 1)
+            if (valid) {
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) {
+                    const float4 out = make_float4(ldexpf(d[rb][0] * f[rb].x, -e2), ldexpf(d[rb][1] * f[rb].y, -e2),
+                                                   ldexpf(d[rb][2] * f[rb].z, -e2), ldexpf(d[rb][3] * f[rb].w, -e2));
+                    *reinterpret_cast<float4*>(&state[cur ^ 1][seq][rb * 16 + rsub]) = out;
                 }
+                if ((lane >> 4) == 0) estep[par][seq] = e2;
             }
         }
         __syncthreads();
         cur ^= 1;
     }
+    if (wave == 0) {
+        if (L >= 2) expo_home += estep[(L - 2) & 1][lane];
+        expo[lane] = expo_home;
+    }
+    __syncthreads();
 
-    // last step in FP32 from the unrounded T: four threads per sequence, 16 rows each
-    const int s = tid >> 2, part = tid & 3;
+    // last step in FP32 from the unrounded T: kTps threads per sequence
+    constexpr int kRows = kStates / kTps;
+    const int s = tid / kTps, part = tid % kTps;
     double acc_a = 0.0, acc_m = 0.0;
     if (s < nseq) {
         const float* st = state[cur][s];
         if (L >= 1) {
             const int c = symbols[(seq0 + s) * (int64_t)L + L - 1];
             if (c < n_sym) {
-                for (int r = part * 16; r < part * 16 + 16; ++r) {
+                for (int r = part * kRows; r < part * kRows + kRows; ++r) {
                     const float* trow = t32 + ((size_t)c * kStates + r) * kStates;
                     float v = 0.f;
                     for (int k = 0; k < kStates; ++k) v = fmaf(trow[k], st[k], v);
@@ -230,16 +294,17 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
                 }
             }
         } else {
-            for (int r = part * 16; r < part * 16 + 16; ++r) {
+            for (int r = part * kRows; r < part * kRows + kRows; ++r) {
                 acc_a += (double)st[r] * (double)accw[r];
                 acc_m += (double)st[r];
             }
         }
     }
-    acc_a += __shfl_xor(acc_a, 1, 64);
-    acc_a += __shfl_xor(acc_a, 2, 64);
-    acc_m += __shfl_xor(acc_m, 1, 64);
-    acc_m += __shfl_xor(acc_m, 2, 64);
+#pragma unroll
+    for (int o = 1; o < kTps; o <<= 1) {
+        acc_a += __shfl_xor(acc_a, o, 64);
+        acc_m += __shfl_xor(acc_m, o, 64);
+    }
     if (s < nseq && part == 0) {
         const double scale = (double)expo[s] * 0.69314718055994530942;   // ln 2
         out_ll[seq0 + s] = log(acc_a) + scale;
@@ -249,16 +314,24 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
 
 }  // namespace
 
-#define PNFA_KERNEL(NAME, BF16, SPLIT)                                                                            \
-    extern "C" __global__ void __launch_bounds__(PNFA_BLOCK) __attribute__((amdgpu_waves_per_eu(1, 1)))          \
+// Kernels: pnfa64_<mode>_w<W>, W = waves per workgroup (32 / symbols per wave);
+// one workgroup of 64 * W threads per 64 sequences.
+#define PNFA_KERNEL(NAME, BF16, SPLIT, SPW)                                                                       \
+    extern "C" __global__ void __launch_bounds__(64 * (32 / SPW))                                                  \
+    __attribute__((amdgpu_waves_per_eu(32 / SPW / 4, 32 / SPW / 4)))                                              \
     NAME(const uint16_t* __restrict__ tlp, const float* __restrict__ colscale, const float* __restrict__ t32,     \
          int n_sym, const float* __restrict__ s0, int e0, const float* __restrict__ accw,                         \
          const uint8_t* __restrict__ symbols, int64_t n_seq, int L, double* __restrict__ out_ll,                  \
          double* __restrict__ out_logmass) {                                                                      \
-        pnfa_forward<BF16, SPLIT>(tlp, colscale, t32, n_sym, s0, e0, accw, symbols, n_seq, L, out_ll, out_logmass); \
+        pnfa_forward<BF16, SPLIT, SPW>(tlp, colscale, t32, n_sym, s0, e0, accw, symbols, n_seq, L, out_ll,        \
+                                       out_logmass);                                                              \
     }
 
-PNFA_KERNEL(pnfa64_f16x2, false, true)
-PNFA_KERNEL(pnfa64_bf16x2, true, true)
-PNFA_KERNEL(pnfa64_f16, false, false)
-PNFA_KERNEL(pnfa64_bf16, true, false)
+#define PNFA_KERNELS(W, SPW)                                  \
+    PNFA_KERNEL(pnfa64_f16x2_w##W, false, true, SPW)          \
+    PNFA_KERNEL(pnfa64_bf16x2_w##W, true, true, SPW)          \
+    PNFA_KERNEL(pnfa64_f16_w##W, false, false, SPW)           \
+    PNFA_KERNEL(pnfa64_bf16_w##W, true, false, SPW)
+
+PNFA_KERNELS(4, 8)
+PNFA_KERNELS(8, 4)

@@ -288,9 +288,12 @@ class ProbNfaEngine:
             self.set_mode(mode)
 
     def info(self) -> dict:
-        m, n, ok = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        _lib.call("trex_pnfa_info", self.handle, ctypes.byref(m), ctypes.byref(n), ctypes.byref(ok))
-        return {"mode": PNFA_MODE_NAMES[m.value], "n_symbols": n.value, "f16_ok": bool(ok.value)}
+        m, n, ok, w = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.call("trex_pnfa_info", self.handle, ctypes.byref(m), ctypes.byref(n), ctypes.byref(ok), ctypes.byref(w))
+        return {"mode": PNFA_MODE_NAMES[m.value], "n_symbols": n.value, "f16_ok": bool(ok.value), "waves": w.value}
+
+    def set_waves(self, waves: int) -> None:
+        _lib.call("trex_pnfa_set_waves", self.handle, int(waves))
 
     def set_mode(self, mode: str) -> None:
         if mode not in PNFA_MODES:
