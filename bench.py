@@ -95,8 +95,14 @@ def main() -> int:
     ap.add_argument("--variant", default="auto")
     ap.add_argument("--prefetch", type=int, default=8, help="16-byte vectors in flight per lane (8 or 16)")
     ap.add_argument("--layout", default="staggered", choices=["staggered", "csr"])
-    ap.add_argument("--legs", default="dfa,nfa,prob", help="comma list of dfa, nfa, prob")
+    ap.add_argument("--legs", default="dfa,nfa,prob,mix,serve", help="comma list of dfa, nfa, prob, mix, serve")
     ap.add_argument("--prob-modes", default="f16x2,bf16x2,f16,bf16")
+    ap.add_argument("--serve-qps", type=float, default=50.0, help="Poisson arrival rate of the serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-burst", type=int, default=24, help="requests of the served-vs-plain burst")
+    ap.add_argument("--serve-hbm-gb", type=int, default=64, help="HBM budget of the serving arena")
+    ap.add_argument("--serve-prob-cap", type=int, default=1 << 16,
+                    help="longest probabilistic sequence in serving requests (each symbol is a sequential step)")
     ap.add_argument("--sweep", action="store_true", help="time every kernel variant and layout, then exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also time a plain 1 GiB device copy (HBM traffic counter calibration)")
@@ -151,6 +157,12 @@ def main() -> int:
     if "prob" in legs:
         secondary["prob_nfa64"] = prob_leg(args, stream)
         ok &= secondary["prob_nfa64"].get("within_tolerance") in (None, True)
+    if "mix" in legs:
+        secondary["uneven_mix"] = mix_leg(args, stream)
+        ok &= secondary["uneven_mix"].get("verified_bit_exact") in (None, True)
+    if "serve" in legs:
+        secondary["serving"] = serve_leg(args, stream)
+        ok &= secondary["serving"]["burst"]["same_results"]
 
     if not result:   # a run without the headline leg: report what ran
         result = {"metric": "secondary legs only", "value": None, "n_gpus": 1, "steps": args.steps,
@@ -414,6 +426,301 @@ def prob_leg(args, stream) -> dict:
                          "kernel": head["kernel"], "alg_flops_per_launch": flops,
                          "note": "algorithmic FLOPs = 2*64^2 per symbol (one matvec); the split mode issues 2x"},
             "modes": per_mode, "cpu_baseline": cpu, "within_tolerance": within}
+
+
+MIX_STRINGS = 65_536
+MIX_PATTERNS = 64                  # 32 DFA-16 + 32 Boolean NFA-64
+MIX_LEN = (32, 1 << 24)            # log-uniform string lengths
+MIX_SUBSET = (0, 1, 2, 3, 32, 33, 34, 35)   # patterns of the batched-vs-plain comparison
+MIX_GEN_SEED = 0x5EED03
+
+
+def mix_leg(args, stream) -> dict:
+    """Uneven mix: one persistent launch vs one launch per pattern."""
+    from trex_amd import _lib
+    from trex_amd.autotune import tune_mix
+    from trex_amd.device import DeviceBuffer, Event
+    from trex_amd.engine import ChunkPlan, DfaEngine, NfaEngine
+    from trex_amd.mix import MixEngine
+    from trex_amd.tables import dfa_byte_table
+    from trex_amd.workloads import ALPHABET_32, random_dfa_pattern, random_nfa64
+
+    rng = np.random.default_rng(SEED + 3)
+    dfa_tables = [dfa_byte_table(random_dfa_pattern(rng, ALPHABET_32)[1]) for _ in range(MIX_PATTERNS // 2)]
+    nfas = [random_nfa64(rng) for _ in range(MIX_PATTERNS // 2)]
+    engines = [DfaEngine(t) for t in dfa_tables] + [NfaEngine.from_matrices(nm) for nm in nfas]
+    lengths = np.floor(np.exp(rng.uniform(np.log(MIX_LEN[0]), np.log(MIX_LEN[1]), size=MIX_STRINGS))).astype(np.int64)
+    pattern_of = rng.integers(0, MIX_PATTERNS, size=MIX_STRINGS).astype(np.int32)
+    offsets = np.zeros(MIX_STRINGS + 1, np.int64)
+    np.cumsum(lengths, out=offsets[1:])
+    total = int(offsets[-1])
+    t0 = time.perf_counter()
+    d_bytes = DeviceBuffer(-(-total // 16) * 16)
+    _lib.call("trex_gen_symbols", d_bytes.ptr, d_bytes.nbytes, MIX_GEN_SEED, 0x40, 32, stream.handle)
+    stream.synchronize()
+    log(f"mix: {MIX_STRINGS} strings, {total / 1e9:.1f} GB generated in HBM in {time.perf_counter() - t0:.2f}s")
+    n = MIX_STRINGS
+    d_mask, d_acc = DeviceBuffer(8 * n), DeviceBuffer(n)
+    begin, end = offsets[:-1], offsets[1:]
+
+    def timed(fn, reps):
+        fn()
+        stream.synchronize()
+        evs = [(Event(), Event()) for _ in range(reps)]
+        for a, b in evs:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        stream.synchronize()
+        return [a.elapsed_ms(b) for a, b in evs]
+
+    # autotuned plan on a sample of the batch, then the full batch
+    params, tuning = tune_mix(engines, begin, end, pattern_of, d_bytes.ptr, stream)
+    log(f"mix: tuned {params} ({tuning.get('source')})")
+    mix = MixEngine(engines)
+    info = mix.plan(begin, end, pattern_of, params)
+    ms = timed(lambda: mix.run(d_bytes.ptr, d_mask.ptr, d_acc.ptr, stream), max(2, args.steps // 10))
+    err = mix.error(stream)
+    mix.capture(d_bytes.ptr, d_mask.ptr, d_acc.ptr, stream)
+    ms_graph = timed(lambda: mix.replay(stream), max(2, args.steps // 10))
+    full_mask = d_mask.download(np.uint64, n, stream)
+    full_acc = d_acc.download(np.uint8, n, stream)
+    avg = float(np.median(ms))
+    gchars = total / (avg * 1e-3) / 1e9
+    log(f"mix: one launch {avg:.2f} ms (graph replay {np.median(ms_graph):.2f} ms), {gchars:.1f} Gchars/s, "
+        f"plan {info}, error word {err}")
+
+    # batched vs plain on the subset
+    sub = np.flatnonzero(np.isin(pattern_of, MIX_SUBSET))
+    sub_bytes = int(lengths[sub].sum())
+    sub_mix = MixEngine(engines)
+    sub_mix.plan(begin[sub], end[sub], pattern_of[sub], params)
+    d_smask, d_sacc = DeviceBuffer(8 * len(sub)), DeviceBuffer(len(sub))
+    ms_sub = float(np.median(timed(lambda: sub_mix.run(d_bytes.ptr, d_smask.ptr, d_sacc.ptr, stream), 2)))
+    plain_ms = 0.0
+    plain_mask = np.zeros(len(sub), np.uint64)
+    plain_acc = np.zeros(len(sub), np.uint8)
+    for p in MIX_SUBSET:
+        idx = np.flatnonzero(pattern_of[sub] == p)
+        sb, se = begin[sub][idx], end[sub][idx]
+        d_pm, d_pa = DeviceBuffer(8 * len(idx)), DeviceBuffer(len(idx))
+        e = engines[p]
+        if isinstance(e, DfaEngine):
+            plan = ChunkPlan(sb, se, chunk_len=1 << 16)
+            d_state = DeviceBuffer(len(idx))
+            t = timed(lambda: e.match_plan(plan, d_bytes.ptr, d_state.ptr, d_pa.ptr, stream), 1)[0]
+            plain_mask[idx] = d_state.download(np.uint8, len(idx), stream)
+            d_state.free()
+            plan.close()
+        else:
+            d_b, d_e = DeviceBuffer.from_array(sb, stream), DeviceBuffer.from_array(se, stream)
+            t = timed(lambda: _lib.call("trex_nfa_match_ranges", e.handle, d_bytes.ptr, d_b.ptr, d_e.ptr, len(idx),
+                                        d_pm.ptr, d_pa.ptr, stream.handle), 1)[0]
+            plain_mask[idx] = d_pm.download(np.uint64, len(idx), stream)
+            d_b.free()
+            d_e.free()
+        plain_acc[idx] = d_pa.download(np.uint8, len(idx), stream)
+        plain_ms += t
+        log(f"mix plain: pattern {p} ({type(e).__name__}) {len(idx)} strings, {t:.2f} ms")
+        d_pm.free()
+        d_pa.free()
+    sub_mask = d_smask.download(np.uint64, len(sub), stream)
+    sub_acc = d_sacc.download(np.uint8, len(sub), stream)
+    same_sub = bool(np.array_equal(sub_mask, plain_mask) and np.array_equal(sub_acc, plain_acc)
+                    and np.array_equal(full_mask[sub], plain_mask) and np.array_equal(full_acc[sub], plain_acc))
+    ratio = plain_ms / ms_sub
+    log(f"mix subset {MIX_SUBSET}: {sub_bytes / 1e9:.2f} GB, batched {ms_sub:.2f} ms vs plain {plain_ms:.2f} ms "
+        f"= {ratio:.1f}x; batched == plain: {same_sub}")
+
+    verified = None
+    cpu = None
+    if not args.no_verify:
+        from oracle import cpu as ocpu
+        from trex_amd.engine import nfa_byte_map
+        vr = np.random.default_rng(7)
+        short = np.flatnonzero(lengths <= 1 << 16)
+        long_ = np.flatnonzero(lengths > 1 << 20)
+        pick = np.concatenate([vr.choice(short, min(4096, len(short)), replace=False),
+                               vr.choice(long_, min(24, len(long_)), replace=False)])
+        ok = True
+        t_cpu = 0.0
+        n_checked = 0
+        for p in range(MIX_PATTERNS):
+            idx = pick[pattern_of[pick] == p]
+            if len(idx) == 0:
+                continue
+            sub_off = np.zeros(len(idx) + 1, np.int64)
+            np.cumsum(lengths[idx], out=sub_off[1:])
+            data = np.concatenate([ocpu.gen_symbols(begin[i], lengths[i], MIX_GEN_SEED) for i in idx])
+            t0 = time.perf_counter()
+            if p < MIX_PATTERNS // 2:
+                t = dfa_tables[p]
+                rf, ra = ocpu.dfa_walk(t.delta, t.accept, t.start, data, sub_off)
+            else:
+                nm = nfas[p - MIX_PATTERNS // 2]
+                rf, ra = ocpu.nfa_walk(nm.successor_masks(), nfa_byte_map(nm, "reject"), nm.start_mask(),
+                                       nm.accept_bits(), data, sub_off)
+            t_cpu += time.perf_counter() - t0
+            n_checked += int(sub_off[-1])
+            ok &= bool(np.array_equal(full_mask[idx], rf.astype(np.uint64)) and np.array_equal(full_acc[idx], ra))
+        verified = ok and same_sub and err == 0
+        cpu = {"value": round(n_checked / t_cpu / 1e9, 3), "unit": "Gchars/s", "cores": ocpu.default_threads(),
+               "kind": "port", "sample": f"{len(pick)} strings ({n_checked / 1e9:.2f} GB: 4096 <= 64 KiB and 24 "
+                                         f"> 1 MiB) through the C oracle walks, {t_cpu:.2f} s"}
+        log(f"mix: oracle check of {len(pick)} strings: {ok}")
+    for b in (d_bytes, d_mask, d_acc, d_smask, d_sacc):
+        b.free()
+    return {"workload": "uneven_mix_64_patterns", "strings": n, "bytes": total, "patterns": "32 DFA-16 + 32 NFA-64",
+            "value": round(gchars, 2), "unit": "Gchars/s", "kernel_ms": round(avg, 3),
+            "graph_replay_ms": round(float(np.median(ms_graph)), 3), "plan": info, "params": params.__dict__,
+            "tuning": tuning,
+            "roofline": {"bound": "hbm (DFA strings) / lds (NFA strings)",
+                         "achieved": round(total / (avg * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(total / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "batched_vs_plain": {"patterns": list(MIX_SUBSET), "bytes": sub_bytes, "batched_ms": round(ms_sub, 3),
+                                 "plain_ms": round(plain_ms, 3), "speedup": round(ratio, 2),
+                                 "plain": "one launch per pattern: look-back plan (DFA), lane per string (NFA)"},
+            "verified_bit_exact": verified, "cpu_baseline": cpu}
+
+
+SERVE_PATTERNS = (77, 38, 13)      # DFA-16, Boolean NFA-64, probabilistic NFA-64
+SERVE_POOL = 3 << 30               # host bytes each request's strings are sliced from
+SERVE_MEDIAN, SERVE_SIGMA, SERVE_CAP = 4096, 2.0, 1 << 24
+
+
+def serve_leg(args, stream) -> dict:
+    """Continuous batching: Poisson arrivals into the server, latency per
+    request; a saturated burst served vs one launch per request."""
+    from trex_amd.engine import ChunkPlan, DfaEngine, NfaEngine, ProbNfaEngine
+    from trex_amd.serve import Request, Server
+    from trex_amd.tables import dfa_byte_table
+    from trex_amd.workloads import ALPHABET_32, dirichlet_pnfa, random_dfa_pattern, random_nfa64
+
+    rng = np.random.default_rng(SEED + 4)
+    n_dfa, n_nfa, n_prob = SERVE_PATTERNS
+    engines = [DfaEngine(dfa_byte_table(random_dfa_pattern(rng, ALPHABET_32)[1])) for _ in range(n_dfa)]
+    engines += [NfaEngine.from_matrices(random_nfa64(rng)) for _ in range(n_nfa)]
+    engines += [ProbNfaEngine(*dirichlet_pnfa(rng, 32)) for _ in range(n_prob)]
+    t0 = time.perf_counter()
+    pool = rng.integers(0x40, 0x60, size=SERVE_POOL, dtype=np.uint8)
+    pool_sym = pool & np.uint8(31)
+    log(f"serve: {len(engines)} resident patterns, {SERVE_POOL >> 30} GiB request pool in {time.perf_counter() - t0:.1f}s")
+
+    def make_request(r: np.random.Generator) -> Request:
+        p = int(r.integers(0, len(engines)))
+        while True:
+            n = int(r.integers(1, 4097))
+            cap = args.serve_prob_cap if isinstance(engines[p], ProbNfaEngine) else SERVE_CAP
+            lengths = np.minimum(np.floor(np.exp(r.normal(np.log(SERVE_MEDIAN), SERVE_SIGMA, size=n))),
+                                 cap).astype(np.int64)
+            total = int(lengths.sum())
+            if total <= SERVE_POOL:
+                break
+        off = np.zeros(n + 1, np.int64)
+        np.cumsum(lengths, out=off[1:])
+        start = int(r.integers(0, SERVE_POOL - total + 1))
+        src = pool_sym if isinstance(engines[p], ProbNfaEngine) else pool
+        return Request(p, src[start:start + total], off)
+
+    def pct(x, q):
+        return round(float(np.percentile(x, q)), 3) if len(x) else None
+
+    qps, duration = args.serve_qps, args.serve_seconds
+    arr_rng = np.random.default_rng(11)
+    n_req = int(qps * duration)
+    gaps = arr_rng.exponential(1.0 / qps, size=n_req)
+    reqs = [make_request(arr_rng) for _ in range(n_req)]
+    server = Server(engines, hbm_budget=args.serve_hbm_gb << 30)
+    t_start = time.perf_counter()
+    with server:
+        due = t_start
+        for gap, r in zip(gaps, reqs):
+            due += gap
+            while True:
+                now = time.perf_counter()
+                if now >= due:
+                    break
+                time.sleep(min(due - now, 0.0005))
+            server.submit(r)
+        unfinished = sum(not r.wait(max(0.0, t_start + duration + 120 - time.perf_counter())) for r in reqs)
+    wall = time.perf_counter() - t_start
+    if unfinished:
+        log(f"serve: {unfinished} requests unfinished after the time limit")
+        reqs = [r for r in reqs if r.done]
+    lat = np.array([r.latency_ms for r in reqs])
+    small = np.array([r.nbytes <= 1 << 20 for r in reqs])
+    kinds = np.array(["prob" if isinstance(engines[r.pattern], ProbNfaEngine) else "dfa_nfa" for r in reqs])
+    total_bytes = sum(r.nbytes for r in reqs)
+    by_lane = {}
+    for lane in ("small", "bulk", "prob_small", "prob_bulk"):
+        sel = np.array([r.lane == lane for r in reqs])
+        by_lane[lane] = {"requests": int(sel.sum()), "p50_ms": pct(lat[sel], 50), "p99_ms": pct(lat[sel], 99)}
+    sel_dn = small & (kinds == "dfa_nfa")
+    log(f"serve: {n_req} requests at {qps} QPS in {wall:.1f}s, {total_bytes / 1e9:.1f} GB; "
+        f"p50 {pct(lat, 50)} ms p99 {pct(lat, 99)} ms; <= 1 MiB: {int(small.sum())} requests p50 "
+        f"{pct(lat[small], 50)} p99 {pct(lat[small], 99)} ms (DFA/NFA only: p99 {pct(lat[sel_dn], 99)} ms); "
+        f"ticks {dict(server.ticks)}")
+
+    # saturated burst: served vs one launch per request (DFA / NFA requests)
+    burst_rng = np.random.default_rng(12)
+    burst = []
+    while len(burst) < args.serve_burst:
+        r = make_request(burst_rng)
+        if not isinstance(engines[r.pattern], ProbNfaEngine):
+            burst.append(r)
+    bb = sum(r.nbytes for r in burst)
+    srv = Server(engines, hbm_budget=args.serve_hbm_gb << 30)
+    t0 = time.perf_counter()
+    with srv:
+        for r in burst:
+            srv.submit(r)
+        for r in burst:
+            if not r.wait(300):
+                raise RuntimeError("burst request not served within 300 s")
+    served_s = time.perf_counter() - t0
+    from trex_amd import _lib
+    from trex_amd.device import DeviceBuffer
+    t0 = time.perf_counter()
+    same = True
+    for r in burst:   # plain: each request alone, one launch, synchronously
+        e = engines[r.pattern]
+        n = len(r.offsets) - 1
+        d = DeviceBuffer.from_array(r.data if r.nbytes else np.zeros(16, np.uint8), stream)
+        d_m, d_a = DeviceBuffer(8 * n), DeviceBuffer(n)
+        if isinstance(e, DfaEngine):
+            plan = ChunkPlan(r.offsets[:-1], r.offsets[1:], chunk_len=1 << 16)
+            e.match_plan(plan, d.ptr, d_m.ptr, d_a.ptr, stream)
+            got = d_m.download(np.uint8, n, stream).astype(np.uint64)
+            plan.close()
+        else:
+            d_b, d_e = DeviceBuffer.from_array(r.offsets[:-1], stream), DeviceBuffer.from_array(r.offsets[1:], stream)
+            _lib.call("trex_nfa_match_ranges", e.handle, d.ptr, d_b.ptr, d_e.ptr, n, d_m.ptr, d_a.ptr, stream.handle)
+            got = d_m.download(np.uint64, n, stream)
+            d_b.free()
+            d_e.free()
+        acc = d_a.download(np.uint8, n, stream)
+        same &= bool(np.array_equal(got, r.result[0]) and np.array_equal(acc, r.result[1]))
+        for b in (d, d_m, d_a):
+            b.free()
+    plain_s = time.perf_counter() - t0
+    log(f"serve burst: {len(burst)} DFA/NFA requests, {bb / 1e9:.2f} GB: served {served_s:.2f}s "
+        f"({bb / served_s / 1e9:.1f} Gchars/s) vs plain {plain_s:.2f}s ({bb / plain_s / 1e9:.1f} Gchars/s), "
+        f"same results {same}")
+    return {"workload": "serving_128_patterns", "qps": qps, "requests": n_req, "unfinished": unfinished,
+            "seconds": round(wall, 2), "prob_sequence_cap": args.serve_prob_cap,
+            "bytes": total_bytes, "hbm_budget_gb": args.serve_hbm_gb,
+            "latency_ms": {"all": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
+                           "le_1MiB": {"requests": int(small.sum()), "p50": pct(lat[small], 50),
+                                       "p99": pct(lat[small], 99)},
+                           "le_1MiB_dfa_nfa": {"requests": int(sel_dn.sum()), "p50": pct(lat[sel_dn], 50),
+                                               "p99": pct(lat[sel_dn], 99)},
+                           "by_lane": by_lane},
+            "served_gchars_s": round(total_bytes / wall / 1e9, 3),
+            "burst": {"requests": len(burst), "bytes": bb, "served_s": round(served_s, 3),
+                      "plain_s": round(plain_s, 3), "served_gchars_s": round(bb / served_s / 1e9, 2),
+                      "plain_gchars_s": round(bb / plain_s / 1e9, 2), "speedup": round(plain_s / served_s, 2),
+                      "same_results": same,
+                      "plain": "one synchronous launch per request: look-back plan (DFA), lane per string (NFA)"}}
 
 
 def sweep(args, data, offsets, batch, engines, outs, stream) -> int:

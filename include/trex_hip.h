@@ -51,6 +51,9 @@ int trex_device_check(void);
 /* Copies the calling thread's last error message into buf[0..n); returns its length. */
 int trex_last_error(char* buf, int n);
 
+/* Architecture name (e.g. gfx950:sramecc+:xnack-), compute units and HBM bytes of the current device. */
+int trex_device_info(char* name, int n, int* cu_count, int64_t* hbm_bytes);
+
 /* ------------------------------------------------------------------------
  * Memory and stream plumbing (no reference counterpart: the reference copies
  * synchronously inside each dispatch, cuda/monoid_batch.cu:504-553).
@@ -70,6 +73,16 @@ int trex_event_create(void** ev);
 int trex_event_destroy(void* ev);
 int trex_event_record(void* ev, void* stream);
 int trex_event_elapsed_ms(void* start, void* stop, float* ms);
+/* 0 when every command before the event has completed, 1 when not yet, < 0 on error. */
+int trex_event_query(void* ev);
+int trex_event_sync(void* ev);
+/* Later work on `stream` waits for `ev` (no host wait). */
+int trex_stream_wait_event(void* stream, void* ev);
+
+/* Synthetic input in HBM (no reference counterpart): bytes [0, n_bytes) of
+ * d_out (16-byte aligned, n_bytes a multiple of 16) become lo + (r & (span-1)),
+ * r = byte (i % 8) of splitmix64(seed ^ ((i / 8) * 0x9E3779B97F4A7C15)). */
+int trex_gen_symbols(void* d_out, int64_t n_bytes, uint64_t seed, uint32_t lo, uint32_t span, void* stream);
 
 /* ------------------------------------------------------------------------
  * DFA byte-table engine (hot path (a)).
@@ -198,9 +211,61 @@ int trex_pnfa_rounded(const trex_pnfa* h, int mode, float* t_rounded, float* col
  * FP64. Asynchronous on `stream`. length <= 2^31 - 1. */
 int trex_pnfa_loglik(trex_pnfa* h, const uint8_t* d_symbols, int64_t n_seq, int64_t length, double* d_ll,
                      double* d_logmass, void* stream);
+/* Ragged sequences: sequence j = d_symbols[d_begin[j] .. d_end[j]) (lengths
+ * up to 2^31 - 1; sequences of similar length in the same group of 64 waste
+ * fewer steps, so callers sort by length). */
+int trex_pnfa_loglik_ranges(trex_pnfa* h, const uint8_t* d_symbols, const int64_t* d_begin, const int64_t* d_end,
+                            int64_t n_seq, double* d_ll, double* d_logmass, void* stream);
 /* Host-pointer form (synchronous), with kernel and end-to-end times. */
 int trex_pnfa_loglik_host(trex_pnfa* h, const uint8_t* symbols, int64_t n_seq, int64_t length, double* ll,
                           double* logmass, float* kernel_ms, float* total_ms);
+
+/* ------------------------------------------------------------------------
+ * Mixed batches: many DFA-16 and Boolean NFA-64 patterns and strings of
+ * uneven length in ONE persistent launch fed by a length-sorted work queue.
+ * Replaces the reference's one-engine-per-dispatch model (one global engine
+ * per library and one synchronous launch per batch and pattern,
+ * cuda/monoid_batch.cu:596-671, cuda/nfa_tc_evolution.cu:943-1013); long DFA
+ * strings are chained by the decoupled look-back of monoid_r3_kernel
+ * (cuda/monoid_scan.cu:369-451), long NFA strings by speculative chunks with
+ * checkpoints and an exact fix-up pass.
+ * ------------------------------------------------------------------------ */
+typedef struct trex_mix trex_mix;
+
+#define TREX_MIX_DFA 0
+#define TREX_MIX_NFA 1
+
+/* kinds[i]: TREX_MIX_DFA (handles[i] is a trex_dfa* of <= 16 states with a
+ * 32-byte symbol window) or TREX_MIX_NFA (a trex_nfa* with a window).
+ * handles may be NULL: the object then only plans (no device needed). */
+int trex_mix_create(const int32_t* kinds, const void* const* handles, int32_t n_patterns, trex_mix** out);
+int trex_mix_destroy(trex_mix* m);
+/* String i = bytes[begin[i] .. end[i]) is matched against pattern
+ * pattern_of[i]. Strings longer than dfa_split / nfa_split bytes are cut into
+ * chunks of dfa_chunk (multiple of 16) / nfa_chunk (multiple of ck) bytes;
+ * NFA chunks record their state set every ck bytes (ck multiple of 16). */
+int trex_mix_plan(trex_mix* m, const int64_t* begin, const int64_t* end, const int32_t* pattern_of, int64_t n_strings,
+                  int64_t dfa_split, int64_t dfa_chunk, int64_t nfa_split, int64_t nfa_chunk, int32_t ck);
+int trex_mix_info(const trex_mix* m, int64_t* n_items, int64_t* n_batches, int64_t* n_chunks, int64_t* n_fixups);
+/* Copies the plan (items, then batches in queue order); any pointer may be NULL. */
+int trex_mix_layout(const trex_mix* m, int64_t* ibeg, int64_t* iend, int32_t* ia, int32_t* ib, uint32_t* iflags,
+                    int32_t* bfirst, int32_t* bcount, int32_t* bpattern, int32_t* brole);
+/* One launch sequence on `stream` (a fill kernel for the queue counters, then
+ * the persistent kernel): d_out_mask[i] = DFA final state or NFA final set,
+ * d_out_accept[i] = accept bit, for every planned string. max_spins bounds
+ * every wait (0: default). */
+int trex_mix_run(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, uint8_t* d_out_accept, uint32_t max_spins,
+                 void* stream);
+/* Persistent workgroups of later runs (0: two per compute unit). Fewer leave
+ * compute units to kernels on other streams. */
+int trex_mix_set_grid(trex_mix* m, int64_t workgroups);
+/* Error word of the last run (0: no wait expired). Synchronises `stream`. */
+int trex_mix_error(trex_mix* m, uint32_t* error, void* stream);
+/* Captures trex_mix_run into a HIP graph (non-default stream), replayed by
+ * trex_mix_replay with the same buffers. */
+int trex_mix_capture(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, uint8_t* d_out_accept,
+                     uint32_t max_spins, void* stream);
+int trex_mix_replay(trex_mix* m, void* stream);
 
 /* ------------------------------------------------------------------------
  * Long strings: chunk plans and the decoupled look-back scan (hot path (b)).

@@ -99,3 +99,14 @@ def pnfa_forward(T, start, accept, symbols, col_scale=None, T_last=None, threads
     if rc:
         raise ValueError(f"oracle_pnfa_forward rc={rc}")
     return ll, lm
+
+
+def gen_symbols(start: int, n: int, seed: int, lo: int = 0x40, span: int = 32) -> np.ndarray:
+    """Bytes [start, start + n) of the device-generated synthetic input."""
+    out = np.empty(n, np.uint8)
+    f = lib().oracle_gen_symbols
+    f.restype = None
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32]
+    if n:
+        f(out.ctypes.data, int(start), int(n), int(seed), int(lo), int(span))
+    return out

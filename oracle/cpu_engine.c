@@ -218,3 +218,26 @@ int oracle_pnfa_forward(const double* Tt, const double* Tt_last, const double* s
         if (tid[t]) pthread_join(tid[t], NULL);
     return 0;
 }
+
+/*
+ *  oracle_gen_symbols: bytes [start, start + n) of the synthetic input that
+ *    trex_gen_symbols writes on the device: byte i = lo + (r & (span - 1)),
+ *    r = byte (i % 8) of splitmix64(seed ^ ((i / 8) * 0x9E3779B97F4A7C15)).
+ */
+static uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+void oracle_gen_symbols(uint8_t* out, int64_t start, int64_t n, uint64_t seed, uint32_t lo, uint32_t span) {
+    int64_t i = start;
+    const int64_t stop = start + n;
+    while (i < stop) {
+        const uint64_t w = (uint64_t)i >> 3;
+        const uint64_t h = splitmix64(seed ^ (w * 0x9E3779B97F4A7C15ull));
+        for (int k = (int)(i & 7); k < 8 && i < stop; ++k, ++i)
+            out[i - start] = (uint8_t)(lo + ((h >> (8 * k)) & (span - 1)));
+    }
+}

@@ -338,6 +338,32 @@ def test_gpu_nonstochastic_and_edges(gpu_lib):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("waves", [4, 8])
+def test_gpu_ragged_sequences(gpu_lib, waves):
+    """Sequences of different lengths (0, 1, 2, ..., 400) in one launch: each
+    matches the FP64 oracle run on its own length."""
+    rng = np.random.default_rng(19)
+    T, start, accept = dirichlet_pnfa(rng, 32)
+    lengths = np.concatenate([[0, 1, 2, 3, 64, 65, 66], rng.integers(0, 400, size=300)])
+    offsets = np.zeros(len(lengths) + 1, np.int64)
+    np.cumsum(lengths, out=offsets[1:])
+    data = rng.integers(0, 32, size=int(offsets[-1])).astype(np.uint8)
+    bad = np.flatnonzero(lengths > 10)[:3]
+    data[offsets[bad] + 5] = 40                        # outside the alphabet
+    eng = _engine(T, start, accept)
+    eng.set_waves(waves)
+    ll, lm = eng.loglik_ragged(data, offsets)
+    for L in np.unique(lengths):
+        idx = np.flatnonzero(lengths == L)
+        sy = np.stack([data[offsets[i]:offsets[i + 1]] for i in idx]) if L else np.zeros((len(idx), 0), np.uint8)
+        ref_ll, ref_lm = oprob.forward_loglik(T, start, accept, sy)
+        fin = np.isfinite(ref_ll)
+        assert np.array_equal(np.isfinite(ll[idx]), fin), L
+        assert np.all(_rel(ll[idx][fin], ref_ll[fin]) <= SPLIT_TOL), L
+    assert np.all(ll[bad] == -np.inf)
+
+
+@pytest.mark.gpu
 def test_gpu_device_pointer_dispatch(gpu_lib):
     from trex_amd.device import DeviceBuffer, Stream
     rng = np.random.default_rng(16)

@@ -1,0 +1,80 @@
+"""Continuous-batching server: the HBM arena allocator (CPU) and, on the GPU,
+that every request's result is the same served in a mix of concurrent
+requests as computed alone (bit-exact for DFA / NFA results, and also for the
+probabilistic log-likelihoods, whose per-sequence arithmetic does not depend on
+the other sequences of a launch)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+
+def test_arena_first_fit_and_merge():
+    from trex_amd.serve import ALIGN, Arena
+
+    class _Fake(Arena):
+        def __init__(self, n):   # no device buffer
+            self.size = n
+            self.free = [(0, n)]
+            self.used = 0
+
+    a = _Fake(10 * ALIGN)
+    x = a.alloc(1)
+    y = a.alloc(3 * ALIGN)
+    z = a.alloc(2 * ALIGN + 1)
+    assert (x, y, z) == (0, ALIGN, 4 * ALIGN)
+    assert a.alloc(4 * ALIGN) == -1            # 3 * ALIGN left
+    a.release(y, 3 * ALIGN)
+    assert a.alloc(4 * ALIGN) == -1            # two separate holes
+    a.release(z, 2 * ALIGN + 1)
+    assert a.free == [(ALIGN, 9 * ALIGN)]      # merged with the tail
+    assert a.alloc(9 * ALIGN) == ALIGN and a.used == 10 * ALIGN
+
+
+def _requests(rng, n_patterns, kinds, count, max_strings, median, sigma, cap):
+    from trex_amd.serve import Request
+    reqs = []
+    for _ in range(count):
+        p = int(rng.integers(0, n_patterns))
+        n = int(rng.integers(1, max_strings + 1))
+        lengths = np.minimum(np.floor(np.exp(rng.normal(np.log(median), sigma, size=n))), cap).astype(np.int64)
+        offsets = np.zeros(n + 1, np.int64)
+        np.cumsum(lengths, out=offsets[1:])
+        hi = 32 if kinds[p] == "prob" else 0x60
+        lo = 0 if kinds[p] == "prob" else 0x40
+        data = rng.integers(lo, hi, size=int(offsets[-1]), dtype=np.uint8)
+        reqs.append(Request(p, data, offsets))
+    return reqs
+
+
+@pytest.mark.gpu
+def test_gpu_served_equals_alone(gpu_lib):
+    from trex_amd.engine import DfaEngine, NfaEngine, ProbNfaEngine
+    from trex_amd.serve import Server
+    from trex_amd.tables import dfa_byte_table
+    from trex_amd.workloads import ALPHABET_32, dirichlet_pnfa, random_dfa_pattern, random_nfa64
+
+    rng = np.random.default_rng(21)
+    engines = [DfaEngine(dfa_byte_table(random_dfa_pattern(rng, ALPHABET_32)[1])) for _ in range(3)]
+    engines += [NfaEngine.from_matrices(random_nfa64(rng)) for _ in range(2)]
+    engines += [ProbNfaEngine(*dirichlet_pnfa(rng, 32))]
+    kinds = ["mix"] * 5 + ["prob"]
+    reqs = _requests(rng, len(engines), kinds, 60, 64, 512, 2.0, 300_000)
+    server = Server(engines, hbm_budget=256 << 20, small_bytes=64 << 10)
+    with server:
+        for r in reqs:
+            server.submit(r)
+        for r in reqs:
+            assert r.wait(60), "request not served"
+    assert server.ticks["small"] >= 1 and server.ticks["bulk"] >= 1
+    for r in reqs:
+        e = engines[r.pattern]
+        if isinstance(e, ProbNfaEngine):
+            ll, lm = e.loglik_ragged(r.data, r.offsets)
+            assert np.array_equal(r.result[0], ll) and np.array_equal(r.result[1], lm)
+        else:
+            final, acc, _, _ = e.match_arrays(r.data, r.offsets)
+            assert np.array_equal(r.result[0].astype(final.dtype), final), r.pattern
+            assert np.array_equal(r.result[1], acc)
+        assert r.arrival <= r.admitted <= r.launched <= r.done

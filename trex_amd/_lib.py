@@ -32,6 +32,7 @@ vpp = POINTER(c_void_p)
 _SIGNATURES = {
     "trex_device_check": (c_int, []),
     "trex_last_error": (c_int, [c_char_p, c_int]),
+    "trex_device_info": (c_int, [c_char_p, c_int, intp, i64p]),
     "trex_malloc": (c_int, [vpp, c_size_t]),
     "trex_free": (c_int, [c_void_p]),
     "trex_host_alloc": (c_int, [vpp, c_size_t]),
@@ -47,6 +48,10 @@ _SIGNATURES = {
     "trex_event_destroy": (c_int, [c_void_p]),
     "trex_event_record": (c_int, [c_void_p, c_void_p]),
     "trex_event_elapsed_ms": (c_int, [c_void_p, c_void_p, f32p]),
+    "trex_event_query": (c_int, [c_void_p]),
+    "trex_event_sync": (c_int, [c_void_p]),
+    "trex_stream_wait_event": (c_int, [c_void_p, c_void_p]),
+    "trex_gen_symbols": (c_int, [c_void_p, c_int64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, c_void_p]),
     # DFA
     "trex_dfa_create": (c_int, [u8p, u8p, c_int, c_int, vpp]),
     "trex_dfa_destroy": (c_int, [c_void_p]),
@@ -75,7 +80,21 @@ _SIGNATURES = {
     "trex_pnfa_set_mode": (c_int, [c_void_p, c_int]),
     "trex_pnfa_rounded": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "trex_pnfa_loglik": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
+    "trex_pnfa_loglik_ranges": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
+                                        c_void_p]),
     "trex_pnfa_loglik_host": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, f32p, f32p]),
+    # mixed batches
+    "trex_mix_create": (c_int, [c_void_p, c_void_p, c_int, vpp]),
+    "trex_mix_destroy": (c_int, [c_void_p]),
+    "trex_mix_plan": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64,
+                              c_int]),
+    "trex_mix_info": (c_int, [c_void_p, i64p, i64p, i64p, i64p]),
+    "trex_mix_layout": (c_int, [c_void_p] + [c_void_p] * 9),
+    "trex_mix_run": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_uint32, c_void_p]),
+    "trex_mix_error": (c_int, [c_void_p, POINTER(ctypes.c_uint32), c_void_p]),
+    "trex_mix_set_grid": (c_int, [c_void_p, c_int64]),
+    "trex_mix_capture": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_uint32, c_void_p]),
+    "trex_mix_replay": (c_int, [c_void_p, c_void_p]),
     # chunk plans
     "trex_plan_create": (c_int, [c_void_p, c_void_p, c_int64, c_int64, vpp]),
     "trex_plan_destroy": (c_int, [c_void_p]),
@@ -144,6 +163,19 @@ def call(name: str, *args) -> int:
     rc = getattr(lib(), name)(*args)
     check(rc, name)
     return rc
+
+
+def device_info() -> dict:
+    """Architecture, compute units and HBM bytes of the current device."""
+    buf = ctypes.create_string_buffer(128)
+    cus, hbm = c_int(), c_int64()
+    call("trex_device_info", buf, len(buf), ctypes.byref(cus), ctypes.byref(hbm))
+    return {"arch": buf.value.decode(errors="replace"), "cus": cus.value, "hbm_bytes": hbm.value}
+
+
+def device_name() -> str:
+    d = device_info()
+    return f"{d['arch'].split(':')[0]}/{d['cus']}cu"
 
 
 def device_available() -> bool:

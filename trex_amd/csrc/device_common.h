@@ -1,0 +1,175 @@
+// Device helpers shared by the gfx950 kernels (DFA walks, look-back scan,
+// Boolean NFA, mixed work queue). Header-only, internal linkage.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace trexdev {
+
+// ---------------------------------------------------------------------------
+// Byte streaming
+// ---------------------------------------------------------------------------
+
+// Number of bytes from `p` to the next 16-byte boundary of the address.
+__device__ __forceinline__ int64_t head_len(const uint8_t* bytes, int64_t p) {
+    return (int64_t)((16u - ((uint32_t)(uintptr_t)(bytes + p) & 15u)) & 15u);
+}
+
+// Streams v[0 .. nvec) through `consume` in order with two blocks of PF
+// 16-byte vectors: the loads of block i + 1 are issued before block i is
+// consumed, so PF loads are in flight while the table walk runs.
+template <int PF, class F>
+__device__ __forceinline__ void stream_body(const uint4* __restrict__ v, int64_t nvec, F&& consume) {
+    int64_t i = 0;
+    if (nvec >= PF) {
+        uint4 a[PF];
+#pragma unroll
+        for (int k = 0; k < PF; ++k) a[k] = v[k];
+        for (i = PF; i + PF <= nvec; i += PF) {
+            uint4 b[PF];
+#pragma unroll
+            for (int k = 0; k < PF; ++k) b[k] = v[i + k];
+#pragma unroll
+            for (int k = 0; k < PF; ++k) consume(a[k]);
+#pragma unroll
+            for (int k = 0; k < PF; ++k) a[k] = b[k];
+        }
+#pragma unroll
+        for (int k = 0; k < PF; ++k) consume(a[k]);
+    }
+    for (; i < nvec; ++i) consume(v[i]);
+}
+
+// Copy `n` bytes (multiple of 16) from global to LDS with the whole block.
+__device__ __forceinline__ void stage_table(uint8_t* __restrict__ dst, const uint8_t* __restrict__ src, int n) {
+    const uint4* s4 = reinterpret_cast<const uint4*>(src);
+    uint4* d4 = reinterpret_cast<uint4*>(dst);
+    for (int i = threadIdx.x; i < (n >> 4); i += blockDim.x) d4[i] = s4[i];
+}
+
+// ---------------------------------------------------------------------------
+// 32-byte symbol windows
+// ---------------------------------------------------------------------------
+
+// Records whether any body byte lies outside the aligned window [base, base+32)
+// (win4 = base replicated in four bytes), two bit operations per word.
+struct WinCheck {
+    uint32_t any_hi;   // OR of the top three bits of every body byte
+    uint32_t all;      // AND of every body word
+    __device__ __forceinline__ void init() { any_hi = 0u; all = 0xffffffffu; }
+    __device__ __forceinline__ void add(uint32_t x) { any_hi |= x & 0xe0e0e0e0u; all &= x; }
+    // true iff some body byte had top bits != the window's
+    __device__ __forceinline__ bool outside(uint32_t win4) const {
+        return ((any_hi & ~win4) | (win4 & ~all)) != 0u;
+    }
+};
+
+// Pair index sym0 | sym1 << 5 of the low / high two bytes of a word (sym = byte & 31).
+__device__ __forceinline__ uint32_t pair_lo(uint32_t x) { return (x & 31u) | (((x >> 8) & 31u) << 5); }
+__device__ __forceinline__ uint32_t pair_hi(uint32_t x) { return ((x >> 16) & 31u) | (((x >> 24) & 31u) << 5); }
+
+// ---------------------------------------------------------------------------
+// 16-state functions
+// ---------------------------------------------------------------------------
+
+struct Fn16 {
+    uint32_t d[4];   // byte s (s = 0..15) of the 16-byte vector = image of state s
+};
+
+__device__ __forceinline__ Fn16 fn_identity() {
+    return Fn16{{0x03020100u, 0x07060504u, 0x0b0a0908u, 0x0f0e0d0cu}};
+}
+
+__device__ __forceinline__ Fn16 fn_const(uint32_t s) {
+    const uint32_t c = s * 0x01010101u;
+    return Fn16{{c, c, c, c}};
+}
+
+// Bytes g[f.byte[i]] for the four entries packed in `f` (values < 16).
+__device__ __forceinline__ uint32_t fn_gather(uint32_t f, const Fn16& g) {
+    const uint32_t sel = f & 0x07070707u;
+    const uint32_t lo = __builtin_amdgcn_perm(g.d[1], g.d[0], sel);   // g[0..7]
+    const uint32_t hi = __builtin_amdgcn_perm(g.d[3], g.d[2], sel);   // g[8..15]
+    const uint32_t pick = 0x03020100u | ((f & 0x08080808u) >> 1);     // byte i of hi when bit 3 is set
+    return __builtin_amdgcn_perm(hi, lo, pick);
+}
+
+// (g o f): apply f, then g.
+__device__ __forceinline__ Fn16 fn_then(const Fn16& f, const Fn16& g) {
+    return Fn16{{fn_gather(f.d[0], g), fn_gather(f.d[1], g), fn_gather(f.d[2], g), fn_gather(f.d[3], g)}};
+}
+
+__device__ __forceinline__ bool fn_is_const(const Fn16& f) {
+    const uint32_t c = (f.d[0] & 0xffu) * 0x01010101u;
+    return (f.d[0] == c) & (f.d[1] == c) & (f.d[2] == c) & (f.d[3] == c);
+}
+
+__device__ __forceinline__ uint32_t fn_apply(const Fn16& f, uint32_t s) {
+    return (f.d[s >> 2] >> ((s & 3u) * 8u)) & 0xffu;
+}
+
+__device__ __forceinline__ uint64_t fn_pack(const Fn16& f) {
+    uint64_t out = 0;
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) out |= (uint64_t)fn_apply(f, s) << (4 * s);
+    return out;
+}
+
+__device__ __forceinline__ uint32_t nib(uint64_t f, uint32_t s) { return (uint32_t)(f >> (4 * s)) & 15u; }
+
+// (g o f) on 64-bit nibble functions.
+__device__ __forceinline__ uint64_t nib_then(uint64_t f, uint64_t g) {
+    uint64_t out = 0;
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) out |= (uint64_t)nib(g, nib(f, s)) << (4 * s);
+    return out;
+}
+
+__device__ __forceinline__ Fn16 load_row(const uint8_t* __restrict__ table, uint32_t row) {
+    const uint4 v = *reinterpret_cast<const uint4*>(table + row * 16u);
+    return Fn16{{v.x, v.y, v.z, v.w}};
+}
+
+// ---------------------------------------------------------------------------
+// Cross-wave publication (agent scope)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t load_acquire(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_release(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// Boolean NFA-64 step over nibble-sliced successor tables
+// ---------------------------------------------------------------------------
+
+// One step through the table of class k (256 uint64 entries, entry (p, v) =
+// union of the successors of the states 4p + i for the bits i of v), placed
+// in LDS as the host lays it out: entry (p, v) at row p ^ h, slot v ^ x with
+// h = (k >> 4) & 1, x = k & 15, so that lanes looking up the same nibble value
+// in different classes hit different banks. XOR-ing every nibble of the state
+// with x gives the slot directly; rows of even and odd p move by +h and -h.
+__device__ __forceinline__ uint64_t nfa_step(const uint64_t* __restrict__ tab, uint32_t k, uint64_t s) {
+    const uint32_t x = (k & 15u) * 0x11111111u;
+    const uint32_t lo = (uint32_t)s ^ x, hi = (uint32_t)(s >> 32) ^ x;
+    const int h16 = (int)((k >> 4) & 1u) * 16;
+    const uint64_t* te = tab + k * 256u + h16;   // even p: row p + h
+    const uint64_t* to = tab + k * 256u - h16;   // odd p: row p - h
+    uint64_t r0 = 0, r1 = 0;
+#pragma unroll
+    for (int p = 0; p < 8; p += 2) {
+        r0 |= te[p * 16 + ((lo >> (4 * p)) & 15u)];
+        r1 |= to[(p + 1) * 16 + ((lo >> (4 * p + 4)) & 15u)];
+    }
+#pragma unroll
+    for (int p = 0; p < 8; p += 2) {
+        r0 |= te[(8 + p) * 16 + ((hi >> (4 * p)) & 15u)];
+        r1 |= to[(9 + p) * 16 + ((hi >> (4 * p + 4)) & 15u)];
+    }
+    return r0 | r1;
+}
+
+}  // namespace trexdev

@@ -33,6 +33,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "device_common.h"
+
 #define TREX_BLOCK 256
 // The string kernels run one wave per 64 strings; four waves per SIMD must stay
 // resident, which caps them at 128 VGPRs.
@@ -40,19 +42,7 @@
 
 namespace {
 
-struct WinCheck {
-    uint32_t any_hi;   // OR of the top three bits of every body byte
-    uint32_t all;      // AND of every body word
-    __device__ __forceinline__ void init() { any_hi = 0u; all = 0xffffffffu; }
-    __device__ __forceinline__ void add(uint32_t x) { any_hi |= x & 0xe0e0e0e0u; all &= x; }
-    // true iff some body byte had top bits != the window's
-    __device__ __forceinline__ bool outside(uint32_t win4) const {
-        return ((any_hi & ~win4) | (win4 & ~all)) != 0u;
-    }
-};
-
-__device__ __forceinline__ uint32_t pair_lo(uint32_t x) { return (x & 31u) | (((x >> 8) & 31u) << 5); }
-__device__ __forceinline__ uint32_t pair_hi(uint32_t x) { return ((x >> 16) & 31u) | (((x >> 24) & 31u) << 5); }
+using namespace trexdev;
 
 // Pair-step policies over the LDS pair table `t`.
 struct StepK2W {
@@ -106,43 +96,6 @@ __device__ __forceinline__ uint32_t k1_vec(const uint8_t* __restrict__ t1, uint3
     s = k1_word<STRIDE_LOG2>(t1, s, v.z, stride);
     s = k1_word<STRIDE_LOG2>(t1, s, v.w, stride);
     return s;
-}
-
-// Copy `n` bytes (multiple of 16) from global to LDS with the whole block.
-__device__ __forceinline__ void stage_table(uint8_t* __restrict__ dst, const uint8_t* __restrict__ src, int n) {
-    const uint4* s4 = reinterpret_cast<const uint4*>(src);
-    uint4* d4 = reinterpret_cast<uint4*>(dst);
-    for (int i = threadIdx.x; i < (n >> 4); i += blockDim.x) d4[i] = s4[i];
-}
-
-// Number of bytes from `p` to the next 16-byte boundary of the address.
-__device__ __forceinline__ int64_t head_len(const uint8_t* bytes, int64_t p) {
-    return (int64_t)((16u - ((uint32_t)(uintptr_t)(bytes + p) & 15u)) & 15u);
-}
-
-// Streams v[0 .. nvec) through `consume` in order with two blocks of PF
-// 16-byte vectors: the loads of block i + 1 are issued before block i is
-// consumed, so PF loads are in flight while the table walk runs.
-template <int PF, class F>
-__device__ __forceinline__ void stream_body(const uint4* __restrict__ v, int64_t nvec, F&& consume) {
-    int64_t i = 0;
-    if (nvec >= PF) {
-        uint4 a[PF];
-#pragma unroll
-        for (int k = 0; k < PF; ++k) a[k] = v[k];
-        for (i = PF; i + PF <= nvec; i += PF) {
-            uint4 b[PF];
-#pragma unroll
-            for (int k = 0; k < PF; ++k) b[k] = v[i + k];
-#pragma unroll
-            for (int k = 0; k < PF; ++k) consume(a[k]);
-#pragma unroll
-            for (int k = 0; k < PF; ++k) a[k] = b[k];
-        }
-#pragma unroll
-        for (int k = 0; k < PF; ++k) consume(a[k]);
-    }
-    for (; i < nvec; ++i) consume(v[i]);
 }
 
 template <class Step, int PF>

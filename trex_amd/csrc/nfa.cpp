@@ -17,22 +17,6 @@ constexpr int kNfaBlock = 512;
 constexpr int kMaxSymbols = 62;   // (62 + 2) classes x 2 KB of nibble tables fit one workgroup's LDS
 }
 
-struct trex_nfa {
-    int n_symbols = 0;
-    uint64_t start = 0, accept = 0;
-    int window_base = -1;
-    int variant = TREX_NFA_BYTES;
-    int n_classes = 0, n_classes_win = 0;
-    uint64_t* d_tab = nullptr;      // [n_symbols + 2][16][16]: symbols, reject, skip
-    uint8_t* d_cls = nullptr;       // [256] byte -> class
-    uint64_t* d_tab_win = nullptr;  // [34][16][16]: window bytes base+0..31, reject, skip
-    uint8_t* d_cls_win = nullptr;   // [256]
-    DevBuf in_bytes, in_offsets, out_mask, out_accept;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    std::mutex mu;
-};
-
 namespace {
 
 // Nibble tables of one class from its 64 successor masks.

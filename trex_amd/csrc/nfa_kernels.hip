@@ -25,40 +25,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "device_common.h"
+
 #define TREX_NFA_BLOCK 512
 
 namespace {
 
-struct WinCheck {
-    uint32_t any_hi, all;
-    __device__ __forceinline__ void init() { any_hi = 0u; all = 0xffffffffu; }
-    __device__ __forceinline__ void add(uint32_t x) { any_hi |= x & 0xe0e0e0e0u; all &= x; }
-    __device__ __forceinline__ bool outside(uint32_t win4) const { return ((any_hi & ~win4) | (win4 & ~all)) != 0u; }
-};
-
-// One step through the table of class k (256 uint64 entries, placed as the
-// host lays them out: entry (p, v) at row p ^ h, slot v ^ x with
-// h = (k >> 4) & 1, x = k & 15). XOR-ing every nibble of the state with x
-// gives the slot directly; rows of even and odd p move by +h and -h.
-__device__ __forceinline__ uint64_t nfa_step(const uint64_t* __restrict__ tab, uint32_t k, uint64_t s) {
-    const uint32_t x = (k & 15u) * 0x11111111u;
-    const uint32_t lo = (uint32_t)s ^ x, hi = (uint32_t)(s >> 32) ^ x;
-    const int h16 = (int)((k >> 4) & 1u) * 16;
-    const uint64_t* te = tab + k * 256u + h16;   // even p: row p + h
-    const uint64_t* to = tab + k * 256u - h16;   // odd p: row p - h
-    uint64_t r0 = 0, r1 = 0;
-#pragma unroll
-    for (int p = 0; p < 8; p += 2) {
-        r0 |= te[p * 16 + ((lo >> (4 * p)) & 15u)];
-        r1 |= to[(p + 1) * 16 + ((lo >> (4 * p + 4)) & 15u)];
-    }
-#pragma unroll
-    for (int p = 0; p < 8; p += 2) {
-        r0 |= te[(8 + p) * 16 + ((hi >> (4 * p)) & 15u)];
-        r1 |= to[(9 + p) * 16 + ((hi >> (4 * p + 4)) & 15u)];
-    }
-    return r0 | r1;
-}
+using namespace trexdev;
 
 template <bool WINDOW>
 __device__ __forceinline__ void nfa_kernel(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,

@@ -222,6 +222,25 @@ int trex_pnfa_rounded(const trex_pnfa* h, int mode, float* t_rounded, float* col
     return TREX_OK;
 }
 
+namespace {
+
+int pnfa_launch(trex_pnfa* h, const uint8_t* d_symbols, const int64_t* d_begin, const int64_t* d_end, int64_t n_seq,
+                int L, double* d_ll, double* d_logmass, void* stream) {
+    const int64_t groups = (n_seq + kSeqPerGroup - 1) / kSeqPerGroup;
+    if (groups > 0x7fffffff) return fail(TREX_ERR_INVALID, "too many sequences");
+    const bool f16 = uses_f16(h->mode);
+    const uint16_t* tlp = f16 ? h->d_q16 : h->d_qb16;
+    const float* scale = f16 ? h->d_scale16 : h->d_scaleb16;
+    int n_sym = h->n_symbols, e0 = h->e0;
+    void* args[] = {&tlp, &scale, &h->d_t32, &n_sym, &h->d_s0, &e0, &h->d_acc, &d_symbols, &d_begin, &d_end,
+                    &n_seq, &L, &d_ll, &d_logmass};
+    char name[48];
+    std::snprintf(name, sizeof(name), "pnfa64_%s_w%d", mode_name(h->mode), h->waves);
+    return launch(name, (unsigned)groups, 64u * (unsigned)h->waves, 0, (hipStream_t)stream, args);
+}
+
+}  // namespace
+
 int trex_pnfa_loglik(trex_pnfa* h, const uint8_t* d_symbols, int64_t n_seq, int64_t length, double* d_ll,
                      double* d_logmass, void* stream) {
     if (!h) return fail(TREX_ERR_INVALID, "null handle");
@@ -229,16 +248,16 @@ int trex_pnfa_loglik(trex_pnfa* h, const uint8_t* d_symbols, int64_t n_seq, int6
     if (length > 0x7fffffff) return fail(TREX_ERR_INVALID, "length %lld beyond 2^31-1", (long long)length);
     if (n_seq == 0) return TREX_OK;
     if (!d_ll || !d_logmass || (length > 0 && !d_symbols)) return fail(TREX_ERR_INVALID, "null buffer");
-    const int64_t groups = (n_seq + kSeqPerGroup - 1) / kSeqPerGroup;
-    if (groups > 0x7fffffff) return fail(TREX_ERR_INVALID, "too many sequences");
-    const bool f16 = uses_f16(h->mode);
-    const uint16_t* tlp = f16 ? h->d_q16 : h->d_qb16;
-    const float* scale = f16 ? h->d_scale16 : h->d_scaleb16;
-    int n_sym = h->n_symbols, e0 = h->e0, L = (int)length;
-    void* args[] = {&tlp, &scale, &h->d_t32, &n_sym, &h->d_s0, &e0, &h->d_acc, &d_symbols, &n_seq, &L, &d_ll, &d_logmass};
-    char name[48];
-    std::snprintf(name, sizeof(name), "pnfa64_%s_w%d", mode_name(h->mode), h->waves);
-    return launch(name, (unsigned)groups, 64u * (unsigned)h->waves, 0, (hipStream_t)stream, args);
+    return pnfa_launch(h, d_symbols, nullptr, nullptr, n_seq, (int)length, d_ll, d_logmass, stream);
+}
+
+int trex_pnfa_loglik_ranges(trex_pnfa* h, const uint8_t* d_symbols, const int64_t* d_begin, const int64_t* d_end,
+                            int64_t n_seq, double* d_ll, double* d_logmass, void* stream) {
+    if (!h) return fail(TREX_ERR_INVALID, "null handle");
+    if (n_seq < 0) return fail(TREX_ERR_INVALID, "negative size");
+    if (n_seq == 0) return TREX_OK;
+    if (!d_ll || !d_logmass || !d_begin || !d_end || !d_symbols) return fail(TREX_ERR_INVALID, "null buffer");
+    return pnfa_launch(h, d_symbols, d_begin, d_end, n_seq, 0, d_ll, d_logmass, stream);
 }
 
 int trex_pnfa_loglik_host(trex_pnfa* h, const uint8_t* symbols, int64_t n_seq, int64_t length, double* ll,

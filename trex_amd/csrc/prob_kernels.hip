@@ -124,18 +124,26 @@ __device__ __forceinline__ void split8(const float (&x)[8], typename Lp<BF16>::v
 template <bool BF16, bool SPLIT, int SPW>
 __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __restrict__ colscale,
                              const float* __restrict__ t32, int n_sym, const float* __restrict__ s0, int e0,
-                             const float* __restrict__ accw, const uint8_t* __restrict__ symbols, int64_t n_seq,
+                             const float* __restrict__ accw, const uint8_t* __restrict__ symbols,
+                             const int64_t* __restrict__ sbegin, const int64_t* __restrict__ send, int64_t n_seq,
                              int L, double* __restrict__ out_ll, double* __restrict__ out_logmass)
 {
     typedef typename Lp<BF16>::vec vec;
     constexpr int kBlock = 64 * (kMaxSymbols / SPW);
     constexpr int kTps = kBlock / kSeqPerGroup;   // threads per sequence in the last step
     constexpr int kStage = kChunk + 1;            // staged symbols per sequence: a chunk plus the next one
-    __shared__ __attribute__((aligned(16))) float state[2][kSeqPerGroup][kRow];
+    constexpr uint8_t kDone = 0xfe;               // no rounded step at this position (the sequence has ended)
+    constexpr uint8_t kBad = 0xff;                // symbol outside the alphabet: the state empties
+    // One state row per sequence, updated in place: a row is read and then
+    // rewritten by the one wave that owns its sequence's symbol at that step.
+    __shared__ __attribute__((aligned(16))) float state[kSeqPerGroup][kRow];
     __shared__ __attribute__((aligned(16))) float dscale[kMaxSymbols][kStates];
     __shared__ uint8_t syms[kSeqPerGroup][kChunk + 4];
     __shared__ int32_t estep[2][kSeqPerGroup];   // exponent taken out at a step, by step parity
     __shared__ int32_t expo[kSeqPerGroup];
+    __shared__ int64_t sq_beg[kSeqPerGroup];
+    __shared__ int32_t sq_len[kSeqPerGroup];
+    __shared__ int32_t max_len;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -164,15 +172,34 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
                 }
             }
     }
+    if (tid == 0) max_len = 0;
+    __syncthreads();
+    if (tid < kSeqPerGroup) {
+        int64_t b = 0, len = 0;
+        if (tid < nseq) {
+            const int64_t j = seq0 + tid;
+            b = sbegin ? sbegin[j] : j * (int64_t)L;
+            len = sbegin ? send[j] - b : (int64_t)L;
+        }
+        sq_beg[tid] = b;
+        sq_len[tid] = (int32_t)len;
+        atomicMax(&max_len, (int32_t)len);
+    }
     auto stage = [&](int t) {
         for (int i = tid; i < kSeqPerGroup * kStage; i += kBlock) {
             const int s = i / kStage, k = i % kStage;
-            syms[s][k] = (s < nseq && t + k < L) ? symbols[(seq0 + s) * (int64_t)L + t + k] : (uint8_t)255;
+            uint8_t v = kDone;
+            if (t + k < sq_len[s] - 1) {
+                const uint8_t c = symbols[sq_beg[s] + t + k];
+                v = c < n_sym ? c : kBad;
+            }
+            syms[s][k] = v;
         }
     };
     for (int i = tid; i < kMaxSymbols * kStates; i += kBlock)
         dscale[i / kStates][i % kStates] = i < n_sym * kStates ? colscale[i] : 1.f;
     if (tid < 2 * kSeqPerGroup) estep[tid / kSeqPerGroup][tid % kSeqPerGroup] = 0;
+    __syncthreads();
     stage(0);
     __syncthreads();
     // The state kept in LDS is already multiplied by the column factors of the
@@ -180,13 +207,13 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
     for (int i = tid; i < kSeqPerGroup * kStates; i += kBlock) {
         const int s = i / kStates, k = i % kStates;
         const int c0 = syms[s][0];
-        state[0][s][k] = s0[k] * ((L > 1 && c0 < n_sym) ? dscale[c0][k] : 1.f);
+        state[s][k] = s0[k] * (c0 < n_sym ? dscale[c0][k] : 1.f);
     }
     int expo_home = e0;   // wave 0, lane j: log2-scale of sequence j
+    const int steps = max_len - 1;
     __syncthreads();
 
-    int cur = 0;
-    for (int t = 0; t < L - 1; ++t) {
+    for (int t = 0; t < steps; ++t) {
         if (t % kChunk == 0 && t > 0) {
             stage(t);
             __syncthreads();
@@ -194,11 +221,12 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
         const int par = t & 1;
         const int tc = t % kChunk;
         if (wave == 0 && t > 0) expo_home += estep[par ^ 1][lane];
-        const int cj = lane < nseq ? (int)syms[lane][tc] : 255;
-        // a symbol outside the alphabet empties the state
-        if (wave == 0 && lane < nseq && cj >= n_sym) {
-            float4* row = reinterpret_cast<float4*>(&state[cur ^ 1][lane][0]);
-            for (int k = 0; k < kStates / 4; ++k) row[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        const int cj = syms[lane][tc];
+        if (wave == 0 && cj >= n_sym) {
+            if (cj == kBad) {   // a symbol outside the alphabet empties the state
+                float4* row = reinterpret_cast<float4*>(&state[lane][0]);
+                for (int k = 0; k < kStates / 4; ++k) row[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
             estep[par][lane] = 0;
         }
         // One tile holds up to 16 sequences on any of this wave's symbols: the
@@ -206,7 +234,6 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
         // column's state sits only in its own symbol's k block.
         const uint64_t m = __ballot(cj < n_sym && cj / SPW == wave);
         const int cnt = __popcll(m);
-        const bool scale_next = t + 1 < L - 1;   // the next step also runs on the rounded matrices
         for (int g = 0; g * 16 < cnt; ++g) {
             const int n = g * 16 + (lane & 15);
             const bool valid = n < cnt;
@@ -217,14 +244,15 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
             vec bhi[2], blo[2];
 #pragma unroll
             for (int kb = 0; kb < 2; ++kb) {
-                const float4* src = reinterpret_cast<const float4*>(&state[cur][seq][kb * 32 + koff]);
+                const float4* src = reinterpret_cast<const float4*>(&state[seq][kb * 32 + koff]);
                 const float4 x0 = src[0], x1 = src[1];
                 const float x[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
                 split8<BF16, SPLIT>(x, bhi[kb], blo[kb]);
             }
             // column factors of the next symbol for this lane's 16 result rows
+            // (none when the next step is the last one or the sequence ends)
             float4 f[4];
-            const bool use_f = scale_next && cn < n_sym;
+            const bool use_f = cn < n_sym;
 #pragma unroll
             for (int rb = 0; rb < 4; ++rb)
                 f[rb] = use_f ? *reinterpret_cast<const float4*>(&dscale[cn][rb * 16 + rsub])
@@ -262,16 +290,15 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
                 for (int rb = 0; rb < 4; ++rb) {
                     const float4 out = make_float4(ldexpf(d[rb][0] * f[rb].x, -e2), ldexpf(d[rb][1] * f[rb].y, -e2),
                                                    ldexpf(d[rb][2] * f[rb].z, -e2), ldexpf(d[rb][3] * f[rb].w, -e2));
-                    *reinterpret_cast<float4*>(&state[cur ^ 1][seq][rb * 16 + rsub]) = out;
+                    *reinterpret_cast<float4*>(&state[seq][rb * 16 + rsub]) = out;
                 }
                 if ((lane >> 4) == 0) estep[par][seq] = e2;
             }
         }
         __syncthreads();
-        cur ^= 1;
     }
     if (wave == 0) {
-        if (L >= 2) expo_home += estep[(L - 2) & 1][lane];
+        if (steps >= 1) expo_home += estep[(steps - 1) & 1][lane];
         expo[lane] = expo_home;
     }
     __syncthreads();
@@ -281,9 +308,9 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
     const int s = tid / kTps, part = tid % kTps;
     double acc_a = 0.0, acc_m = 0.0;
     if (s < nseq) {
-        const float* st = state[cur][s];
-        if (L >= 1) {
-            const int c = symbols[(seq0 + s) * (int64_t)L + L - 1];
+        const float* st = state[s];
+        if (sq_len[s] >= 1) {
+            const int c = symbols[sq_beg[s] + sq_len[s] - 1];
             if (c < n_sym) {
                 for (int r = part * kRows; r < part * kRows + kRows; ++r) {
                     const float* trow = t32 + ((size_t)c * kStates + r) * kStates;
@@ -315,16 +342,19 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
 }  // namespace
 
 // Kernels: pnfa64_<mode>_w<W>, W = waves per workgroup (32 / symbols per wave);
-// one workgroup of 64 * W threads per 64 sequences.
+// one workgroup of 64 * W threads per 64 sequences. Sequence j is
+// symbols[sbegin[j] .. send[j]), or symbols[j * L .. (j + 1) * L) when sbegin
+// is null.
 #define PNFA_KERNEL(NAME, BF16, SPLIT, SPW)                                                                       \
     extern "C" __global__ void __launch_bounds__(64 * (32 / SPW))                                                  \
     __attribute__((amdgpu_waves_per_eu(32 / SPW / 4, 32 / SPW / 4)))                                              \
     NAME(const uint16_t* __restrict__ tlp, const float* __restrict__ colscale, const float* __restrict__ t32,     \
          int n_sym, const float* __restrict__ s0, int e0, const float* __restrict__ accw,                         \
-         const uint8_t* __restrict__ symbols, int64_t n_seq, int L, double* __restrict__ out_ll,                  \
+         const uint8_t* __restrict__ symbols, const int64_t* __restrict__ sbegin,                                 \
+         const int64_t* __restrict__ send, int64_t n_seq, int L, double* __restrict__ out_ll,                     \
          double* __restrict__ out_logmass) {                                                                      \
-        pnfa_forward<BF16, SPLIT, SPW>(tlp, colscale, t32, n_sym, s0, e0, accw, symbols, n_seq, L, out_ll,        \
-                                       out_logmass);                                                              \
+        pnfa_forward<BF16, SPLIT, SPW>(tlp, colscale, t32, n_sym, s0, e0, accw, symbols, sbegin, send, n_seq, L,  \
+                                       out_ll, out_logmass);                                                      \
     }
 
 #define PNFA_KERNELS(W, SPW)                                  \

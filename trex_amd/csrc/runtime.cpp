@@ -281,4 +281,44 @@ int trex_event_elapsed_ms(void* start, void* stop, float* ms) {
     return e == hipSuccess ? TREX_OK : hip_fail(e, "hipEventElapsedTime");
 }
 
+int trex_event_query(void* ev) {
+    const hipError_t e = hipEventQuery((hipEvent_t)ev);
+    if (e == hipSuccess) return 0;
+    if (e == hipErrorNotReady) return 1;
+    return hip_fail(e, "hipEventQuery");
+}
+
+int trex_event_sync(void* ev) {
+    TREX_HIP(hipEventSynchronize((hipEvent_t)ev), "hipEventSynchronize");
+    return TREX_OK;
+}
+
+int trex_stream_wait_event(void* stream, void* ev) {
+    TREX_HIP(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)ev, 0), "hipStreamWaitEvent");
+    return TREX_OK;
+}
+
+int trex_device_info(char* name, int n, int* cu_count, int64_t* hbm_bytes) {
+    int dev = 0;
+    TREX_HIP(hipGetDevice(&dev), "hipGetDevice");
+    hipDeviceProp_t prop;
+    TREX_HIP(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    if (name && n > 0) snprintf(name, (size_t)n, "%s", prop.gcnArchName);
+    if (cu_count) *cu_count = prop.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = (int64_t)prop.totalGlobalMem;
+    return TREX_OK;
+}
+
+int trex_gen_symbols(void* d_out, int64_t n_bytes, uint64_t seed, uint32_t lo, uint32_t span, void* stream) {
+    if (!d_out || n_bytes < 0 || n_bytes % 16 || ((uintptr_t)d_out & 15))
+        return fail(TREX_ERR_INVALID, "gen_symbols needs a 16-byte aligned buffer and a multiple of 16 bytes");
+    if (span < 1 || span > 256 || (span & (span - 1)) || lo + span > 256)
+        return fail(TREX_ERR_INVALID, "span must be a power of two with lo + span <= 256");
+    int64_t n16 = n_bytes / 16;
+    if (n16 == 0) return TREX_OK;
+    const unsigned grid = (unsigned)std::min<int64_t>((n16 + 255) / 256, 65536);
+    void* args[] = {&d_out, &n16, &seed, &lo, &span};
+    return launch("gen_symbols", grid, 256, 0, (hipStream_t)stream, args);
+}
+
 }  // extern "C"

@@ -8,6 +8,8 @@
 #include <mutex>
 #include <string>
 
+#include "trex_hip.h"
+
 namespace trex {
 
 extern thread_local std::string g_last_error;
@@ -58,4 +60,21 @@ struct trex_dfa {
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::mutex mu;            // serialises host dispatches on this handle
+};
+
+// A resident Boolean NFA-64 (the handle behind trex_nfa_*).
+struct trex_nfa {
+    int n_symbols = 0;
+    uint64_t start = 0, accept = 0;
+    int window_base = -1;
+    int variant = TREX_NFA_BYTES;
+    int n_classes = 0, n_classes_win = 0;
+    uint64_t* d_tab = nullptr;      // [n_symbols + 2][16][16]: symbols, reject, skip (bank-swizzled)
+    uint8_t* d_cls = nullptr;       // [256] byte -> class
+    uint64_t* d_tab_win = nullptr;  // [34][16][16]: window bytes base+0..31, reject, skip
+    uint8_t* d_cls_win = nullptr;   // [256]
+    trex::DevBuf in_bytes, in_offsets, out_mask, out_accept;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::mutex mu;
 };

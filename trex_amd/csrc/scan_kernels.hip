@@ -35,80 +35,16 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "device_common.h"
+
 #define TREX_BLOCK 256
 
 namespace {
 
+using namespace trexdev;
+
 constexpr uint32_t kStatusAggregate = 1u;
 constexpr uint32_t kStatusPrefix = 2u;
-
-struct Fn16 {
-    uint32_t d[4];   // byte s (s = 0..15) of the 16-byte vector = image of state s
-};
-
-__device__ __forceinline__ Fn16 fn_identity() {
-    return Fn16{{0x03020100u, 0x07060504u, 0x0b0a0908u, 0x0f0e0d0cu}};
-}
-
-__device__ __forceinline__ Fn16 fn_const(uint32_t s) {
-    const uint32_t c = s * 0x01010101u;
-    return Fn16{{c, c, c, c}};
-}
-
-// Bytes g[f.byte[i]] for the four entries packed in `f` (values < 16).
-__device__ __forceinline__ uint32_t fn_gather(uint32_t f, const Fn16& g) {
-    const uint32_t sel = f & 0x07070707u;
-    const uint32_t lo = __builtin_amdgcn_perm(g.d[1], g.d[0], sel);   // g[0..7]
-    const uint32_t hi = __builtin_amdgcn_perm(g.d[3], g.d[2], sel);   // g[8..15]
-    const uint32_t pick = 0x03020100u | ((f & 0x08080808u) >> 1);     // byte i of hi when bit 3 is set
-    return __builtin_amdgcn_perm(hi, lo, pick);
-}
-
-// (g o f): apply f, then g.
-__device__ __forceinline__ Fn16 fn_then(const Fn16& f, const Fn16& g) {
-    return Fn16{{fn_gather(f.d[0], g), fn_gather(f.d[1], g), fn_gather(f.d[2], g), fn_gather(f.d[3], g)}};
-}
-
-__device__ __forceinline__ bool fn_is_const(const Fn16& f) {
-    const uint32_t c = (f.d[0] & 0xffu) * 0x01010101u;
-    return (f.d[0] == c) & (f.d[1] == c) & (f.d[2] == c) & (f.d[3] == c);
-}
-
-__device__ __forceinline__ uint32_t fn_apply(const Fn16& f, uint32_t s) {
-    return (f.d[s >> 2] >> ((s & 3u) * 8u)) & 0xffu;
-}
-
-__device__ __forceinline__ uint64_t fn_pack(const Fn16& f) {
-    uint64_t out = 0;
-#pragma unroll
-    for (uint32_t s = 0; s < 16; ++s) out |= (uint64_t)fn_apply(f, s) << (4 * s);
-    return out;
-}
-
-__device__ __forceinline__ uint32_t nib(uint64_t f, uint32_t s) { return (uint32_t)(f >> (4 * s)) & 15u; }
-
-// (g o f) on 64-bit nibble functions.
-__device__ __forceinline__ uint64_t nib_then(uint64_t f, uint64_t g) {
-    uint64_t out = 0;
-#pragma unroll
-    for (uint32_t s = 0; s < 16; ++s) out |= (uint64_t)nib(g, nib(f, s)) << (4 * s);
-    return out;
-}
-
-__device__ __forceinline__ Fn16 load_row(const uint8_t* __restrict__ table, uint32_t row) {
-    const uint4 v = *reinterpret_cast<const uint4*>(table + row * 16u);
-    return Fn16{{v.x, v.y, v.z, v.w}};
-}
-
-struct WinCheck {
-    uint32_t any_hi, all;
-    __device__ __forceinline__ void init() { any_hi = 0u; all = 0xffffffffu; }
-    __device__ __forceinline__ void add(uint32_t x) { any_hi |= x & 0xe0e0e0e0u; all &= x; }
-    __device__ __forceinline__ bool outside(uint32_t win4) const { return ((any_hi & ~win4) | (win4 & ~all)) != 0u; }
-};
-
-__device__ __forceinline__ uint32_t pair_lo(uint32_t x) { return (x & 31u) | (((x >> 8) & 31u) << 5); }
-__device__ __forceinline__ uint32_t pair_hi(uint32_t x) { return ((x >> 16) & 31u) | (((x >> 24) & 31u) << 5); }
 
 // The per-lane scanner: either walking one state or composing a function.
 template <bool WINDOW>
@@ -146,13 +82,6 @@ struct Scanner {
         }
     }
 };
-
-__device__ __forceinline__ uint32_t load_status(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void store_status(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 template <bool WINDOW>
 __device__ void chunk_scan(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ cbegin,
@@ -233,11 +162,11 @@ __device__ void chunk_scan(const uint8_t* __restrict__ bytes, const int64_t* __r
         uint64_t my_fn = 0;
         if (active) {
             if (pred < 0) {
-                store_status(&status[c], kStatusPrefix | (sc.s << 8));
+                store_release(&status[c], kStatusPrefix | (sc.s << 8));
             } else {
                 my_fn = sc.walking ? fn_pack(fn_const(sc.s)) : fn_pack(sc.f);
                 agg[c] = my_fn;
-                store_status(&status[c], kStatusAggregate);
+                store_release(&status[c], kStatusAggregate);
             }
         }
 
@@ -249,7 +178,7 @@ __device__ void chunk_scan(const uint8_t* __restrict__ bytes, const int64_t* __r
             uint32_t spins = 0;
             bool done = false;
             while (!done) {
-                const uint32_t st = load_status(&status[j]);
+                const uint32_t st = load_acquire(&status[j]);
                 if (st == 0u) {
                     if (++spins > max_spins) {
                         atomicOr(error, 1u);
@@ -272,7 +201,7 @@ __device__ void chunk_scan(const uint8_t* __restrict__ bytes, const int64_t* __r
                     }
                 }
             }
-            store_status(&status[c], kStatusPrefix | (state << 8));
+            store_release(&status[c], kStatusPrefix | (state << 8));
         }
         if (active && (flags & 2u)) {
             const int32_t sid = csid[c];

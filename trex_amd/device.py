@@ -18,6 +18,10 @@ class Stream:
     def synchronize(self) -> None:
         _lib.call("trex_stream_sync", self.handle)
 
+    def wait(self, event: "Event") -> None:
+        """Later work on this stream waits for `event` (no host wait)."""
+        _lib.call("trex_stream_wait_event", self.handle, event.handle)
+
     def close(self) -> None:
         if self.handle:
             _lib.lib().trex_stream_destroy(self.handle)
@@ -38,6 +42,16 @@ class Event:
 
     def record(self, stream: Stream | None = None) -> None:
         _lib.call("trex_event_record", self.handle, stream.handle if stream else None)
+
+    def query(self) -> bool:
+        """True when all work recorded before the event has completed."""
+        rc = _lib.lib().trex_event_query(self.handle)
+        if rc < 0:
+            _lib.check(rc, "trex_event_query")
+        return rc == 0
+
+    def synchronize(self) -> None:
+        _lib.call("trex_event_sync", self.handle)
 
     def elapsed_ms(self, later: "Event") -> float:
         ms = ctypes.c_float()
@@ -108,3 +122,27 @@ class DeviceBuffer:
 
 def synchronize() -> None:
     _lib.call("trex_device_sync")
+
+
+class PinnedBuffer:
+    """Page-locked host memory (``trex_host_alloc``) seen as a uint8 numpy array,
+    so host <-> device copies run asynchronously at full link speed."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        h = ctypes.c_void_p()
+        _lib.call("trex_host_alloc", ctypes.byref(h), max(self.nbytes, 1))
+        self.ptr = h.value
+        self.array = np.ctypeslib.as_array((ctypes.c_uint8 * max(self.nbytes, 1)).from_address(self.ptr))
+
+    def free(self) -> None:
+        if self.ptr:
+            self.array = None
+            _lib.lib().trex_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -312,6 +312,34 @@ class ProbNfaEngine:
         _lib.call("trex_pnfa_loglik", self.handle, d_symbols, n_seq, length, d_ll, d_logmass,
                   stream.handle if stream else None)
 
+    def loglik_ranges_device(self, d_symbols: int, d_begin: int, d_end: int, n_seq: int, d_ll: int,
+                             d_logmass: int, stream: Stream | None = None) -> None:
+        """Ragged sequences: sequence j = symbols[begin[j]:end[j]] (device arrays)."""
+        _lib.call("trex_pnfa_loglik_ranges", self.handle, d_symbols, d_begin, d_end, int(n_seq), d_ll, d_logmass,
+                  stream.handle if stream else None)
+
+    def loglik_ragged(self, data: np.ndarray, offsets: np.ndarray):
+        """Host CSR batch of symbol sequences -> (ll float64[n], logmass float64[n]). Sequences are
+        grouped by length (longest first) for the launch and returned in input order."""
+        data, offsets = _as_csr(data=data, offsets=offsets)
+        n = len(offsets) - 1
+        ll = np.empty(n, np.float64)
+        lm = np.empty(n, np.float64)
+        if n == 0:
+            return ll, lm
+        order = np.argsort(-(offsets[1:] - offsets[:-1]), kind="stable")
+        stream = Stream()
+        d_sym = DeviceBuffer.from_array(data if len(data) else np.zeros(16, np.uint8), stream)
+        d_b = DeviceBuffer.from_array(offsets[:-1][order], stream)
+        d_e = DeviceBuffer.from_array(offsets[1:][order], stream)
+        d_ll, d_lm = DeviceBuffer(8 * n), DeviceBuffer(8 * n)
+        self.loglik_ranges_device(d_sym.ptr, d_b.ptr, d_e.ptr, n, d_ll.ptr, d_lm.ptr, stream)
+        ll[order] = d_ll.download(np.float64, n, stream)
+        lm[order] = d_lm.download(np.float64, n, stream)
+        for buf in (d_sym, d_b, d_e, d_ll, d_lm):
+            buf.free()
+        return ll, lm
+
     def loglik(self, symbols: np.ndarray):
         """symbols [n_seq, L] uint8 -> (ll float64[n], logmass float64[n], kernel_ms, total_ms)."""
         symbols = np.ascontiguousarray(symbols, dtype=np.uint8)
