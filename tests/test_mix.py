@@ -220,7 +220,7 @@ def test_gpu_mix_rejects_unsupported_pattern(gpu_lib):
     from trex_amd.engine import DfaEngine
     from trex_amd.regex_to_dfa import compile_regex
     from trex_amd.tables import dfa_byte_table
-    e = DfaEngine(dfa_byte_table(compile_regex("(a|~)*b")))   # live bytes in two windows
+    e = DfaEngine(dfa_byte_table(compile_regex("(a|A)*b")))   # live bytes 0x41 and 0x61: two windows
     with pytest.raises(TrexError):
         MixEngine([e])
 
