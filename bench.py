@@ -97,6 +97,9 @@ def main() -> int:
     ap.add_argument("--layout", default="staggered", choices=["staggered", "csr"])
     ap.add_argument("--legs", default="dfa,nfa,prob,mix,serve", help="comma list of dfa, nfa, prob, mix, serve")
     ap.add_argument("--prob-modes", default="f16x2,bf16x2,f16,bf16")
+    ap.add_argument("--mix-strings", type=int, default=MIX_STRINGS, help="strings of the uneven-mix leg")
+    ap.add_argument("--mix-tune", type=int, default=1, help="autotune the uneven-mix plan (0: defaults)")
+    ap.add_argument("--mix-max-spins", type=int, default=0, help="bound of every wait in the mixed queue (0: default)")
     ap.add_argument("--serve-qps", type=float, default=50.0, help="Poisson arrival rate of the serving leg")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-burst", type=int, default=24, help="requests of the served-vs-plain burst")
@@ -441,7 +444,7 @@ def mix_leg(args, stream) -> dict:
     from trex_amd.autotune import tune_mix
     from trex_amd.device import DeviceBuffer, Event
     from trex_amd.engine import ChunkPlan, DfaEngine, NfaEngine
-    from trex_amd.mix import MixEngine
+    from trex_amd.mix import MixEngine, MixParams
     from trex_amd.tables import dfa_byte_table
     from trex_amd.workloads import ALPHABET_32, random_dfa_pattern, random_nfa64
 
@@ -449,17 +452,18 @@ def mix_leg(args, stream) -> dict:
     dfa_tables = [dfa_byte_table(random_dfa_pattern(rng, ALPHABET_32)[1]) for _ in range(MIX_PATTERNS // 2)]
     nfas = [random_nfa64(rng) for _ in range(MIX_PATTERNS // 2)]
     engines = [DfaEngine(t) for t in dfa_tables] + [NfaEngine.from_matrices(nm) for nm in nfas]
-    lengths = np.floor(np.exp(rng.uniform(np.log(MIX_LEN[0]), np.log(MIX_LEN[1]), size=MIX_STRINGS))).astype(np.int64)
-    pattern_of = rng.integers(0, MIX_PATTERNS, size=MIX_STRINGS).astype(np.int32)
-    offsets = np.zeros(MIX_STRINGS + 1, np.int64)
+    n_strings = args.mix_strings
+    lengths = np.floor(np.exp(rng.uniform(np.log(MIX_LEN[0]), np.log(MIX_LEN[1]), size=n_strings))).astype(np.int64)
+    pattern_of = rng.integers(0, MIX_PATTERNS, size=n_strings).astype(np.int32)
+    offsets = np.zeros(n_strings + 1, np.int64)
     np.cumsum(lengths, out=offsets[1:])
     total = int(offsets[-1])
     t0 = time.perf_counter()
     d_bytes = DeviceBuffer(-(-total // 16) * 16)
     _lib.call("trex_gen_symbols", d_bytes.ptr, d_bytes.nbytes, MIX_GEN_SEED, 0x40, 32, stream.handle)
     stream.synchronize()
-    log(f"mix: {MIX_STRINGS} strings, {total / 1e9:.1f} GB generated in HBM in {time.perf_counter() - t0:.2f}s")
-    n = MIX_STRINGS
+    log(f"mix: {n_strings} strings, {total / 1e9:.1f} GB generated in HBM in {time.perf_counter() - t0:.2f}s")
+    n = n_strings
     d_mask, d_acc = DeviceBuffer(8 * n), DeviceBuffer(n)
     begin, end = offsets[:-1], offsets[1:]
 
@@ -475,11 +479,21 @@ def mix_leg(args, stream) -> dict:
         return [a.elapsed_ms(b) for a, b in evs]
 
     # autotuned plan on a sample of the batch, then the full batch
-    params, tuning = tune_mix(engines, begin, end, pattern_of, d_bytes.ptr, stream)
-    log(f"mix: tuned {params} ({tuning.get('source')})")
+    if args.mix_tune:
+        params, tuning = tune_mix(engines, begin, end, pattern_of, d_bytes.ptr, stream, log=log)
+    else:
+        params, tuning = MixParams(), {"source": "defaults"}
+    log(f"mix: parameters {params} ({tuning.get('source')})")
     mix = MixEngine(engines)
+    t0 = time.perf_counter()
     info = mix.plan(begin, end, pattern_of, params)
-    ms = timed(lambda: mix.run(d_bytes.ptr, d_mask.ptr, d_acc.ptr, stream), max(2, args.steps // 10))
+    log(f"mix: planned {info} in {time.perf_counter() - t0:.2f}s")
+    t0 = time.perf_counter()
+    mix.run(d_bytes.ptr, d_mask.ptr, d_acc.ptr, stream, args.mix_max_spins)
+    stream.synchronize()
+    log(f"mix: first run {time.perf_counter() - t0:.3f}s, error word {mix.error(stream)}")
+    ms = timed(lambda: mix.run(d_bytes.ptr, d_mask.ptr, d_acc.ptr, stream, args.mix_max_spins),
+               max(2, args.steps // 10))
     err = mix.error(stream)
     mix.capture(d_bytes.ptr, d_mask.ptr, d_acc.ptr, stream)
     ms_graph = timed(lambda: mix.replay(stream), max(2, args.steps // 10))

@@ -261,6 +261,9 @@ int trex_mix_run(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, uint
 int trex_mix_set_grid(trex_mix* m, int64_t workgroups);
 /* Error word of the last run (0: no wait expired). Synchronises `stream`. */
 int trex_mix_error(trex_mix* m, uint32_t* error, void* stream);
+/* Speculative NFA chunks of the last run whose fix-up met a checkpoint (the
+ * rest were walked to their end). Synchronises `stream`. */
+int trex_mix_stats(trex_mix* m, uint32_t* converged, void* stream);
 /* Captures trex_mix_run into a HIP graph (non-default stream), replayed by
  * trex_mix_replay with the same buffers. */
 int trex_mix_capture(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, uint8_t* d_out_accept,

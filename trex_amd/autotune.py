@@ -83,7 +83,7 @@ def _length_summary(lengths: np.ndarray) -> list:
 
 
 def tune_mix(engines, begin, end, pattern_of, d_bytes: int, stream: Stream, sample: int = 4096,
-             grid: dict | None = None, seed: int = 0, use_cache: bool = True):
+             grid: dict | None = None, seed: int = 0, use_cache: bool = True, log=None):
     """Best :class:`MixParams` for this batch, and a dict describing the choice."""
     begin = np.asarray(begin, np.int64)
     end = np.asarray(end, np.int64)
@@ -111,6 +111,8 @@ def tune_mix(engines, begin, end, pattern_of, d_bytes: int, stream: Stream, samp
         mix.plan(sb, se, sp, params)
         ms = _time(lambda: mix.run(d_bytes, d_mask.ptr, d_acc.ptr, stream), stream, 2)
         results.append((ms, params))
+        if log:
+            log(f"tune_mix: {params} {ms:.3f} ms")
     d_mask.free()
     d_acc.free()
     mix.close()

@@ -138,6 +138,13 @@ __device__ __forceinline__ Fn16 load_row(const uint8_t* __restrict__ table, uint
 __device__ __forceinline__ uint32_t load_acquire(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
+// For spin loops: a relaxed agent-scope load sees other agents' releases
+// without invalidating caches on every iteration; once the awaited value is
+// seen, one fence_acquire orders the reads that follow.
+__device__ __forceinline__ uint32_t load_relaxed(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void fence_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 __device__ __forceinline__ void store_release(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }

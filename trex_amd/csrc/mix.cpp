@@ -70,6 +70,7 @@ struct MixArgs {
     uint64_t* ckpt;
     uint64_t* spec_exit;
     uint32_t* error;
+    uint32_t* converged;
     uint32_t max_spins;
     uint64_t* out_mask;
     uint8_t* out_accept;
@@ -110,7 +111,7 @@ struct trex_mix {
     MixPattern* d_pats = nullptr;
     Plan plan;
     DevBuf d_ibeg, d_iend, d_ia, d_ib, d_ic, d_iflags, d_bfirst, d_bcount, d_bpattern, d_brole;
-    DevBuf d_words;   // ticket, error, done[n_slots], status[n_items]
+    DevBuf d_words;   // ticket, error, converged, done[n_slots], status[n_items]
     DevBuf d_agg, d_ckpt, d_exit;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -339,7 +340,7 @@ int trex_mix_plan(trex_mix* m, const int64_t* begin, const int64_t* end, const i
         TREX_TRY(upload_vec(m->d_bcount, Q.bcount));
         TREX_TRY(upload_vec(m->d_bpattern, Q.bpattern));
         TREX_TRY(upload_vec(m->d_brole, Q.brole));
-        TREX_TRY(m->d_words.reserve(4 * (2 + (size_t)Q.n_slots + Q.items())));
+        TREX_TRY(m->d_words.reserve(4 * (3 + (size_t)Q.n_slots + Q.items())));
         TREX_TRY(m->d_agg.reserve(8 * std::max<size_t>(Q.items(), 1)));
         TREX_TRY(m->d_ckpt.reserve(8 * std::max<size_t>((size_t)Q.n_spec * Q.n_ck, 1)));
         TREX_TRY(m->d_exit.reserve(8 * std::max<size_t>((size_t)Q.n_spec, 1)));
@@ -382,7 +383,7 @@ int trex_mix_run(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, uint
     if (!d_bytes || !d_out_mask || !d_out_accept) return fail(TREX_ERR_INVALID, "null buffer");
     hipStream_t s = (hipStream_t)stream;
     uint32_t* words = m->d_words.as<uint32_t>();
-    int64_t n_words = 2 + Q.n_slots + (int64_t)Q.items();
+    int64_t n_words = 3 + Q.n_slots + (int64_t)Q.items();
     uint32_t zero = 0;
     {
         const unsigned grid = (unsigned)std::min<int64_t>((n_words + kBlock - 1) / kBlock, 4096);
@@ -407,8 +408,9 @@ int trex_mix_run(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, uint
     a.n_ck = Q.n_ck;
     a.ticket = words;
     a.error = words + 1;
-    a.done = words + 2;
-    a.status = words + 2 + Q.n_slots;
+    a.converged = words + 2;
+    a.done = words + 3;
+    a.status = words + 3 + Q.n_slots;
     a.agg = m->d_agg.as<uint64_t>();
     a.ckpt = m->d_ckpt.as<uint64_t>();
     a.spec_exit = m->d_exit.as<uint64_t>();
@@ -435,6 +437,16 @@ int trex_mix_error(trex_mix* m, uint32_t* error, void* stream) {
     if (!m->has_device || m->plan.bfirst.empty()) return TREX_OK;
     TREX_HIP(hipMemcpyAsync(error, m->d_words.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, (hipStream_t)stream),
              "D2H error");
+    TREX_HIP(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+    return TREX_OK;
+}
+
+int trex_mix_stats(trex_mix* m, uint32_t* converged, void* stream) {
+    if (!m || !converged) return fail(TREX_ERR_INVALID, "null argument");
+    *converged = 0;
+    if (!m->has_device || m->plan.bfirst.empty()) return TREX_OK;
+    TREX_HIP(hipMemcpyAsync(converged, m->d_words.as<uint32_t>() + 2, 4, hipMemcpyDeviceToHost, (hipStream_t)stream),
+             "D2H stats");
     TREX_HIP(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
     return TREX_OK;
 }

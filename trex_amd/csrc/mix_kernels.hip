@@ -21,7 +21,8 @@
 //              set equals the chunk's checkpoint at the same position, after
 //              which both walks coincide, so the chunk's exit is the recorded
 //              one. A chunk that never meets its checkpoints is walked to its
-//              end. The result is exact either way.
+//              end, and an empty set ends the fix-up (the empty set is
+//              absorbing). The result is exact either way.
 //
 // Forward progress: the queue puts every batch that is waited on before the
 // batches that wait on it (DFA chunks after their predecessors, fix-ups after
@@ -83,6 +84,7 @@ struct MixArgs {
     uint64_t* ckpt;           // [n_spec][n_ck]
     uint64_t* spec_exit;      // [n_spec]
     uint32_t* error;
+    uint32_t* converged;      // fix-up chunks that met a checkpoint
     uint32_t max_spins;
     uint64_t* out_mask;       // per string: DFA final state / NFA final set
     uint8_t* out_accept;      // per string
@@ -207,16 +209,17 @@ __device__ void dfa_chunk(const MixArgs& A, const MixPattern& P, const DfaLds& L
         int32_t j = pred;
         uint32_t spins = 0;
         while (true) {
-            const uint32_t st = load_acquire(&A.status[j]);
+            const uint32_t st = load_relaxed(&A.status[j]);
             if (st == 0u) {
                 if (++spins > A.max_spins) {
                     atomicOr(A.error, 1u);
                     state = P.dead;
                     break;
                 }
-                __builtin_amdgcn_s_sleep(1);
+                __builtin_amdgcn_s_sleep(2);
                 continue;
             }
+            fence_acquire();   // agg[j] was written before status[j] was released
             if ((st & 0xffu) == kStatusPrefix) {
                 state = nib(my_fn, nib(acc_fn, (st >> 8) & 15u));
                 break;
@@ -398,24 +401,27 @@ mix_persistent(MixArgs A)
             const uint32_t need = (uint32_t)(c1 - c0);
             uint32_t spins = 0;
             bool ok = true;
-            while (load_acquire(&A.done[A.ia[item]]) < need) {
+            while (load_relaxed(&A.done[A.ia[item]]) < need) {
                 if (++spins > A.max_spins) {
                     atomicOr(A.error, 4u);
                     ok = false;
                     break;
                 }
-                __builtin_amdgcn_s_sleep(2);
+                __builtin_amdgcn_s_sleep(8);
             }
+            fence_acquire();   // checkpoints and exits were written before each chunk counted itself
             uint64_t s = 0;
             if (ok) {
                 s = A.spec_exit[A.ic[c0]];   // the first chunk was walked from the start set
-                for (int64_t c = c0 + 1; c < c1; ++c) {
+                // the empty set is absorbing: once the true set is empty, so are all later exits
+                for (int64_t c = c0 + 1; c < c1 && s != 0; ++c) {
                     const int32_t o = A.ic[c];
                     bool met;
                     const uint64_t t = nfa_walk_ck<false, true>(A.bytes, A.ibeg[c], A.iend[c], s, ntab, ncls,
                                                                 P.win4, A.ck, nullptr,
                                                                 A.ckpt + (int64_t)o * A.n_ck, A.n_ck, met);
                     s = met ? A.spec_exit[o] : t;
+                    if (met) atomicAdd(A.converged, 1u);
                 }
             }
             A.out_mask[A.ib[item]] = s;

@@ -178,16 +178,17 @@ __device__ void chunk_scan(const uint8_t* __restrict__ bytes, const int64_t* __r
             uint32_t spins = 0;
             bool done = false;
             while (!done) {
-                const uint32_t st = load_acquire(&status[j]);
+                const uint32_t st = load_relaxed(&status[j]);
                 if (st == 0u) {
                     if (++spins > max_spins) {
                         atomicOr(error, 1u);
                         state = dead;
                         break;
                     }
-                    __builtin_amdgcn_s_sleep(1);
+                    __builtin_amdgcn_s_sleep(2);
                     continue;
                 }
+                fence_acquire();   // agg[j] was written before status[j] was released
                 if ((st & 0xffu) == kStatusPrefix) {
                     state = nib(my_fn, nib(acc_fn, (st >> 8) & 15u));
                     done = true;

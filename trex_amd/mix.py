@@ -109,6 +109,12 @@ class MixEngine:
         _lib.call("trex_mix_error", self.handle, ctypes.byref(e), stream.handle if stream else None)
         return e.value
 
+    def converged(self, stream: Stream | None = None) -> int:
+        """Speculative NFA chunks of the last run whose fix-up met a checkpoint."""
+        c = ctypes.c_uint32()
+        _lib.call("trex_mix_stats", self.handle, ctypes.byref(c), stream.handle if stream else None)
+        return c.value
+
     def capture(self, d_bytes: int, d_mask: int, d_accept: int, stream: Stream, max_spins: int = 0) -> None:
         _lib.call("trex_mix_capture", self.handle, d_bytes, d_mask, d_accept, int(max_spins), stream.handle)
 
