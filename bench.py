@@ -115,6 +115,10 @@ def main() -> int:
                           "n_gpus": args.gpus}), flush=True)
         return 2
     legs = [x for x in args.legs.split(",") if x]
+    if "serve" in legs:
+        # one hardware queue per server stream (copy, two DFA/NFA lanes, four probabilistic
+        # slots), set before the HIP runtime starts: streams that share a queue serialise
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
     from trex_amd import _lib
     from trex_amd.device import DeviceBuffer, Stream
@@ -616,9 +620,13 @@ def serve_leg(args, stream) -> dict:
     engines += [NfaEngine.from_matrices(random_nfa64(rng)) for _ in range(n_nfa)]
     engines += [ProbNfaEngine(*dirichlet_pnfa(rng, 32)) for _ in range(n_prob)]
     t0 = time.perf_counter()
-    pool = rng.integers(0x40, 0x60, size=SERVE_POOL, dtype=np.uint8)
-    pool_sym = pool & np.uint8(31)
-    log(f"serve: {len(engines)} resident patterns, {SERVE_POOL >> 30} GiB request pool in {time.perf_counter() - t0:.1f}s")
+    from trex_amd.device import PinnedBuffer
+    pinned, pinned_sym = PinnedBuffer(SERVE_POOL), PinnedBuffer(SERVE_POOL)   # requests arrive in pinned memory
+    pool, pool_sym = pinned.array, pinned_sym.array
+    pool[:] = rng.integers(0x40, 0x60, size=SERVE_POOL, dtype=np.uint8)
+    np.bitwise_and(pool, np.uint8(31), out=pool_sym)
+    log(f"serve: {len(engines)} resident patterns, {SERVE_POOL >> 30} GiB pinned request pool in "
+        f"{time.perf_counter() - t0:.1f}s")
 
     def make_request(r: np.random.Generator) -> Request:
         p = int(r.integers(0, len(engines)))
@@ -670,6 +678,13 @@ def serve_leg(args, stream) -> dict:
         sel = np.array([r.lane == lane for r in reqs])
         by_lane[lane] = {"requests": int(sel.sum()), "p50_ms": pct(lat[sel], 50), "p99_ms": pct(lat[sel], 99)}
     sel_dn = small & (kinds == "dfa_nfa")
+    phases = {}
+    for name, a_, b_ in (("admit", "arrival", "admitted"), ("wait_slot", "admitted", "launched"),
+                         ("run", "launched", "done")):
+        dt = np.array([(getattr(r, b_) - getattr(r, a_)) * 1e3 for r in reqs])
+        phases[name] = {"p50_ms_le_1MiB": pct(dt[small], 50), "p99_ms_le_1MiB": pct(dt[small], 99),
+                        "p50_ms_all": pct(dt, 50)}
+    log(f"serve: latency phases {phases}")
     log(f"serve: {n_req} requests at {qps} QPS in {wall:.1f}s, {total_bytes / 1e9:.1f} GB; "
         f"p50 {pct(lat, 50)} ms p99 {pct(lat, 99)} ms; <= 1 MiB: {int(small.sum())} requests p50 "
         f"{pct(lat[small], 50)} p99 {pct(lat[small], 99)} ms (DFA/NFA only: p99 {pct(lat[sel_dn], 99)} ms); "
@@ -728,7 +743,7 @@ def serve_leg(args, stream) -> dict:
                                        "p99": pct(lat[small], 99)},
                            "le_1MiB_dfa_nfa": {"requests": int(sel_dn.sum()), "p50": pct(lat[sel_dn], 50),
                                                "p99": pct(lat[sel_dn], 99)},
-                           "by_lane": by_lane},
+                           "by_lane": by_lane, "phases": phases},
             "served_gchars_s": round(total_bytes / wall / 1e9, 3),
             "burst": {"requests": len(burst), "bytes": bb, "served_s": round(served_s, 3),
                       "plain_s": round(plain_s, 3), "served_gchars_s": round(bb / served_s / 1e9, 2),

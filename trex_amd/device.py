@@ -11,6 +11,7 @@ from trex_amd import _lib
 
 class Stream:
     def __init__(self):
+        """A non-blocking stream (default priority)."""
         h = ctypes.c_void_p()
         _lib.call("trex_stream_create", ctypes.byref(h))
         self.handle = h.value

@@ -6,7 +6,9 @@ Boolean NFA-64 and probabilistic NFA-64 engines) and serves match requests
 
 * admission by HBM budget: a request's bytes are copied into a device arena
   (on a copy stream, overlapping running work) as soon as the arena has room;
-  requests that have not found room wait in FIFO order per lane;
+  requests that have not found room wait in FIFO order per lane. Small
+  requests have an arena of their own, so a backlog of large ones never
+  blocks their admission;
 * four lanes, each with its own stream: DFA/NFA requests of at most
   ``small_bytes`` bytes, larger DFA/NFA requests, and the same two classes of
   probabilistic requests. A lane launches a *tick* -- every admitted request
@@ -14,13 +16,22 @@ Boolean NFA-64 and probabilistic NFA-64 engines) and serves match requests
   so a new tick queues behind the running one and requests that arrive while
   others are on the GPU join the next tick;
 * a DFA/NFA tick is one persistent mixed-queue launch (:mod:`trex_amd.mix`);
-  a probabilistic tick is one ragged launch per request. The bulk lane's
-  persistent grid is one workgroup per compute unit, so a small-request tick
-  can run beside it;
+  the bulk lane's persistent grid is one workgroup per compute unit and it
+  keeps one tick in flight (a second kernel on the same stream would be
+  dispatched onto the remaining room as the first one's workgroups are
+  placed), so a small-request tick always finds room beside it. A probabilistic request is a tick of
+  its own (one ragged launch); each probabilistic lane has ``prob_slots``
+  slots with a stream each, so a few long sequential launches overlap one
+  another while the number of compute units they hold stays bounded;
 * results are copied back into pinned host memory on the lane stream behind
   the kernels; a tick is complete when its event has fired, and then every
   request of the tick gets its result and completion time, and its arena space
   is released.
+
+The server uses seven streams (copy, two DFA/NFA lanes, two slots in each
+probabilistic lane); streams that share a hardware queue run their kernels in
+order, so a process that serves should start with ``GPU_MAX_HW_QUEUES`` of at
+least 8 (the HIP runtime reads it once, before the first GPU call).
 
 The reference has no serving path: one synchronous launch per request and
 pattern through a process-global engine (cuda/monoid_batch.cu:596-671).
@@ -57,6 +68,8 @@ class Request:
     result: tuple | None = None
     error: str | None = None
     _arena: int = -1
+    _arena_obj: "Arena | None" = None
+    _copied: "Event | None" = None
     _event: threading.Event = field(default_factory=threading.Event)
 
     @property
@@ -109,7 +122,8 @@ class Arena:
 class _Slot:
     """Device and pinned buffers of one in-flight tick."""
 
-    def __init__(self, engines_dfa_nfa):
+    def __init__(self, engines_dfa_nfa, stream: Stream):
+        self.stream = stream
         self.mix = MixEngine(engines_dfa_nfa) if engines_dfa_nfa else None
         self.cap = 0
         self.d_mask = self.d_acc = self.d_begin = self.d_end = self.d_lm = None
@@ -137,7 +151,8 @@ class _Slot:
 class Server:
     def __init__(self, engines: list, hbm_budget: int = 8 << 30, small_bytes: int = 1 << 20,
                  small_params: MixParams | None = None, bulk_params: MixParams | None = None,
-                 bulk_grid: int | None = None):
+                 bulk_grid: int | None = None, small_budget: int | None = None, prob_slots: int = 2,
+                 bulk_slots: int = 1):
         self.engines = list(engines)
         self.kind = []
         mix_index = {}
@@ -154,56 +169,72 @@ class Server:
         self.mix_index = mix_index
         self.small_bytes = int(small_bytes)
         self.arena = Arena(hbm_budget)
+        self.small_arena = Arena(small_budget if small_budget is not None else max(64 * self.small_bytes, 1 << 30))
         self.copy_stream = Stream()
-        self.lanes = {name: {"stream": Stream(), "pending": collections.deque(), "slots": [], "queue": collections.deque()}
+        cus = _lib.device_info()["cus"]
+        self.lanes = {name: {"pending": collections.deque(), "slots": [], "queue": collections.deque()}
                       for name in ("small", "bulk", "prob_small", "prob_bulk")}
         for name, lane in self.lanes.items():
-            is_mix = not name.startswith("prob")
-            lane["slots"] = [_Slot(mix_engines if is_mix else None) for _ in range(2)]
+            if name.startswith("prob"):
+                lane["slots"] = [_Slot(None, Stream()) for _ in range(prob_slots)]
+            else:
+                stream = Stream()
+                lane["slots"] = [_Slot(mix_engines, stream) for _ in range(bulk_slots if name == "bulk" else 2)]
             lane["params"] = (small_params or MixParams(dfa_split=64 << 10, dfa_chunk=16 << 10, nfa_split=8 << 10,
                                                         nfa_chunk=8 << 10, ck=64)) if name == "small" else \
                 (bulk_params or MixParams())
-        grid = bulk_grid if bulk_grid is not None else _lib.device_info()["cus"]
         for slot in self.lanes["bulk"]["slots"]:
-            slot.mix.set_grid(grid)
+            slot.mix.set_grid(bulk_grid if bulk_grid is not None else cus)
         for slot in self.lanes["small"]["slots"]:
-            slot.mix.set_grid(_lib.device_info()["cus"])
-        self._arrivals: collections.deque[Request] = collections.deque()
-        self._lock = threading.Lock()
-        self._wake = threading.Event()
+            slot.mix.set_grid(cus)
+        self._lock = threading.Lock()   # arenas, lane queues and pending lists
+        for lane in self.lanes.values():
+            lane["cv"] = threading.Condition(self._lock)
         self._stop = False
-        self._thread: threading.Thread | None = None
+        self._threads: list[threading.Thread] = []
         self.completed: list[Request] = []
         self.ticks = collections.Counter()
 
     # -- client side ---------------------------------------------------------------
     def submit(self, req: Request) -> Request:
+        """Admits the request at once when the arena has room (its bytes are
+        copied on the copy stream), else queues it in FIFO order for its lane."""
         if not 0 <= req.pattern < len(self.engines):
             raise ValueError(f"pattern {req.pattern} out of range")
         req.offsets = np.ascontiguousarray(req.offsets, dtype=np.int64)
         req.data = np.ascontiguousarray(req.data, dtype=np.uint8)
         if req.offsets[0] != 0:
             raise ValueError("offsets must start at 0")
-        if req.nbytes > self.arena.size:
+        req.lane = self._lane_of(req)
+        if req.nbytes > self._arena_of(req.lane).size:
             raise ValueError("request larger than the HBM budget")
         req.arrival = time.perf_counter()
         with self._lock:
-            self._arrivals.append(req)
-        self._wake.set()
+            lane = self.lanes[req.lane]
+            if lane["pending"] or not self._try_admit(req):
+                lane["pending"].append(req)
+            else:
+                lane["queue"].append(req)
+                lane["cv"].notify()
         return req
 
     def start(self) -> "Server":
         self._stop = False
-        self._thread = threading.Thread(target=self._loop, name="trex-serve", daemon=True)
-        self._thread.start()
+        self._threads = [threading.Thread(target=self._worker, args=(name,), name=f"trex-serve-{name}", daemon=True)
+                         for name in self.lanes]
+        for t in self._threads:
+            t.start()
         return self
 
     def stop(self) -> None:
-        self._stop = True
-        self._wake.set()
-        if self._thread:
-            self._thread.join()
-            self._thread = None
+        """Serves everything submitted so far, then stops the lane threads."""
+        with self._lock:
+            self._stop = True
+            for lane in self.lanes.values():
+                lane["cv"].notify_all()
+        for t in self._threads:
+            t.join()
+        self._threads = []
 
     def __enter__(self):
         return self.start()
@@ -211,80 +242,92 @@ class Server:
     def __exit__(self, *exc):
         self.stop()
 
-    # -- dispatcher ----------------------------------------------------------------
+    # -- admission -------------------------------------------------------------------
     def _lane_of(self, req: Request) -> str:
         small = req.nbytes <= self.small_bytes
         if self.kind[req.pattern] == "mix":
             return "small" if small else "bulk"
         return "prob_small" if small else "prob_bulk"
 
-    def _loop(self) -> None:
+    def _arena_of(self, lane: str) -> "Arena":
+        return self.small_arena if lane.endswith("small") else self.arena
+
+    def _try_admit(self, r: Request) -> bool:
+        """Lock held. Reserves arena space and starts the copy."""
+        arena = self._arena_of(r.lane)
+        off = arena.alloc(max(r.nbytes, 16))
+        if off < 0:
+            return False
+        r._arena, r._arena_obj = off, arena
+        if r.nbytes:
+            _lib.call("trex_memcpy_h2d", arena.buf.ptr + off, r.data.ctypes.data, r.nbytes, self.copy_stream.handle)
+        r._copied = Event()
+        r._copied.record(self.copy_stream)
+        r.admitted = time.perf_counter()
+        return True
+
+    def _release(self, reqs: list[Request]) -> None:
+        """Lock held. Frees the requests' arena space and admits waiting requests."""
+        for r in reqs:
+            r._arena_obj.release(r._arena, max(r.nbytes, 16))
+        for lane in self.lanes.values():
+            while lane["pending"] and self._try_admit(lane["pending"][0]):
+                lane["queue"].append(lane["pending"].popleft())
+                lane["cv"].notify()
+
+    # -- lane workers ------------------------------------------------------------------
+    def _worker(self, name: str) -> None:
+        """One thread per lane: launch ticks while a slot is free, complete them
+        in launch order; every wait releases the GIL."""
+        lane = self.lanes[name]
+        prob = name.startswith("prob")
+        inflight: collections.deque[_Slot] = collections.deque()
         while True:
+            reqs, slot = [], None
             with self._lock:
-                while self._arrivals:
-                    r = self._arrivals.popleft()
-                    r.lane = self._lane_of(r)
-                    self.lanes[r.lane]["pending"].append(r)
-            progressed = self._admit()
-            progressed |= self._complete()
-            progressed |= self._launch()
-            if self._stop and not any(l["pending"] or l["queue"] or any(s.busy for s in l["slots"])
-                                      for l in self.lanes.values()):
-                return
-            if not progressed:
-                self._wake.wait(0.0002)
-                self._wake.clear()
-
-    def _admit(self) -> bool:
-        """Copies pending requests into the arena while it has room (FIFO per lane)."""
-        progressed = False
-        for name in ("small", "prob_small", "bulk", "prob_bulk"):
-            lane = self.lanes[name]
-            while lane["pending"]:
-                r = lane["pending"][0]
-                off = self.arena.alloc(max(r.nbytes, 16))
-                if off < 0:
-                    break
-                lane["pending"].popleft()
-                r._arena = off
-                if r.nbytes:
-                    _lib.call("trex_memcpy_h2d", self.arena.buf.ptr + off, r.data.ctypes.data, r.nbytes,
-                              self.copy_stream.handle)
-                r.admitted = time.perf_counter()
-                lane["queue"].append(r)
-                progressed = True
-        return progressed
-
-    def _launch(self) -> bool:
-        progressed = False
-        for name, lane in self.lanes.items():
-            if not lane["queue"]:
+                if not inflight:
+                    while not lane["queue"] and not (self._stop and not lane["pending"]):
+                        lane["cv"].wait(0.05)
+                    if not lane["queue"] and self._stop and not lane["pending"]:
+                        return
+                slot = next((s for s in lane["slots"] if not s.busy), None)
+                if lane["queue"] and slot is not None:
+                    if prob:
+                        reqs = [lane["queue"].popleft()]   # one ragged launch per tick
+                    else:
+                        reqs = list(lane["queue"])
+                        lane["queue"].clear()
+            if reqs:
+                stream = slot.stream
+                for r in reqs:
+                    stream.wait(r._copied)
+                if prob:
+                    self._launch_prob(slot, reqs, stream)
+                else:
+                    self._launch_mix(slot, reqs, stream, lane["params"])
+                slot.event.record(stream)
+                slot.requests = reqs
+                slot.busy = True
+                now = time.perf_counter()
+                for r in reqs:
+                    r.launched = now
+                inflight.append(slot)
+                self.ticks[name] += 1
                 continue
-            slot = next((s for s in lane["slots"] if not s.busy), None)
-            if slot is None:
-                continue
-            reqs = list(lane["queue"])
-            lane["queue"].clear()
-            copied = Event()
-            copied.record(self.copy_stream)
-            stream = lane["stream"]
-            stream.wait(copied)
-            if name.startswith("prob"):
-                self._launch_prob(slot, reqs, stream)
-            else:
-                self._launch_mix(slot, reqs, stream, lane["params"])
-            slot.event.record(stream)
-            slot.requests = reqs
-            slot.busy = True
-            now = time.perf_counter()
-            for r in reqs:
-                r.launched = now
-            self.ticks[name] += 1
-            progressed = True
-        return progressed
+            if inflight:
+                oldest = inflight[0]
+                if len(inflight) == len(lane["slots"]):
+                    oldest.event.synchronize()   # every slot busy: wait for the oldest tick
+                elif not oldest.event.query():
+                    with self._lock:
+                        if not lane["queue"]:
+                            lane["cv"].wait(0.0002)
+                    continue
+                inflight.popleft()
+                self._complete(name, oldest)
 
     def _launch_mix(self, slot: _Slot, reqs: list[Request], stream: Stream, params: MixParams) -> None:
-        base = self.arena.buf.ptr
+        base = reqs[0]._arena_obj.buf.ptr
         begin = np.concatenate([r._arena + r.offsets[:-1] for r in reqs])
         end = np.concatenate([r._arena + r.offsets[1:] for r in reqs])
         pat = np.concatenate([np.full(len(r.offsets) - 1, self.mix_index[r.pattern], np.int32) for r in reqs])
@@ -297,7 +340,7 @@ class Server:
             _lib.call("trex_memcpy_d2h", slot.h_acc.ptr, slot.d_acc.ptr, n, stream.handle)
 
     def _launch_prob(self, slot: _Slot, reqs: list[Request], stream: Stream) -> None:
-        base = self.arena.buf.ptr
+        base = reqs[0]._arena_obj.buf.ptr
         n = sum(len(r.offsets) - 1 for r in reqs)
         slot.reserve(n)
         begin = np.empty(n, np.int64)
@@ -328,36 +371,31 @@ class Server:
         _lib.call("trex_memcpy_d2h", slot.h_f64.ptr, slot.d_mask.ptr, 8 * n, stream.handle)
         _lib.call("trex_memcpy_d2h", slot.h_f64.ptr + 8 * n, slot.d_lm.ptr, 8 * n, stream.handle)
 
-    def _complete(self) -> bool:
-        progressed = False
-        for name, lane in self.lanes.items():
-            for slot in lane["slots"]:
-                if not slot.busy or not slot.event.query():
-                    continue
-                now = time.perf_counter()
-                if name.startswith("prob"):
-                    n = sum(len(r.offsets) - 1 for r in slot.requests)
-                    f64 = slot.h_f64.array.view(np.float64)
-                    for r, (k0, m, order) in zip(slot.requests, slot.order):
-                        ll = np.empty(m, np.float64)
-                        lm = np.empty(m, np.float64)
-                        ll[order] = f64[k0:k0 + m]
-                        lm[order] = f64[n + k0:n + k0 + m]
-                        r.result = (ll, lm)
-                else:
-                    k = 0
-                    mask = slot.h_mask.array.view(np.uint64)
-                    acc = slot.h_acc.array
-                    for r in slot.requests:
-                        m = len(r.offsets) - 1
-                        r.result = (mask[k:k + m].copy(), acc[k:k + m].copy())
-                        k += m
-                for r in slot.requests:
-                    r.done = now
-                    self.arena.release(r._arena, max(r.nbytes, 16))
-                    r._event.set()
-                    self.completed.append(r)
-                slot.requests = []
-                slot.busy = False
-                progressed = True
-        return progressed
+    def _complete(self, name: str, slot: _Slot) -> None:
+        now = time.perf_counter()
+        if name.startswith("prob"):
+            n = sum(len(r.offsets) - 1 for r in slot.requests)
+            f64 = slot.h_f64.array.view(np.float64)
+            for r, (k0, m, order) in zip(slot.requests, slot.order):
+                ll = np.empty(m, np.float64)
+                lm = np.empty(m, np.float64)
+                ll[order] = f64[k0:k0 + m]
+                lm[order] = f64[n + k0:n + k0 + m]
+                r.result = (ll, lm)
+        else:
+            k = 0
+            mask = slot.h_mask.array.view(np.uint64)
+            acc = slot.h_acc.array
+            for r in slot.requests:
+                m = len(r.offsets) - 1
+                r.result = (mask[k:k + m].copy(), acc[k:k + m].copy())
+                k += m
+        reqs = slot.requests
+        slot.requests = []
+        with self._lock:
+            self._release(reqs)
+            slot.busy = False
+            self.completed.extend(reqs)
+        for r in reqs:
+            r.done = now
+            r._event.set()
