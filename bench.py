@@ -116,9 +116,12 @@ def main() -> int:
         return 2
     legs = [x for x in args.legs.split(",") if x]
     if "serve" in legs:
-        # one hardware queue per server stream (copy, two DFA/NFA lanes, four probabilistic
-        # slots), set before the HIP runtime starts: streams that share a queue serialise
-        os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+        # one hardware queue per server stream (a copy stream, two DFA/NFA lanes, four
+        # probabilistic slots), the bench's own and the default stream, set before the
+        # HIP runtime starts: streams that share a queue run in order, so a small
+        # request's tick would wait behind a long probabilistic launch. Set outright:
+        # an environment that exports HIP's default of 4 would otherwise keep it.
+        os.environ["GPU_MAX_HW_QUEUES"] = str(max(12, int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0)))
 
     from trex_amd import _lib
     from trex_amd.device import DeviceBuffer, Stream

@@ -65,6 +65,12 @@ int trex_host_free(void* h_ptr);
 int trex_memcpy_h2d(void* d_dst, const void* src, size_t n, void* stream);
 int trex_memcpy_d2h(void* dst, const void* d_src, size_t n, void* stream);
 int trex_memcpy_d2d(void* d_dst, const void* d_src, size_t n, void* stream);
+/* n bytes copied by a kernel on `stream` rather than a copy engine, so that a
+ * short copy does not queue behind long ones on the engines. dst: device
+ * memory; src: device memory or pinned host memory (trex_host_alloc); both
+ * 16-byte aligned. No reference counterpart (its string uploads are
+ * synchronous cudaMemcpy calls, cuda/monoid_batch.cu:506-507). */
+int trex_memcpy_kernel(void* d_dst, const void* src, int64_t n, void* stream);
 int trex_stream_create(void** stream);
 int trex_stream_destroy(void* stream);
 int trex_stream_sync(void* stream);
@@ -203,12 +209,19 @@ int trex_pnfa_set_mode(trex_pnfa* h, int mode);
 /* Waves per 64-sequence workgroup: 4 (eight symbols' matrices per wave) or 8
  * (four per wave, two waves per SIMD). */
 int trex_pnfa_set_waves(trex_pnfa* h, int waves);
+/* Most workgroups per kernel launch (0, the default: one per group of 64
+ * sequences). With fewer, the groups go out in consecutive launches on the
+ * stream, which bounds the compute units a long call holds while other work
+ * shares the device; put long sequences first. No counterpart in the
+ * reference (it has no probabilistic kernel). */
+int trex_pnfa_set_grid(trex_pnfa* h, int64_t workgroups);
 /* The operator a mode applies on every step but the last: the rounded T as FP32
  * [n_symbols][64][64] and the column mass factors [n_symbols][64], so that
  * s' = T_rounded (col_scale[c] * s). Either output may be null. */
 int trex_pnfa_rounded(const trex_pnfa* h, int mode, float* t_rounded, float* col_scale);
 /* d_symbols: [n_seq][length] uint8 symbol indices; d_ll, d_logmass: [n_seq]
- * FP64. Asynchronous on `stream`. length <= 2^31 - 1. */
+ * FP64, only written (pinned host memory works too). Asynchronous on
+ * `stream`. length <= 2^31 - 1. */
 int trex_pnfa_loglik(trex_pnfa* h, const uint8_t* d_symbols, int64_t n_seq, int64_t length, double* d_ll,
                      double* d_logmass, void* stream);
 /* Ragged sequences: sequence j = d_symbols[d_begin[j] .. d_end[j]) (lengths
@@ -243,15 +256,21 @@ int trex_mix_destroy(trex_mix* m);
 /* String i = bytes[begin[i] .. end[i]) is matched against pattern
  * pattern_of[i]. Strings longer than dfa_split / nfa_split bytes are cut into
  * chunks of dfa_chunk (multiple of 16) / nfa_chunk (multiple of ck) bytes;
- * NFA chunks record their state set every ck bytes (ck multiple of 16). */
+ * NFA chunks record their state set every ck bytes (ck multiple of 16).
+ * Host work only: the plan is staged in pinned memory and copied to the
+ * device, asynchronously and by a copy kernel, by the next trex_mix_run or
+ * trex_mix_capture on that call's stream. A handle holds one plan on the device: runs of one handle on
+ * different streams must be ordered by the caller. */
 int trex_mix_plan(trex_mix* m, const int64_t* begin, const int64_t* end, const int32_t* pattern_of, int64_t n_strings,
                   int64_t dfa_split, int64_t dfa_chunk, int64_t nfa_split, int64_t nfa_chunk, int32_t ck);
 int trex_mix_info(const trex_mix* m, int64_t* n_items, int64_t* n_batches, int64_t* n_chunks, int64_t* n_fixups);
 /* Copies the plan (items, then batches in queue order); any pointer may be NULL. */
 int trex_mix_layout(const trex_mix* m, int64_t* ibeg, int64_t* iend, int32_t* ia, int32_t* ib, uint32_t* iflags,
                     int32_t* bfirst, int32_t* bcount, int32_t* bpattern, int32_t* brole);
-/* One launch sequence on `stream` (a fill kernel for the queue counters, then
- * the persistent kernel): d_out_mask[i] = DFA final state or NFA final set,
+/* One launch sequence on `stream` (the plan's copy after a new plan, a fill
+ * kernel for the queue counters, then the persistent kernel). The outputs are
+ * only written, so they may be pinned host memory (trex_host_alloc): the
+ * results then need no copy back. d_out_mask[i] = DFA final state or NFA final set,
  * d_out_accept[i] = accept bit, for every planned string. max_spins bounds
  * every wait (0: default). */
 int trex_mix_run(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, uint8_t* d_out_accept, uint32_t max_spins,

@@ -364,6 +364,31 @@ def test_gpu_ragged_sequences(gpu_lib, waves):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("grid", [1, 3])
+def test_gpu_grid_cap_same_results(gpu_lib, grid):
+    """A grid cap splits a call into consecutive launches of at most `grid`
+    workgroups: uniform and ragged results are bit-identical to one launch."""
+    rng = np.random.default_rng(23)
+    T, start, accept = dirichlet_pnfa(rng, 32)
+    eng = _engine(T, start, accept)
+    sy = rng.integers(0, 32, size=(64 * 7 + 5, 129)).astype(np.uint8)
+    lengths = rng.integers(0, 300, size=64 * 5 + 17)
+    offsets = np.zeros(len(lengths) + 1, np.int64)
+    np.cumsum(lengths, out=offsets[1:])
+    data = rng.integers(0, 32, size=int(offsets[-1])).astype(np.uint8)
+    ll0, lm0, _, _ = eng.loglik(sy)
+    rl0, rm0 = eng.loglik_ragged(data, offsets)
+    eng.set_grid(grid)
+    ll1, lm1, _, _ = eng.loglik(sy)
+    rl1, rm1 = eng.loglik_ragged(data, offsets)
+    for a, b in ((ll0, ll1), (lm0, lm1), (rl0, rl1), (rm0, rm1)):
+        assert np.array_equal(a, b)
+    from trex_amd._lib import TrexError
+    with pytest.raises(TrexError):
+        eng.set_grid(-1)
+
+
+@pytest.mark.gpu
 def test_gpu_device_pointer_dispatch(gpu_lib):
     from trex_amd.device import DeviceBuffer, Stream
     rng = np.random.default_rng(16)

@@ -78,3 +78,31 @@ def test_gpu_served_equals_alone(gpu_lib):
             assert np.array_equal(r.result[0].astype(final.dtype), final), r.pattern
             assert np.array_equal(r.result[1], acc)
         assert r.arrival <= r.admitted <= r.launched <= r.done
+
+
+@pytest.mark.gpu
+def test_gpu_kernel_copy_pinned_and_device(gpu_lib):
+    """trex_memcpy_kernel: pinned host -> device and device -> device copies of
+    sizes around the 16-byte vector and the grid limit are exact; unaligned or
+    pageable sources are rejected before any launch."""
+    from trex_amd import _lib
+    from trex_amd.device import DeviceBuffer, PinnedBuffer, Stream
+    rng = np.random.default_rng(5)
+    stream = Stream()
+    n_max = 1024 * 256 * 16 + 77          # beyond one grid's worth of vectors, with a tail
+    src = PinnedBuffer(n_max)
+    src.array[:] = rng.integers(0, 256, size=n_max, dtype=np.uint8)
+    d = DeviceBuffer(n_max + 32)
+    d2 = DeviceBuffer(n_max + 32)
+    for n in (1, 15, 16, 17, 4096 + 3, n_max):
+        _lib.call("trex_memcpy_kernel", d.ptr, src.ptr, n, stream.handle)
+        _lib.call("trex_memcpy_kernel", d2.ptr, d.ptr, n, stream.handle)
+        got = d2.download(np.uint8, n, stream)
+        assert np.array_equal(got, src.array[:n]), n
+    from trex_amd._lib import TrexError
+    with pytest.raises(TrexError):
+        _lib.call("trex_memcpy_kernel", d.ptr + 1, src.ptr, 64, stream.handle)
+    pageable = np.zeros(4096, np.uint8)
+    if pageable.ctypes.data % 16 == 0:
+        with pytest.raises(TrexError):
+            _lib.call("trex_memcpy_kernel", d.ptr, pageable.ctypes.data, 4096, stream.handle)

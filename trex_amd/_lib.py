@@ -40,6 +40,7 @@ _SIGNATURES = {
     "trex_memcpy_h2d": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "trex_memcpy_d2h": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "trex_memcpy_d2d": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
+    "trex_memcpy_kernel": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
     "trex_stream_create": (c_int, [vpp]),
     "trex_stream_destroy": (c_int, [c_void_p]),
     "trex_stream_sync": (c_int, [c_void_p]),
@@ -77,6 +78,7 @@ _SIGNATURES = {
     "trex_pnfa_destroy": (c_int, [c_void_p]),
     "trex_pnfa_info": (c_int, [c_void_p, intp, intp, intp, intp]),
     "trex_pnfa_set_waves": (c_int, [c_void_p, c_int]),
+    "trex_pnfa_set_grid": (c_int, [c_void_p, c_int64]),
     "trex_pnfa_set_mode": (c_int, [c_void_p, c_int]),
     "trex_pnfa_rounded": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "trex_pnfa_loglik": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p]),

@@ -97,11 +97,6 @@ int trex_dfa_create(const uint8_t* delta, const uint8_t* accept, int n_states, i
         rc = upload((void**)&h->d_t2, t2.data(), t2.size());
         if (!rc) rc = upload((void**)&h->d_f2, f2.data(), f2.size() * sizeof(uint64_t));
     }
-    if (!rc) {
-        hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
-        for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&h->ev[i]);
-        if (e != hipSuccess) rc = hip_fail(e, "stream/event create");
-    }
     if (rc) { trex_dfa_destroy(h); return rc; }
     if (h->window_base >= 0) h->auto_variant = TREX_DFA_K2_WINDOW_F;
     else if (n_states <= 16) h->auto_variant = TREX_DFA_K1_16;
@@ -212,6 +207,7 @@ int trex_dfa_match_host(trex_dfa* h, const uint8_t* bytes, const int64_t* offset
     std::vector<int64_t> rebased((size_t)n_strings + 1);
     for (int64_t i = 0; i <= n_strings; ++i) rebased[i] = offsets[i] - base;
 
+    TREX_TRY(ensure_host_dispatch(h->stream, h->ev));
     hipStream_t s = h->stream;
     TREX_HIP(hipEventRecord(h->ev[0], s), "hipEventRecord");
     if (total > 0) TREX_HIP(hipMemcpyAsync(h->in_bytes.ptr, bytes + base, (size_t)total, hipMemcpyHostToDevice, s), "H2D bytes");

@@ -110,7 +110,19 @@ struct trex_mix {
     bool has_device = false;
     MixPattern* d_pats = nullptr;
     Plan plan;
-    DevBuf d_ibeg, d_iend, d_ia, d_ib, d_ic, d_iflags, d_bfirst, d_bcount, d_bpattern, d_brole;
+    // The plan's ten arrays, packed at 256-byte aligned offsets: built in
+    // pinned host memory by trex_mix_plan, copied to d_plan by the next run on
+    // that run's stream, by a copy kernel (no synchronous call while other work
+    // is on the device, and no wait behind long copies on the copy engines).
+    uint8_t* h_stage = nullptr;
+    size_t h_cap = 0, plan_bytes = 0;
+    std::vector<uint8_t*> h_retired;   // outgrown staging, freed with the handle (hipHostFree synchronises)
+    size_t off[10] = {};
+    size_t need[5] = {};               // bytes of d_plan, d_words, d_agg, d_ckpt, d_exit for the staged plan
+    bool dirty = false;
+    hipEvent_t staged = nullptr;   // recorded behind the last upload from h_stage
+    bool staged_pending = false;
+    DevBuf d_plan;
     DevBuf d_words;   // ticket, error, converged, done[n_slots], status[n_items]
     DevBuf d_agg, d_ckpt, d_exit;
     hipGraphExec_t exec = nullptr;
@@ -127,10 +139,74 @@ void drop_graph(trex_mix* m) {
     m->graph = nullptr;
 }
 
-template <class T>
-int upload_vec(DevBuf& buf, const std::vector<T>& v) {
-    TREX_TRY(buf.reserve(std::max<size_t>(v.size() * sizeof(T), 16)));
-    if (!v.empty()) TREX_HIP(hipMemcpy(buf.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "H2D plan");
+// Grows a plan buffer in stream order (hipFreeAsync / hipMallocAsync on the
+// run's stream), by at least half its capacity: unlike hipFree, nothing here
+// waits for the work of other streams.
+int grow_async(DevBuf& buf, size_t n, hipStream_t s) {
+    if (n <= buf.cap) return TREX_OK;
+    const size_t want = std::max(n, buf.cap + buf.cap / 2);
+    if (buf.ptr) TREX_HIP(hipFreeAsync(buf.ptr, s), "hipFreeAsync");
+    buf.ptr = nullptr;
+    buf.cap = 0;
+    TREX_HIP(hipMallocAsync(&buf.ptr, want, s), "hipMallocAsync");
+    buf.cap = want;
+    return TREX_OK;
+}
+
+void release_async(DevBuf& buf) {
+    if (buf.ptr) (void)hipFreeAsync(buf.ptr, nullptr);
+    buf.ptr = nullptr;
+    buf.cap = 0;
+}
+
+// Packs the plan's arrays into h_stage and sizes the device buffers.
+int stage_plan(trex_mix* m) {
+    const Plan& Q = m->plan;
+    auto bytes = [](const auto& v) { return v.size() * sizeof(v[0]); };
+    const size_t sizes[10] = {bytes(Q.ibeg), bytes(Q.iend), bytes(Q.ia), bytes(Q.ib), bytes(Q.ic),
+                              bytes(Q.iflags), bytes(Q.bfirst), bytes(Q.bcount), bytes(Q.bpattern), bytes(Q.brole)};
+    const void* srcs[10] = {Q.ibeg.data(), Q.iend.data(), Q.ia.data(), Q.ib.data(), Q.ic.data(),
+                            Q.iflags.data(), Q.bfirst.data(), Q.bcount.data(), Q.bpattern.data(), Q.brole.data()};
+    size_t total = 0;
+    for (int i = 0; i < 10; ++i) {
+        m->off[i] = total;
+        total += (sizes[i] + 255) & ~(size_t)255;
+    }
+    total = std::max<size_t>(total, 256);
+    if (m->staged_pending) {   // the last upload may still read h_stage
+        TREX_HIP(hipEventSynchronize(m->staged), "hipEventSynchronize");
+        m->staged_pending = false;
+    }
+    if (total > m->h_cap) {
+        const size_t want = std::max(total, m->h_cap + m->h_cap / 2);
+        uint8_t* p = nullptr;
+        TREX_HIP(hipHostMalloc((void**)&p, want, hipHostMallocDefault), "hipHostMalloc plan staging");
+        if (m->h_stage) m->h_retired.push_back(m->h_stage);
+        m->h_stage = p;
+        m->h_cap = want;
+    }
+    for (int i = 0; i < 10; ++i)
+        if (sizes[i]) std::memcpy(m->h_stage + m->off[i], srcs[i], sizes[i]);
+    m->plan_bytes = total;
+    m->need[0] = total;
+    m->need[1] = 4 * (3 + (size_t)Q.n_slots + Q.items());
+    m->need[2] = 8 * std::max<size_t>(Q.items(), 1);
+    m->need[3] = 8 * std::max<size_t>((size_t)Q.n_spec * Q.n_ck, 1);
+    m->need[4] = 8 * std::max<size_t>((size_t)Q.n_spec, 1);
+    m->dirty = true;
+    return TREX_OK;
+}
+
+// Enqueues the copy of a freshly staged plan on `s`.
+int upload_plan(trex_mix* m, hipStream_t s) {
+    if (!m->dirty) return TREX_OK;
+    if (!m->staged) TREX_HIP(hipEventCreateWithFlags(&m->staged, hipEventDisableTiming), "hipEventCreate");
+    DevBuf* bufs[5] = {&m->d_plan, &m->d_words, &m->d_agg, &m->d_ckpt, &m->d_exit};
+    for (int i = 0; i < 5; ++i) TREX_TRY(grow_async(*bufs[i], m->need[i], s));
+    TREX_TRY(trex_memcpy_kernel(m->d_plan.ptr, m->h_stage, (int64_t)m->plan_bytes, s));
+    TREX_HIP(hipEventRecord(m->staged, s), "hipEventRecord");
+    m->staged_pending = true;
+    m->dirty = false;
     return TREX_OK;
 }
 
@@ -196,10 +272,13 @@ int trex_mix_create(const int32_t* kinds, const void* const* handles, int32_t n_
 int trex_mix_destroy(trex_mix* m) {
     if (!m) return TREX_OK;
     drop_graph(m);
+    if (m->staged_pending) hipEventSynchronize(m->staged);
+    if (m->staged) hipEventDestroy(m->staged);
+    if (m->h_stage) hipHostFree(m->h_stage);
+    for (uint8_t* p : m->h_retired) hipHostFree(p);
     hipFree(m->d_pats);
-    for (DevBuf* b : {&m->d_ibeg, &m->d_iend, &m->d_ia, &m->d_ib, &m->d_ic, &m->d_iflags, &m->d_bfirst, &m->d_bcount,
-                      &m->d_bpattern, &m->d_brole, &m->d_words, &m->d_agg, &m->d_ckpt, &m->d_exit})
-        b->release();
+    for (DevBuf* b : {&m->d_plan, &m->d_words, &m->d_agg, &m->d_ckpt, &m->d_exit})
+        release_async(*b);
     delete m;
     return TREX_OK;
 }
@@ -328,23 +407,7 @@ int trex_mix_plan(trex_mix* m, const int64_t* begin, const int64_t* end, const i
         }
     m->plan = std::move(P);
 
-    if (m->has_device) {
-        const Plan& Q = m->plan;
-        TREX_TRY(upload_vec(m->d_ibeg, Q.ibeg));
-        TREX_TRY(upload_vec(m->d_iend, Q.iend));
-        TREX_TRY(upload_vec(m->d_ia, Q.ia));
-        TREX_TRY(upload_vec(m->d_ib, Q.ib));
-        TREX_TRY(upload_vec(m->d_ic, Q.ic));
-        TREX_TRY(upload_vec(m->d_iflags, Q.iflags));
-        TREX_TRY(upload_vec(m->d_bfirst, Q.bfirst));
-        TREX_TRY(upload_vec(m->d_bcount, Q.bcount));
-        TREX_TRY(upload_vec(m->d_bpattern, Q.bpattern));
-        TREX_TRY(upload_vec(m->d_brole, Q.brole));
-        TREX_TRY(m->d_words.reserve(4 * (3 + (size_t)Q.n_slots + Q.items())));
-        TREX_TRY(m->d_agg.reserve(8 * std::max<size_t>(Q.items(), 1)));
-        TREX_TRY(m->d_ckpt.reserve(8 * std::max<size_t>((size_t)Q.n_spec * Q.n_ck, 1)));
-        TREX_TRY(m->d_exit.reserve(8 * std::max<size_t>((size_t)Q.n_spec, 1)));
-    }
+    if (m->has_device) TREX_TRY(stage_plan(m));
     return TREX_OK;
 }
 
@@ -382,6 +445,7 @@ int trex_mix_run(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, uint
     if (Q.bfirst.empty()) return TREX_OK;
     if (!d_bytes || !d_out_mask || !d_out_accept) return fail(TREX_ERR_INVALID, "null buffer");
     hipStream_t s = (hipStream_t)stream;
+    TREX_TRY(upload_plan(m, s));
     uint32_t* words = m->d_words.as<uint32_t>();
     int64_t n_words = 3 + Q.n_slots + (int64_t)Q.items();
     uint32_t zero = 0;
@@ -393,18 +457,19 @@ int trex_mix_run(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, uint
     MixArgs a;
     a.bytes = d_bytes;
     a.patterns = m->d_pats;
-    a.bfirst = m->d_bfirst.as<int32_t>();
-    a.bcount = m->d_bcount.as<int32_t>();
-    a.bpattern = m->d_bpattern.as<int32_t>();
-    a.brole = m->d_brole.as<int32_t>();
+    uint8_t* pb = m->d_plan.as<uint8_t>();
+    a.ibeg = reinterpret_cast<const int64_t*>(pb + m->off[0]);
+    a.iend = reinterpret_cast<const int64_t*>(pb + m->off[1]);
+    a.ia = reinterpret_cast<const int32_t*>(pb + m->off[2]);
+    a.ib = reinterpret_cast<const int32_t*>(pb + m->off[3]);
+    a.ic = reinterpret_cast<const int32_t*>(pb + m->off[4]);
+    a.iflags = reinterpret_cast<const uint32_t*>(pb + m->off[5]);
+    a.bfirst = reinterpret_cast<const int32_t*>(pb + m->off[6]);
+    a.bcount = reinterpret_cast<const int32_t*>(pb + m->off[7]);
+    a.bpattern = reinterpret_cast<const int32_t*>(pb + m->off[8]);
+    a.brole = reinterpret_cast<const int32_t*>(pb + m->off[9]);
     a.n_batches = (int32_t)Q.bfirst.size();
     a.ck = Q.ck;
-    a.ibeg = m->d_ibeg.as<int64_t>();
-    a.iend = m->d_iend.as<int64_t>();
-    a.ia = m->d_ia.as<int32_t>();
-    a.ib = m->d_ib.as<int32_t>();
-    a.ic = m->d_ic.as<int32_t>();
-    a.iflags = m->d_iflags.as<uint32_t>();
     a.n_ck = Q.n_ck;
     a.ticket = words;
     a.error = words + 1;
@@ -457,6 +522,7 @@ int trex_mix_capture(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, 
     if (!stream) return fail(TREX_ERR_INVALID, "graph capture needs a non-default stream");
     drop_graph(m);
     hipStream_t s = (hipStream_t)stream;
+    if (m->has_device) TREX_TRY(upload_plan(m, s));   // before the capture: replays launch only the kernels
     TREX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     const int rc = trex_mix_run(m, d_bytes, d_out_mask, d_out_accept, max_spins, stream);
     hipGraph_t g = nullptr;

@@ -105,11 +105,6 @@ int trex_nfa_create(const uint64_t* succ, int n_symbols, const int16_t* byte_to_
         rc = upload((void**)&h->d_tab_win, tw_lds.data(), tw_lds.size() * 8);
         if (!rc) rc = upload((void**)&h->d_cls_win, cw.data(), cw.size());
     }
-    if (!rc) {
-        hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
-        for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&h->ev[i]);
-        if (e != hipSuccess) rc = hip_fail(e, "stream/event create");
-    }
     if (rc) { trex_nfa_destroy(h); return rc; }
     h->variant = h->window_base >= 0 ? TREX_NFA_WINDOW : TREX_NFA_BYTES;
     *out = h;
@@ -194,6 +189,7 @@ int trex_nfa_match_host(trex_nfa* h, const uint8_t* bytes, const int64_t* offset
     TREX_TRY(h->out_accept.reserve((size_t)n_strings));
     std::vector<int64_t> rebased((size_t)n_strings + 1);
     for (int64_t i = 0; i <= n_strings; ++i) rebased[i] = offsets[i] - base;
+    TREX_TRY(ensure_host_dispatch(h->stream, h->ev));
     hipStream_t s = h->stream;
     TREX_HIP(hipEventRecord(h->ev[0], s), "hipEventRecord");
     if (total > 0) TREX_HIP(hipMemcpyAsync(h->in_bytes.ptr, bytes + base, (size_t)total, hipMemcpyHostToDevice, s), "H2D bytes");

@@ -100,6 +100,7 @@ struct trex_pnfa {
     int n_symbols = 0;
     int mode = TREX_PNFA_F16X2;
     int waves = 8;           // waves per workgroup (4: eight symbols per wave, 8: four)
+    int64_t grid = 0;        // most workgroups per kernel launch (0: one per group of 64 sequences)
     bool f16_ok = true;      // every |T| fits binary16
     int e0 = 0;              // s0 = s0n * 2^e0 with max |s0n| in [0.5, 1)
     std::vector<uint16_t> q16, qb16;
@@ -157,11 +158,6 @@ int trex_pnfa_create(const float* T, int n_symbols, const float* start, const fl
     if (!rc) rc = upload((void**)&h->d_t32, T, n * 4);
     if (!rc) rc = upload((void**)&h->d_s0, s0n.data(), kStates * 4);
     if (!rc) rc = upload((void**)&h->d_acc, accept, kStates * 4);
-    if (!rc) {
-        hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
-        for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&h->ev[i]);
-        if (e != hipSuccess) rc = hip_fail(e, "stream/event create");
-    }
     if (rc) { trex_pnfa_destroy(h); return rc; }
     *out = h;
     return TREX_OK;
@@ -210,6 +206,13 @@ int trex_pnfa_set_waves(trex_pnfa* h, int waves) {
     return TREX_OK;
 }
 
+int trex_pnfa_set_grid(trex_pnfa* h, int64_t workgroups) {
+    if (!h) return fail(TREX_ERR_INVALID, "null handle");
+    if (workgroups < 0 || workgroups > 0x7fffffff) return fail(TREX_ERR_INVALID, "workgroups out of range");
+    h->grid = workgroups;
+    return TREX_OK;
+}
+
 int trex_pnfa_rounded(const trex_pnfa* h, int mode, float* t_rounded, float* col_scale) {
     if (!h) return fail(TREX_ERR_INVALID, "null handle");
     if (!mode_name(mode)) return fail(TREX_ERR_INVALID, "unknown mode %d", mode);
@@ -232,11 +235,26 @@ int pnfa_launch(trex_pnfa* h, const uint8_t* d_symbols, const int64_t* d_begin, 
     const uint16_t* tlp = f16 ? h->d_q16 : h->d_qb16;
     const float* scale = f16 ? h->d_scale16 : h->d_scaleb16;
     int n_sym = h->n_symbols, e0 = h->e0;
-    void* args[] = {&tlp, &scale, &h->d_t32, &n_sym, &h->d_s0, &e0, &h->d_acc, &d_symbols, &d_begin, &d_end,
-                    &n_seq, &L, &d_ll, &d_logmass};
     char name[48];
     std::snprintf(name, sizeof(name), "pnfa64_%s_w%d", mode_name(h->mode), h->waves);
-    return launch(name, (unsigned)groups, 64u * (unsigned)h->waves, 0, (hipStream_t)stream, args);
+    // A grid cap splits the groups over consecutive launches on the stream.
+    // Callers put long sequences first, so the first launch's longest group
+    // sets the pace as it would for one workgroup taking every cap-th group.
+    const int64_t per = h->grid > 0 ? h->grid : groups;
+    for (int64_t g0 = 0; g0 < groups; g0 += per) {
+        const int64_t first = g0 * kSeqPerGroup;
+        int64_t n = std::min<int64_t>(n_seq - first, per * kSeqPerGroup);
+        const uint8_t* sym = d_begin ? d_symbols : d_symbols + (d_symbols ? first * (int64_t)L : 0);
+        const int64_t* b = d_begin ? d_begin + first : nullptr;
+        const int64_t* e = d_end ? d_end + first : nullptr;
+        double* ll = d_ll + first;
+        double* lm = d_logmass + first;
+        void* args[] = {&tlp, &scale, &h->d_t32, &n_sym, &h->d_s0, &e0, &h->d_acc, &sym, &b, &e,
+                        &n, &L, &ll, &lm};
+        const unsigned grid = (unsigned)((n + kSeqPerGroup - 1) / kSeqPerGroup);
+        TREX_TRY(launch(name, grid, 64u * (unsigned)h->waves, 0, (hipStream_t)stream, args));
+    }
+    return TREX_OK;
 }
 
 }  // namespace
@@ -274,6 +292,7 @@ int trex_pnfa_loglik_host(trex_pnfa* h, const uint8_t* symbols, int64_t n_seq, i
     TREX_TRY(h->in_sym.reserve(n_sym_bytes + 16));
     TREX_TRY(h->out_ll.reserve(sizeof(double) * (size_t)n_seq));
     TREX_TRY(h->out_lm.reserve(sizeof(double) * (size_t)n_seq));
+    TREX_TRY(ensure_host_dispatch(h->stream, h->ev));
     hipStream_t s = h->stream;
     TREX_HIP(hipEventRecord(h->ev[0], s), "hipEventRecord");
     if (n_sym_bytes) TREX_HIP(hipMemcpyAsync(h->in_sym.ptr, symbols, n_sym_bytes, hipMemcpyHostToDevice, s), "H2D symbols");

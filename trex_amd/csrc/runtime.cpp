@@ -159,6 +159,23 @@ int device_cu_count() {
     return cached;
 }
 
+int ensure_host_dispatch(hipStream_t& stream, hipEvent_t (&ev)[4]) {
+    if (stream) return TREX_OK;
+    hipStream_t s = nullptr;
+    TREX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    for (int i = 0; i < 4; ++i) {
+        const hipError_t e = hipEventCreate(&ev[i]);
+        if (e != hipSuccess) {
+            for (int j = 0; j < i; ++j) { hipEventDestroy(ev[j]); ev[j] = nullptr; }
+            ev[i] = nullptr;
+            hipStreamDestroy(s);
+            return hip_fail(e, "hipEventCreate");
+        }
+    }
+    stream = s;
+    return TREX_OK;
+}
+
 int upload(void** dptr, const void* src, size_t n) {
     hipError_t e = hipMalloc(dptr, std::max<size_t>(n, 16));
     if (e != hipSuccess) { *dptr = nullptr; return hip_fail(e, "hipMalloc"); }
@@ -307,6 +324,27 @@ int trex_device_info(char* name, int n, int* cu_count, int64_t* hbm_bytes) {
     if (cu_count) *cu_count = prop.multiProcessorCount;
     if (hbm_bytes) *hbm_bytes = (int64_t)prop.totalGlobalMem;
     return TREX_OK;
+}
+
+int trex_memcpy_kernel(void* dst, const void* src, int64_t n, void* stream) {
+    if (n < 0) return fail(TREX_ERR_INVALID, "negative size");
+    if (n == 0) return TREX_OK;
+    if (!dst || !src || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+        return fail(TREX_ERR_INVALID, "copy_bytes needs 16-byte aligned, non-null pointers");
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, src) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(TREX_ERR_INVALID, "source is neither device memory nor pinned host memory");
+    }
+    if (a.type != hipMemoryTypeHost && a.type != hipMemoryTypeDevice)
+        return fail(TREX_ERR_INVALID, "source is neither device memory nor pinned host memory (type %d)", (int)a.type);
+    // Pinned host memory is mapped into the device's address space at its host address.
+    const int64_t n16 = n >> 4;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n16 + 255) / 256, 1024));
+    const uint8_t* s = static_cast<const uint8_t*>(src);
+    uint8_t* d = static_cast<uint8_t*>(dst);
+    void* args[] = {&d, &s, &n};
+    return launch("copy_bytes", grid, 256, 0, (hipStream_t)stream, args);
 }
 
 int trex_gen_symbols(void* d_out, int64_t n_bytes, uint64_t seed, uint32_t lo, uint32_t span, void* stream) {

@@ -23,6 +23,12 @@ int launch(const char* name, unsigned grid, unsigned block, unsigned lds, hipStr
 // hipMalloc + synchronous copy of a host table.
 int upload(void** dptr, const void* src, size_t n);
 
+// Creates the stream and four timing events of a handle's synchronous host
+// entry point on its first use (call with the handle's mutex held). A process
+// that keeps many patterns resident and drives them on its own streams never
+// creates them, so they take none of the runtime's hardware queues.
+int ensure_host_dispatch(hipStream_t& stream, hipEvent_t (&ev)[4]);
+
 struct DevBuf {
     void* ptr = nullptr;
     size_t cap = 0;

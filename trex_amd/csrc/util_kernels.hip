@@ -37,3 +37,19 @@ gen_symbols(uint4* __restrict__ out, int64_t n16, uint64_t seed, uint32_t lo, ui
                             sym4((uint32_t)b, lo4, mask4), sym4((uint32_t)(b >> 32), lo4, mask4));
     }
 }
+
+// Copies n bytes with the shader path instead of a copy engine, so a short
+// copy never queues behind long ones on the engines. dst and src are 16-byte
+// aligned; src may be pinned host memory mapped into the device's address
+// space. Grid-stride over 16-byte vectors, then the tail bytes.
+extern "C" __global__ void __launch_bounds__(256)
+copy_bytes(uint8_t* __restrict__ dst, const uint8_t* __restrict__ src, int64_t n)
+{
+    const int64_t n16 = n >> 4;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint4* s4 = reinterpret_cast<const uint4*>(src);
+    uint4* d4 = reinterpret_cast<uint4*>(dst);
+    for (int64_t v = t; v < n16; v += stride) d4[v] = s4[v];
+    if (t < (n & 15)) dst[(n16 << 4) + t] = src[(n16 << 4) + t];
+}

@@ -295,6 +295,11 @@ class ProbNfaEngine:
     def set_waves(self, waves: int) -> None:
         _lib.call("trex_pnfa_set_waves", self.handle, int(waves))
 
+    def set_grid(self, workgroups: int) -> None:
+        """Caps the workgroups of one kernel launch (0: one per 64 sequences);
+        the groups of 64 sequences then go out in consecutive launches."""
+        _lib.call("trex_pnfa_set_grid", self.handle, int(workgroups))
+
     def set_mode(self, mode: str) -> None:
         if mode not in PNFA_MODES:
             raise ValueError(f"unknown mode {mode!r}; choose from {sorted(PNFA_MODES)}")

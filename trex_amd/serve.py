@@ -8,7 +8,10 @@ Boolean NFA-64 and probabilistic NFA-64 engines) and serves match requests
   (on a copy stream, overlapping running work) as soon as the arena has room;
   requests that have not found room wait in FIFO order per lane. Small
   requests have an arena of their own, so a backlog of large ones never
-  blocks their admission;
+  blocks their admission; they are staged in pinned memory and copied by a
+  kernel on their lane's stream just ahead of their tick, so they neither
+  queue behind the copies of large ones on the copy engines nor wait on
+  another stream;
 * four lanes, each with its own stream: DFA/NFA requests of at most
   ``small_bytes`` bytes, larger DFA/NFA requests, and the same two classes of
   probabilistic requests. A lane launches a *tick* -- every admitted request
@@ -16,22 +19,29 @@ Boolean NFA-64 and probabilistic NFA-64 engines) and serves match requests
   so a new tick queues behind the running one and requests that arrive while
   others are on the GPU join the next tick;
 * a DFA/NFA tick is one persistent mixed-queue launch (:mod:`trex_amd.mix`);
-  the bulk lane's persistent grid is one workgroup per compute unit and it
-  keeps one tick in flight (a second kernel on the same stream would be
-  dispatched onto the remaining room as the first one's workgroups are
-  placed), so a small-request tick always finds room beside it. A probabilistic request is a tick of
-  its own (one ragged launch); each probabilistic lane has ``prob_slots``
-  slots with a stream each, so a few long sequential launches overlap one
-  another while the number of compute units they hold stays bounded;
-* results are copied back into pinned host memory on the lane stream behind
-  the kernels; a tick is complete when its event has fired, and then every
-  request of the tick gets its result and completion time, and its arena space
-  is released.
+  the bulk lane's persistent grid is one workgroup on half the compute units
+  and it keeps one tick in flight (a second kernel on the same stream would
+  be dispatched onto the remaining room as the first one's workgroups are
+  placed), so a small-request tick always finds room beside it. A
+  probabilistic request is a tick of its own (one ragged call, at most
+  ``prob_grid`` workgroups per kernel launch, set on every probabilistic
+  engine the server holds); each probabilistic lane has ``prob_slots`` slots
+  with a stream each, so a few long sequential launches overlap one another
+  while the compute units they hold stay bounded;
+* no call that synchronises the whole device (``hipFree``, a synchronous
+  copy) runs while serving: tick buffers are reserved up front and grow
+  geometrically, and mixed-queue plans are copied asynchronously;
+* tick plans and sequence ranges go to the device by copy kernels, and the
+  kernels write their results straight into pinned host memory, so no tick
+  waits behind the copy engines; a tick is complete when its event has fired,
+  and then every request of the tick gets its result and completion time, and
+  its arena space is released.
 
-The server uses seven streams (copy, two DFA/NFA lanes, two slots in each
-probabilistic lane); streams that share a hardware queue run their kernels in
-order, so a process that serves should start with ``GPU_MAX_HW_QUEUES`` of at
-least 8 (the HIP runtime reads it once, before the first GPU call).
+The server uses seven streams (a copy stream, two DFA/NFA lanes, two slots
+in each probabilistic lane); streams that share a hardware queue run their
+work in order, so a process that serves should start with
+``GPU_MAX_HW_QUEUES`` of at least 8 (the HIP runtime reads it once, before the
+first GPU call; the default stream needs one too).
 
 The reference has no serving path: one synchronous launch per request and
 pattern through a process-global engine (cuda/monoid_batch.cu:596-671).
@@ -40,8 +50,10 @@ pattern through a process-global engine (cuda/monoid_batch.cu:596-671).
 from __future__ import annotations
 
 import collections
+import os
 import threading
 import time
+import warnings
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -126,7 +138,7 @@ class _Slot:
         self.stream = stream
         self.mix = MixEngine(engines_dfa_nfa) if engines_dfa_nfa else None
         self.cap = 0
-        self.d_mask = self.d_acc = self.d_begin = self.d_end = self.d_lm = None
+        self.d_begin = None
         self.h_mask = self.h_acc = None
         self.h_f64 = None
         self.event = Event()
@@ -137,14 +149,13 @@ class _Slot:
         if n <= self.cap:
             return
         cap = max(n, 2 * self.cap, 1024)
-        for name in ("d_mask", "d_acc", "d_begin", "d_end", "d_lm"):
-            b = getattr(self, name)
-            if b is not None:
-                b.free()
-        self.d_mask, self.d_acc = DeviceBuffer(8 * cap), DeviceBuffer(cap)
-        self.d_begin, self.d_end, self.d_lm = DeviceBuffer(8 * cap), DeviceBuffer(8 * cap), DeviceBuffer(8 * cap)
-        self.h_mask, self.h_acc = PinnedBuffer(8 * cap), PinnedBuffer(cap)
-        self.h_f64 = PinnedBuffer(16 * cap)
+        if self.d_begin is not None:
+            self.d_begin.free()
+        if self.mix is not None:   # DFA/NFA results land in pinned memory
+            self.h_mask, self.h_acc = PinnedBuffer(8 * cap), PinnedBuffer(cap)
+        else:                      # probabilistic ranges (begin, end) and results (LL, log-mass)
+            self.d_begin = DeviceBuffer(16 * cap)
+            self.h_f64 = PinnedBuffer(16 * cap)
         self.cap = cap
 
 
@@ -152,7 +163,12 @@ class Server:
     def __init__(self, engines: list, hbm_budget: int = 8 << 30, small_bytes: int = 1 << 20,
                  small_params: MixParams | None = None, bulk_params: MixParams | None = None,
                  bulk_grid: int | None = None, small_budget: int | None = None, prob_slots: int = 2,
-                 bulk_slots: int = 1):
+                 bulk_slots: int = 1, prob_grid: int = 32, reserve_strings: int = 1 << 18):
+        hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4)   # HIP's default is 4
+        if hwq < 8:
+            warnings.warn(f"GPU_MAX_HW_QUEUES is {hwq}: the server's seven streams share hardware queues, so "
+                          "small requests wait behind long launches; start the process with 8 or more",
+                          RuntimeWarning, stacklevel=2)
         self.engines = list(engines)
         self.kind = []
         mix_index = {}
@@ -169,8 +185,9 @@ class Server:
         self.mix_index = mix_index
         self.small_bytes = int(small_bytes)
         self.arena = Arena(hbm_budget)
-        self.small_arena = Arena(small_budget if small_budget is not None else max(64 * self.small_bytes, 1 << 30))
-        self.copy_stream = Stream()
+        self.small_arena = Arena(small_budget if small_budget is not None else max(64 * self.small_bytes, 256 << 20))
+        self.small_stage = PinnedBuffer(self.small_arena.size)   # host mirror of the small arena
+        self.copy_stream = Stream()   # copies of large requests (copy engines)
         cus = _lib.device_info()["cus"]
         self.lanes = {name: {"pending": collections.deque(), "slots": [], "queue": collections.deque()}
                       for name in ("small", "bulk", "prob_small", "prob_bulk")}
@@ -180,13 +197,19 @@ class Server:
             else:
                 stream = Stream()
                 lane["slots"] = [_Slot(mix_engines, stream) for _ in range(bulk_slots if name == "bulk" else 2)]
-            lane["params"] = (small_params or MixParams(dfa_split=64 << 10, dfa_chunk=16 << 10, nfa_split=8 << 10,
-                                                        nfa_chunk=8 << 10, ck=64)) if name == "small" else \
+            lane["params"] = (small_params or MixParams(dfa_split=4 << 10, dfa_chunk=4 << 10, nfa_split=2 << 10,
+                                                        nfa_chunk=2 << 10, ck=64)) if name == "small" else \
                 (bulk_params or MixParams())
         for slot in self.lanes["bulk"]["slots"]:
-            slot.mix.set_grid(bulk_grid if bulk_grid is not None else cus)
+            slot.mix.set_grid(bulk_grid if bulk_grid is not None else max(1, cus // 2))
         for slot in self.lanes["small"]["slots"]:
             slot.mix.set_grid(cus)
+        for lane in self.lanes.values():
+            for slot in lane["slots"]:
+                slot.reserve(reserve_strings)
+        for e in self.engines:
+            if isinstance(e, ProbNfaEngine):
+                e.set_grid(prob_grid)
         self._lock = threading.Lock()   # arenas, lane queues and pending lists
         for lane in self.lanes.values():
             lane["cv"] = threading.Condition(self._lock)
@@ -259,10 +282,18 @@ class Server:
         if off < 0:
             return False
         r._arena, r._arena_obj = off, arena
-        if r.nbytes:
-            _lib.call("trex_memcpy_h2d", arena.buf.ptr + off, r.data.ctypes.data, r.nbytes, self.copy_stream.handle)
-        r._copied = Event()
-        r._copied.record(self.copy_stream)
+        if arena is self.small_arena:
+            # staged on the host now; copied by a kernel on the lane's stream just
+            # ahead of its tick (no wait across streams)
+            if r.nbytes:
+                self.small_stage.array[off:off + r.nbytes] = r.data[:r.nbytes]
+            r._copied = None
+        else:
+            if r.nbytes:
+                _lib.call("trex_memcpy_h2d", arena.buf.ptr + off, r.data.ctypes.data, r.nbytes,
+                          self.copy_stream.handle)
+            r._copied = Event()
+            r._copied.record(self.copy_stream)
         r.admitted = time.perf_counter()
         return True
 
@@ -278,7 +309,10 @@ class Server:
     # -- lane workers ------------------------------------------------------------------
     def _worker(self, name: str) -> None:
         """One thread per lane: launch ticks while a slot is free, complete them
-        in launch order; every wait releases the GIL."""
+        in launch order; every wait releases the GIL. A request joins a tick
+        once its bytes are on the device: the host polls the copy's event, so
+        no stream ever waits on another (such waits have taken tens of
+        milliseconds to resolve on ROCm)."""
         lane = self.lanes[name]
         prob = name.startswith("prob")
         inflight: collections.deque[_Slot] = collections.deque()
@@ -292,15 +326,20 @@ class Server:
                         return
                 slot = next((s for s in lane["slots"] if not s.busy), None)
                 if lane["queue"] and slot is not None:
-                    if prob:
-                        reqs = [lane["queue"].popleft()]   # one ragged launch per tick
-                    else:
-                        reqs = list(lane["queue"])
-                        lane["queue"].clear()
+                    q = lane["queue"]
+                    for r in list(q):
+                        if r._copied is None or r._copied.query():
+                            reqs.append(r)
+                            if prob:   # one ragged launch per tick
+                                break
+                    for r in reqs:
+                        q.remove(r)
             if reqs:
                 stream = slot.stream
                 for r in reqs:
-                    stream.wait(r._copied)
+                    if r._copied is None and r.nbytes:   # small: staged bytes, copied on this stream
+                        _lib.call("trex_memcpy_kernel", r._arena_obj.buf.ptr + r._arena,
+                                  self.small_stage.ptr + r._arena, r.nbytes, stream.handle)
                 if prob:
                     self._launch_prob(slot, reqs, stream)
                 else:
@@ -314,17 +353,13 @@ class Server:
                 inflight.append(slot)
                 self.ticks[name] += 1
                 continue
-            if inflight:
-                oldest = inflight[0]
-                if len(inflight) == len(lane["slots"]):
-                    oldest.event.synchronize()   # every slot busy: wait for the oldest tick
-                elif not oldest.event.query():
-                    with self._lock:
-                        if not lane["queue"]:
-                            lane["cv"].wait(0.0002)
-                    continue
-                inflight.popleft()
-                self._complete(name, oldest)
+            if inflight and (len(inflight) == len(lane["slots"]) and not lane["queue"]):
+                inflight[0].event.synchronize()   # every slot busy, nothing else to watch
+            if inflight and inflight[0].event.query():
+                self._complete(name, inflight.popleft())
+                continue
+            with self._lock:   # a tick or a copy is still running: look again shortly
+                lane["cv"].wait(0.0002)
 
     def _launch_mix(self, slot: _Slot, reqs: list[Request], stream: Stream, params: MixParams) -> None:
         base = reqs[0]._arena_obj.buf.ptr
@@ -334,10 +369,8 @@ class Server:
         n = len(begin)
         slot.reserve(n)
         slot.mix.plan(begin, end, pat, params)
-        if n:
-            slot.mix.run(base, slot.d_mask.ptr, slot.d_acc.ptr, stream)
-            _lib.call("trex_memcpy_d2h", slot.h_mask.ptr, slot.d_mask.ptr, 8 * n, stream.handle)
-            _lib.call("trex_memcpy_d2h", slot.h_acc.ptr, slot.d_acc.ptr, n, stream.handle)
+        if n:   # results written by the kernel into pinned host memory
+            slot.mix.run(base, slot.h_mask.ptr, slot.h_acc.ptr, stream)
 
     def _launch_prob(self, slot: _Slot, reqs: list[Request], stream: Stream) -> None:
         base = reqs[0]._arena_obj.buf.ptr
@@ -357,19 +390,19 @@ class Server:
             k += m
         if n == 0:
             return
+        # ranges: staged in the pinned tick buffer, copied by a kernel into
+        # d_begin (begin, then end); LL and log-mass come back into the same
+        # pinned buffer, written by the kernels after the copy has run
         pin = slot.h_f64.array.view(np.int64)
         pin[:n] = begin
         pin[n:2 * n] = end
-        _lib.call("trex_memcpy_h2d", slot.d_begin.ptr, slot.h_f64.ptr, 8 * n, stream.handle)
-        _lib.call("trex_memcpy_h2d", slot.d_end.ptr, slot.h_f64.ptr + 8 * n, 8 * n, stream.handle)
+        _lib.call("trex_memcpy_kernel", slot.d_begin.ptr, slot.h_f64.ptr, 16 * n, stream.handle)
+        d_end = slot.d_begin.ptr + 8 * n
         for r, (k0, m, _) in zip(reqs, slot.order):
             if m:
-                self.engines[r.pattern].loglik_ranges_device(base, slot.d_begin.ptr + 8 * k0, slot.d_end.ptr + 8 * k0,
-                                                             m, slot.d_mask.ptr + 8 * k0, slot.d_lm.ptr + 8 * k0,
+                self.engines[r.pattern].loglik_ranges_device(base, slot.d_begin.ptr + 8 * k0, d_end + 8 * k0, m,
+                                                             slot.h_f64.ptr + 8 * k0, slot.h_f64.ptr + 8 * (n + k0),
                                                              stream)
-        # LL (d_mask as float64) and log-mass, copied back behind the kernels
-        _lib.call("trex_memcpy_d2h", slot.h_f64.ptr, slot.d_mask.ptr, 8 * n, stream.handle)
-        _lib.call("trex_memcpy_d2h", slot.h_f64.ptr + 8 * n, slot.d_lm.ptr, 8 * n, stream.handle)
 
     def _complete(self, name: str, slot: _Slot) -> None:
         now = time.perf_counter()
