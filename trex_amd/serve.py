@@ -311,8 +311,8 @@ class Server:
         """One thread per lane: launch ticks while a slot is free, complete them
         in launch order; every wait releases the GIL. A request joins a tick
         once its bytes are on the device: the host polls the copy's event, so
-        no stream ever waits on another (such waits have taken tens of
-        milliseconds to resolve on ROCm)."""
+        no stream holds a wait for another (a wait blocks the hardware queue it
+        sits in, which other streams may share)."""
         lane = self.lanes[name]
         prob = name.startswith("prob")
         inflight: collections.deque[_Slot] = collections.deque()
