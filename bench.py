@@ -45,6 +45,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the headline kernel on this workload, from two rocprofv3
+# counter runs (FETCH_SIZE x 2.000 + WRITE_SIZE, both calibrated against a plain
+# 1 GiB copy); the command lines and counts are in the profile named below.
+PMC_TRAFFIC = {("dfa16_k2f_pf8", "staggered"): 1_092_300_000}
+PMC_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, one pass each: "
+                      "profiles/dfa16_k2f_pf8_1GiB_8patterns.md")
 MFMA_F16_PEAK_TFLOPS = 2500.0
 N_STRINGS = 262_144
 LENGTH = 4096
@@ -100,8 +106,8 @@ def main() -> int:
     ap.add_argument("--mix-strings", type=int, default=MIX_STRINGS, help="strings of the uneven-mix leg")
     ap.add_argument("--mix-tune", type=int, default=1, help="autotune the uneven-mix plan (0: defaults)")
     ap.add_argument("--mix-max-spins", type=int, default=0, help="bound of every wait in the mixed queue (0: default)")
-    ap.add_argument("--serve-qps", type=float, default=50.0, help="Poisson arrival rate of the serving leg")
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-qps", type=float, default=100.0, help="Poisson arrival rate of the serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-burst", type=int, default=24, help="requests of the served-vs-plain burst")
     ap.add_argument("--serve-hbm-gb", type=int, default=64, help="HBM budget of the serving arena")
     ap.add_argument("--serve-prob-cap", type=int, default=1 << 16,
@@ -290,7 +296,9 @@ def dfa_leg(args, data, offsets, batch, stream, result: dict, copy_gbs) -> int:
                    "length": LENGTH, "alphabet": "0x40-0x5F", "kernel_variants": variants,
                    "prefetch": args.prefetch, "layout": args.layout, "launches_per_step": N_PATTERNS},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": PMC_TRAFFIC.get((kernel, args.layout)),
+                     "traffic_source": PMC_TRAFFIC_SOURCE if (kernel, args.layout) in PMC_TRAFFIC else None,
                      "kernel": kernel, "kernel_avg_ms": round(avg_kernel_ms, 5),
                      "alg_bytes_per_launch": alg_bytes, "copy_probe_gbs": copy_gbs},
         "cpu_baseline": cpu,

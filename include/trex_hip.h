@@ -259,7 +259,8 @@ int trex_mix_destroy(trex_mix* m);
  * NFA chunks record their state set every ck bytes (ck multiple of 16).
  * Host work only: the plan is staged in pinned memory and copied to the
  * device, asynchronously and by a copy kernel, by the next trex_mix_run or
- * trex_mix_capture on that call's stream. A handle holds one plan on the device: runs of one handle on
+ * trex_mix_capture on that call's stream (device buffers grow there too, in
+ * stream order). A handle holds one plan on the device: runs of one handle on
  * different streams must be ordered by the caller. */
 int trex_mix_plan(trex_mix* m, const int64_t* begin, const int64_t* end, const int32_t* pattern_of, int64_t n_strings,
                   int64_t dfa_split, int64_t dfa_chunk, int64_t nfa_split, int64_t nfa_chunk, int32_t ck);
