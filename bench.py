@@ -8,9 +8,11 @@ i.i.d. uniform symbols, resident in HBM before timing; every pattern walks
 every string (8 kernel launches per step); outputs are the final state and
 accept bit per string. Gchars/s = 8 x 2^30 / step time.
 
-The batch is held in the engine's staggered HBM layout (groups of 64 strings,
-each group 256 bytes further than the last; --layout csr keeps the packed
-CSR bytes). The dominant kernel's roofline figure uses its algorithmic bytes
+The batch is held in the engine's interleaved HBM layout (each group of 64
+strings stored as rows of 64 x 16 bytes, so a wave's load reads one 1 KiB
+row; --layout staggered keeps each string contiguous with groups 256 bytes
+apart, --layout csr the packed CSR bytes). Building that layout is part of
+the upload, outside the timed region, like the upload itself. The dominant kernel's roofline figure uses its algorithmic bytes
 per launch (input bytes + per-string begin/end + final-state and accept
 bytes) over its average launch duration, measured with HIP events on the
 launch stream. The results of the last step are checked bit-exact against the
@@ -25,9 +27,14 @@ Secondary legs (reported under "secondary"):
     Dirichlet(0.1) columns, uniform start, 16 accept states; 16,384 sequences
     x 2,048 uniform symbols; every mode's relative log-likelihood error
     against the FP64 C oracle (whose time is that leg's CPU baseline) and the
-    MFMA roofline (2 * 64^2 FLOP per symbol).
+    MFMA roofline (2 * 64^2 FLOP per symbol);
+  * uneven_mix: 64 patterns (32 DFA-16, 32 NFA-64) over 65,536 strings of
+    log-uniform lengths in [32 B, 16 MiB] in one persistent autotuned launch,
+    against one launch per pattern on a subset;
+  * serving: Poisson arrivals over 128 resident patterns, p50/p99 latency,
+    and a saturated burst served vs one synchronous launch per request.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs dfa,nfa,prob]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs dfa,nfa,prob,mix,serve]
 """
 
 from __future__ import annotations
@@ -47,10 +54,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the headline kernel on this workload, from two rocprofv3
 # counter runs (FETCH_SIZE x 2.000 + WRITE_SIZE, both calibrated against a plain
-# 1 GiB copy); the command lines and counts are in the profile named below.
-PMC_TRAFFIC = {("dfa16_k2f_pf8", "staggered"): 1_092_300_000}
-PMC_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, one pass each: "
-                      "profiles/dfa16_k2f_pf8_1GiB_8patterns.md")
+# 1 GiB copy); the command lines and counts are in the profile named with each.
+PMC_TRAFFIC = {
+    ("dfa16_k2f_il_pf8", "interleaved"): (1_078_770_000, "profiles/dfa16_k2f_il_pf8_1GiB_8patterns.md"),
+    ("dfa16_k2f_pf8", "staggered"): (1_092_300_000, "profiles/dfa16_k2f_pf8_1GiB_8patterns.md"),
+}
 MFMA_F16_PEAK_TFLOPS = 2500.0
 N_STRINGS = 262_144
 LENGTH = 4096
@@ -58,9 +66,17 @@ N_PATTERNS = 8
 SEED = 2
 PROB_SEQ = 16_384
 PROB_LEN = 2048
-KERNEL_NAMES = {("k2_window", 16): "dfa16_k2w_pf16", ("k2_window", 8): "dfa16_k2w_pf8",
-                ("k2_window_f", 16): "dfa16_k2f_pf16", ("k2_window_f", 8): "dfa16_k2f_pf8",
-                ("k1_16", 16): "dfa16_k1", ("k1_16", 8): "dfa16_k1"}
+KERNEL_FORMS = {"k2_window": "dfa16_k2w", "k2_window_f": "dfa16_k2f", "k2_window_fr": "dfa16_k2fr"}
+
+
+def kernel_name(variant: str, prefetch: int, layout: str) -> str:
+    """Name of the code-object kernel a DFA dispatch launches."""
+    il = "_il" if layout == "interleaved" else ""
+    if variant in KERNEL_FORMS:
+        return f"{KERNEL_FORMS[variant]}{il}_pf{16 if prefetch == 16 else 8}"
+    return ("dfa16_k1" if variant == "k1_16" else "dfa_k1") + il
+
+
 PROB_WAVES = (4, 8)
 
 
@@ -100,7 +116,8 @@ def main() -> int:
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--variant", default="auto")
     ap.add_argument("--prefetch", type=int, default=8, help="16-byte vectors in flight per lane (8 or 16)")
-    ap.add_argument("--layout", default="staggered", choices=["staggered", "csr"])
+    ap.add_argument("--layout", default="interleaved", choices=["interleaved", "staggered", "csr"],
+                    help="HBM layout of the DFA batch (the NFA leg reads a staggered copy when this is interleaved)")
     ap.add_argument("--legs", default="dfa,nfa,prob,mix,serve", help="comma list of dfa, nfa, prob, mix, serve")
     ap.add_argument("--prob-modes", default="f16x2,bf16x2,f16,bf16")
     ap.add_argument("--mix-strings", type=int, default=MIX_STRINGS, help="strings of the uneven-mix leg")
@@ -166,6 +183,9 @@ def main() -> int:
             return rc
         ok &= rc == 0
     if "nfa" in legs:
+        if batch.layout == "interleaved":   # the NFA kernels read contiguous strings
+            batch.free()
+            batch = DeviceBatch(data, offsets, layout="staggered")
         secondary["nfa64"] = nfa_leg(args, data, offsets, batch, stream)
         ok &= secondary["nfa64"].get("verified_bit_exact") in (None, True)
     if batch is not None:
@@ -278,7 +298,7 @@ def dfa_leg(args, data, offsets, batch, stream, result: dict, copy_gbs) -> int:
         a.free()
         b.free()
 
-    kernel = KERNEL_NAMES.get((variants[0], args.prefetch), variants[0])
+    kernel = kernel_name(variants[0], args.prefetch, args.layout)
     result.update({
         "metric": "Gchars/s (DFA-16, NFA-64) + %HBM roofline; p50/p99 latency served vs plain",
         "value": round(gchars, 2),
@@ -297,8 +317,10 @@ def dfa_leg(args, data, offsets, batch, stream, result: dict, copy_gbs) -> int:
                    "prefetch": args.prefetch, "layout": args.layout, "launches_per_step": N_PATTERNS},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": PMC_TRAFFIC.get((kernel, args.layout)),
-                     "traffic_source": PMC_TRAFFIC_SOURCE if (kernel, args.layout) in PMC_TRAFFIC else None,
+                     "traffic": PMC_TRAFFIC.get((kernel, args.layout), (None, None))[0],
+                     "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, one pass each: "
+                                        + PMC_TRAFFIC[(kernel, args.layout)][1])
+                     if (kernel, args.layout) in PMC_TRAFFIC else None,
                      "kernel": kernel, "kernel_avg_ms": round(avg_kernel_ms, 5),
                      "alg_bytes_per_launch": alg_bytes, "copy_probe_gbs": copy_gbs},
         "cpu_baseline": cpu,
@@ -769,8 +791,10 @@ def sweep(args, data, offsets, batch, engines, outs, stream) -> int:
     from trex_amd.device import DeviceBuffer
     from trex_amd.engine import DeviceBatch
     n = len(offsets) - 1
-    other = DeviceBatch(data, offsets, layout="csr" if batch.layout == "staggered" else "staggered")
-    layouts = {batch.layout: batch, other.layout: other}
+    layouts = {batch.layout: batch}
+    for name in ("csr", "staggered", "interleaved"):
+        if name not in layouts:
+            layouts[name] = DeviceBatch(data, offsets, layout=name)
     ref = []
     for i, eng in enumerate(engines):
         eng.match_batch_device(batch, outs[i][0].ptr, outs[i][1].ptr, stream)
@@ -780,13 +804,14 @@ def sweep(args, data, offsets, batch, engines, outs, stream) -> int:
                                          sink.ptr, stream.handle), stream, 5)
     log(f"read probe coalesced stream      : {ms:.4f} ms = {data.nbytes / (ms * 1e-3) / 1e9:.0f} GB/s")
     for name, b in layouts.items():
-        for mode in (1, 2):
+        for mode in ((3,) if name == "interleaved" else (1, 2)):
             ms = time_launches(lambda: _lib.call("trex_read_probe", b.d_bytes.ptr, 16, b.d_begin.ptr, b.d_end.ptr,
                                                  n, mode, sink.ptr, stream.handle), stream, 5)
-            log(f"read probe {name:9s} pf{8 * mode:2d}          : {ms:.4f} ms = "
+            log(f"read probe {name:11s} pf{8 if mode == 3 else 8 * mode:2d}        : {ms:.4f} ms = "
                 f"{data.nbytes / (ms * 1e-3) / 1e9:.0f} GB/s")
     for name, b in layouts.items():
-        for v, pf in [("k1_16", 16), ("k2_window", 8), ("k2_window", 16), ("k2_window_f", 8), ("k2_window_f", 16)]:
+        for v, pf in [("k1_16", 16), ("k2_window", 8), ("k2_window", 16), ("k2_window_f", 8), ("k2_window_f", 16),
+                      ("k2_window_fr", 8), ("k2_window_fr", 16)]:
             times, same = [], True
             for i, eng in enumerate(engines):
                 eng.set_variant(v)
@@ -795,9 +820,11 @@ def sweep(args, data, offsets, batch, engines, outs, stream) -> int:
                                            stream, 3))
                 same &= bool(np.array_equal(outs[i][0].download(np.uint8, n, stream), ref[i]))
             ms = float(np.mean(times))
-            log(f"sweep {name:9s} {v:12s} pf{pf:2d}: kernel avg {ms:.4f} ms ({min(times):.4f}-{max(times):.4f}) "
+            log(f"sweep {name:11s} {v:12s} pf{pf:2d}: kernel avg {ms:.4f} ms ({min(times):.4f}-{max(times):.4f}) "
                 f"= {(data.nbytes + 18 * n) / (ms * 1e-3) / 1e9:.0f} GB/s, same={same}")
-    other.free()
+    for name, b in layouts.items():
+        if b is not batch:
+            b.free()
     return 0
 
 

@@ -111,6 +111,7 @@ int trex_dfa_destroy(trex_dfa* h);
 #define TREX_DFA_K1_16 2      /* <= 16 states, one lookup per byte */
 #define TREX_DFA_K2_WINDOW 3   /* <= 16 states, window bytes, one lookup per byte pair */
 #define TREX_DFA_K2_WINDOW_F 5 /* as 3, pair table of 64-bit nibble functions */
+#define TREX_DFA_K2_WINDOW_FR 6 /* as 5, four interleaved copies of the table in LDS (fewer bank conflicts) */
 
 /* Reports the kernel variant chosen at create time, the window base byte
  * (-1 if none) and the dead state (-1 if none). */
@@ -131,6 +132,16 @@ int trex_dfa_match(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_offsets
  * gaps, as in the staggered HBM layout). */
 int trex_dfa_match_ranges(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_begin, const int64_t* d_end,
                           int64_t n_strings, uint8_t* d_final_state, uint8_t* d_accept, void* stream);
+
+/* Same over the interleaved layout (no reference counterpart; it replaces
+ * the packed CSR upload of cuda/monoid_batch.cu:506-507 for uniform batches):
+ * the strings of a group of 64 consecutive strings are stored as rows of
+ * 64 x 16 bytes, vector k of the group's lane j at group base + (64 k + j) * 16.
+ * d_begin[i] = address of string i's vector 0 (16-byte aligned),
+ * d_end[i] - d_begin[i] = its length; its last len % 16 bytes open vector
+ * len / 16. One wave load then reads one contiguous 1 KiB row. */
+int trex_dfa_match_interleaved(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_begin, const int64_t* d_end,
+                               int64_t n_strings, uint8_t* d_final_state, uint8_t* d_accept, void* stream);
 
 /* Host-pointer dispatch (synchronous): stages the batch through engine-owned
  * device buffers. kernel_ms / total_ms (nullable) receive the kernel time and
@@ -326,7 +337,8 @@ int trex_copy(void* d_dst, const void* d_src, size_t n_bytes, void* stream);
  * grid-stride read of d_src[0..n_bytes), d_sink holds 8192 * 256 words;
  * mode 1 / 2: the per-string access pattern of the window kernels (8 / 16
  * vectors in flight) over 16-byte-aligned ranges [d_begin[i], d_end[i]),
- * d_sink holds n_strings words. */
+ * d_sink holds n_strings words; mode 3: the same over interleaved-layout
+ * ranges (trex_dfa_match_interleaved). */
 int trex_read_probe(const void* d_src, size_t n_bytes, const int64_t* d_begin, const int64_t* d_end,
                     int64_t n_strings, int mode, uint32_t* d_sink, void* stream);
 

@@ -112,7 +112,7 @@ def test_random_patterns_ragged_all_variants(gpu_lib, seed):
     data[hits] = rng.integers(0, 256, size=50, dtype=np.uint8)
     variants = ["k1_generic", "k1_16"]
     if eng.info()["window_base"] >= 0:
-        variants += ["k2_window", "k2_window_f"]
+        variants += ["k2_window", "k2_window_f", "k2_window_fr"]
     results = []
     for v in variants:
         eng.set_variant(v)
@@ -134,15 +134,37 @@ def test_window_variant_out_of_window_bytes(gpu_lib):
     for i in range(512):
         pos = int(offs[i] + (i * 7) % 300)
         data[pos] = [0x00, 0x3F, 0x60, 0x7F, 0xC0, 0xFF, 0x20, 0x41][i % 8]
-    for v in ["k2_window", "k2_window_f"]:
+    for v in ["k2_window", "k2_window_f", "k2_window_fr"]:
         eng.set_variant(v)
         f, a = _check(eng, tab, data, offs)
         assert np.all(f[np.arange(512) % 8 != 7] == tab.dead)
 
 
-@pytest.mark.parametrize("layout", ["csr", "staggered"])
+def test_interleaved_uniform_config2_shape(gpu_lib):
+    """The interleaved layout on whole groups of equal-length strings (the
+    one-copy packing path), k2f and k2fr, against the oracle."""
+    from trex_amd.device import DeviceBuffer
+    from trex_amd.engine import DeviceBatch
+    rng = np.random.default_rng(37)
+    _, dfa = random_dfa_pattern(rng, ALPHABET_32)
+    tab = dfa_byte_table(dfa)
+    eng = DfaEngine(tab)
+    data, offs = uniform_batch(rng, 64 * 40, 4096, 0x40, 0x60)
+    batch = DeviceBatch(data, offs, layout="interleaved")
+    n = len(offs) - 1
+    d_f, d_a = DeviceBuffer(n), DeviceBuffer(n)
+    rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offs)
+    for v in ["k2_window_f", "k2_window_fr"]:
+        eng.set_variant(v)
+        eng.match_batch_device(batch, d_f.ptr, d_a.ptr)
+        assert np.array_equal(d_f.download(np.uint8, n), rf) and np.array_equal(d_a.download(np.uint8, n), ra), v
+
+
+@pytest.mark.parametrize("layout", ["csr", "staggered", "interleaved"])
 def test_device_batch_layouts_bit_exact(gpu_lib, layout):
-    """Ragged batch in both HBM layouts, every variant, against the oracle."""
+    """Ragged batch (empty strings, lengths off the 16-byte grid, a last group
+    of fewer than 64 strings) in every HBM layout, every variant, against the
+    oracle."""
     from trex_amd.device import DeviceBuffer
     from trex_amd.engine import DeviceBatch
     rng = np.random.default_rng(31)
@@ -151,13 +173,14 @@ def test_device_batch_layouts_bit_exact(gpu_lib, layout):
     eng = DfaEngine(tab)
     lengths = rng.integers(0, 3000, size=1000)
     lengths[::97] = 4096
+    lengths[1:9] = [0, 1, 15, 16, 17, 31, 33, 0]
     data, offs = ragged_batch(rng, lengths, 0x40, 0x60)
     data[rng.integers(0, len(data), 20)] = 0x21
     batch = DeviceBatch(data, offs, layout=layout)
     n = len(offs) - 1
     d_f, d_a = DeviceBuffer(n), DeviceBuffer(n)
     rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offs)
-    for v in ["k1_16", "k2_window", "k2_window_f"]:
+    for v in ["k1_generic", "k1_16", "k2_window", "k2_window_f", "k2_window_fr"]:
         eng.set_variant(v)
         for pf in (8, 16):
             eng.set_prefetch(pf)

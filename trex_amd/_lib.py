@@ -61,6 +61,8 @@ _SIGNATURES = {
     "trex_dfa_set_prefetch": (c_int, [c_void_p, c_int]),
     "trex_dfa_match_ranges": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
                                       c_void_p]),
+    "trex_dfa_match_interleaved": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
+                                      c_void_p]),
     # NFA
     "trex_nfa_create": (c_int, [c_void_p, c_int, c_void_p, c_uint64, c_uint64, vpp]),
     "trex_nfa_destroy": (c_int, [c_void_p]),

@@ -16,20 +16,22 @@ __device__ __forceinline__ int64_t head_len(const uint8_t* bytes, int64_t p) {
     return (int64_t)((16u - ((uint32_t)(uintptr_t)(bytes + p) & 15u)) & 15u);
 }
 
-// Streams v[0 .. nvec) through `consume` in order with two blocks of PF
-// 16-byte vectors: the loads of block i + 1 are issued before block i is
-// consumed, so PF loads are in flight while the table walk runs.
-template <int PF, class F>
+// Streams the 16-byte vectors v[0], v[VSTRIDE], ..., v[(nvec - 1) * VSTRIDE]
+// through `consume` in order with two blocks of PF vectors: the loads of block
+// i + 1 are issued before block i is consumed, so PF loads are in flight while
+// the table walk runs. VSTRIDE = 1: a string stored contiguously; VSTRIDE = 64:
+// the interleaved layout, where one wave's 64 strings share each 1 KiB row.
+template <int PF, int VSTRIDE = 1, class F>
 __device__ __forceinline__ void stream_body(const uint4* __restrict__ v, int64_t nvec, F&& consume) {
     int64_t i = 0;
     if (nvec >= PF) {
         uint4 a[PF];
 #pragma unroll
-        for (int k = 0; k < PF; ++k) a[k] = v[k];
+        for (int k = 0; k < PF; ++k) a[k] = v[(int64_t)k * VSTRIDE];
         for (i = PF; i + PF <= nvec; i += PF) {
             uint4 b[PF];
 #pragma unroll
-            for (int k = 0; k < PF; ++k) b[k] = v[i + k];
+            for (int k = 0; k < PF; ++k) b[k] = v[(i + k) * VSTRIDE];
 #pragma unroll
             for (int k = 0; k < PF; ++k) consume(a[k]);
 #pragma unroll
@@ -38,7 +40,7 @@ __device__ __forceinline__ void stream_body(const uint4* __restrict__ v, int64_t
 #pragma unroll
         for (int k = 0; k < PF; ++k) consume(a[k]);
     }
-    for (; i < nvec; ++i) consume(v[i]);
+    for (; i < nvec; ++i) consume(v[i * VSTRIDE]);
 }
 
 // Copy `n` bytes (multiple of 16) from global to LDS with the whole block.

@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <mutex>
 #include <vector>
 
@@ -48,7 +49,8 @@ bool variant_allowed(const trex_dfa* h, int v) {
         case TREX_DFA_K1_GENERIC: return true;
         case TREX_DFA_K1_16: return h->n_states <= 16;
         case TREX_DFA_K2_WINDOW:
-        case TREX_DFA_K2_WINDOW_F: return h->n_states <= 16 && h->window_base >= 0 && h->d_t2 != nullptr;
+        case TREX_DFA_K2_WINDOW_F:
+        case TREX_DFA_K2_WINDOW_FR: return h->n_states <= 16 && h->window_base >= 0 && h->d_t2 != nullptr;
         default: return false;
     }
 }
@@ -146,8 +148,12 @@ int trex_dfa_set_prefetch(trex_dfa* h, int vectors) {
     return TREX_OK;
 }
 
-int trex_dfa_match_ranges(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_begin, const int64_t* d_end,
-                          int64_t n_strings, uint8_t* d_final_state, uint8_t* d_accept, void* stream) {
+namespace {
+
+// One launch of the handle's variant over begin/end ranges; `il` selects the
+// kernels of the interleaved layout.
+int dfa_dispatch(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_begin, const int64_t* d_end,
+                 int64_t n_strings, uint8_t* d_final_state, uint8_t* d_accept, void* stream, bool il) {
     if (!h) return fail(TREX_ERR_INVALID, "null handle");
     if (n_strings < 0) return fail(TREX_ERR_INVALID, "n_strings < 0");
     if (n_strings == 0) return TREX_OK;
@@ -157,16 +163,15 @@ int trex_dfa_match_ranges(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_
     const unsigned grid = (unsigned)grid64;
     hipStream_t s = (hipStream_t)stream;
     uint32_t start = (uint32_t)h->start;
-    if (h->variant == TREX_DFA_K2_WINDOW || h->variant == TREX_DFA_K2_WINDOW_F) {
+    const char* il_tag = il ? "_il" : "";
+    char name[48];
+    if (h->variant == TREX_DFA_K2_WINDOW || h->variant == TREX_DFA_K2_WINDOW_F ||
+        h->variant == TREX_DFA_K2_WINDOW_FR) {
         uint32_t win4 = (uint32_t)h->window_base * 0x01010101u;
         uint32_t dead = (uint32_t)h->dead;
-        const bool pf16 = h->prefetch == 16;
-        uint8_t* table = h->d_t2;
-        const char* name = pf16 ? "dfa16_k2w_pf16" : "dfa16_k2w_pf8";
-        if (h->variant == TREX_DFA_K2_WINDOW_F) {
-            table = h->d_f2;
-            name = pf16 ? "dfa16_k2f_pf16" : "dfa16_k2f_pf8";
-        }
+        uint8_t* table = h->variant == TREX_DFA_K2_WINDOW ? h->d_t2 : h->d_f2;
+        const char* form = h->variant == TREX_DFA_K2_WINDOW ? "k2w" : h->variant == TREX_DFA_K2_WINDOW_F ? "k2f" : "k2fr";
+        std::snprintf(name, sizeof(name), "dfa16_%s%s_pf%d", form, il_tag, h->prefetch == 16 ? 16 : 8);
         void* args[] = {&d_bytes, &d_begin, &d_end, &n_strings, &table, &h->d_t1, &h->d_acc,
                         &win4, &start, &dead, &d_final_state, &d_accept};
         return launch(name, grid, kBlock, 0, s, args);
@@ -175,8 +180,20 @@ int trex_dfa_match_ranges(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_
     void* args[] = {&d_bytes, &d_begin, &d_end, &n_strings, &h->d_t1, &h->d_acc, &stride, &start,
                     &d_final_state, &d_accept};
     const unsigned lds = 256u * stride + stride;
-    if (h->variant == TREX_DFA_K1_16) return launch("dfa16_k1", grid, kBlock, lds, s, args);
-    return launch("dfa_k1", grid, kBlock, lds, s, args);
+    std::snprintf(name, sizeof(name), "%s%s", h->variant == TREX_DFA_K1_16 ? "dfa16_k1" : "dfa_k1", il_tag);
+    return launch(name, grid, kBlock, lds, s, args);
+}
+
+}  // namespace
+
+int trex_dfa_match_ranges(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_begin, const int64_t* d_end,
+                          int64_t n_strings, uint8_t* d_final_state, uint8_t* d_accept, void* stream) {
+    return dfa_dispatch(h, d_bytes, d_begin, d_end, n_strings, d_final_state, d_accept, stream, false);
+}
+
+int trex_dfa_match_interleaved(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_begin, const int64_t* d_end,
+                               int64_t n_strings, uint8_t* d_final_state, uint8_t* d_accept, void* stream) {
+    return dfa_dispatch(h, d_bytes, d_begin, d_end, n_strings, d_final_state, d_accept, stream, true);
 }
 
 int trex_dfa_match(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_offsets, int64_t n_strings,
@@ -241,7 +258,8 @@ int trex_read_probe(const void* d_src, size_t n_bytes, const int64_t* d_begin, c
     if (!d_begin || !d_end || n_strings <= 0) return fail(TREX_ERR_INVALID, "string probe needs ranges");
     const unsigned grid = (unsigned)((n_strings + kBlock - 1) / kBlock);
     void* args[] = {&d_src, &d_begin, &d_end, &n_strings, &d_sink};
-    return launch(mode == 2 ? "read_probe_strings_pf16" : "read_probe_strings_pf8", grid, kBlock, 0, s, args);
+    const char* name = mode == 3 ? "read_probe_il_pf8" : mode == 2 ? "read_probe_strings_pf16" : "read_probe_strings_pf8";
+    return launch(name, grid, kBlock, 0, s, args);
 }
 
 int trex_copy(void* d_dst, const void* d_src, size_t n_bytes, void* stream) {

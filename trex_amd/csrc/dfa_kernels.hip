@@ -24,8 +24,16 @@
 //      k2f  function table, f2[pair] = 16 nibbles (8 KB, uint64): the LDS
 //           read does not depend on the state, so it leaves the dependent
 //           chain, which becomes s = (f >> 4s) & 15
+//      k2fr the k2f table in four interleaved copies, lane l reading copy
+//           l & 3: lanes of different copies never meet in an LDS bank
 //  * dfa_k1 / dfa16_k1 are the general walk, one lookup per byte:
 //    t1[byte * stride + s], for any table of up to 256 states.
+//  * every kernel has an _il form for the interleaved layout: the strings of
+//    a wave (64 consecutive strings) are stored as rows of 64 x 16 bytes,
+//    vector k of lane j at group base + (64 k + j) * 16, so each load of the
+//    wave reads one contiguous 1 KiB row. begin[i] is the address of string
+//    i's vector 0 (16-byte aligned), end[i] - begin[i] its length; its last
+//    len % 16 bytes open vector len / 16.
 //
 // Bytes before the first 16-byte boundary and after the last one go through
 // the exact per-byte table t1 in every kernel.
@@ -44,9 +52,15 @@ namespace {
 
 using namespace trexdev;
 
-// Pair-step policies over the LDS pair table `t`.
+constexpr int kIlGroup = 64;   // strings per group of the interleaved layout (one wave)
+
+// Pair-step policies over the LDS pair table: stage() fills it from the
+// global table, lane_table() is where this lane reads it, word() takes two
+// pair steps.
 struct StepK2W {
     static constexpr int kTableBytes = 1024 * 16;
+    __device__ __forceinline__ static void stage(uint8_t* dst, const uint8_t* src) { stage_table(dst, src, kTableBytes); }
+    __device__ __forceinline__ static const uint8_t* lane_table(const uint8_t* t) { return t; }
     __device__ __forceinline__ static uint32_t word(const uint8_t* __restrict__ t, uint32_t s, uint32_t x) {
         s = t[(pair_lo(x) << 4) | s];
         return t[(pair_hi(x) << 4) | s];
@@ -55,10 +69,35 @@ struct StepK2W {
 
 struct StepK2F {
     static constexpr int kTableBytes = 1024 * 8;
+    __device__ __forceinline__ static void stage(uint8_t* dst, const uint8_t* src) { stage_table(dst, src, kTableBytes); }
+    __device__ __forceinline__ static const uint8_t* lane_table(const uint8_t* t) { return t; }
     __device__ __forceinline__ static uint32_t word(const uint8_t* __restrict__ t, uint32_t s, uint32_t x) {
         const uint64_t* f = reinterpret_cast<const uint64_t*>(t);
         const uint64_t f0 = f[pair_lo(x)];
         const uint64_t f1 = f[pair_hi(x)];
+        s = (uint32_t)(f0 >> (s << 2)) & 15u;
+        return (uint32_t)(f1 >> (s << 2)) & 15u;
+    }
+};
+
+// k2f with the function table in kCopies interleaved copies: entry p of copy
+// c sits at (p * kCopies + c) * 8 bytes and lane l reads copy l % kCopies, so
+// the lanes of one LDS access that read different copies use disjoint banks.
+struct StepK2FR {
+    static constexpr int kCopies = 4;
+    static constexpr int kTableBytes = 1024 * 8 * kCopies;
+    __device__ __forceinline__ static void stage(uint8_t* dst, const uint8_t* src) {
+        const uint64_t* s = reinterpret_cast<const uint64_t*>(src);
+        uint64_t* d = reinterpret_cast<uint64_t*>(dst);
+        for (int i = threadIdx.x; i < 1024 * kCopies; i += blockDim.x) d[i] = s[i / kCopies];
+    }
+    __device__ __forceinline__ static const uint8_t* lane_table(const uint8_t* t) {
+        return t + (threadIdx.x % kCopies) * 8;
+    }
+    __device__ __forceinline__ static uint32_t word(const uint8_t* __restrict__ t, uint32_t s, uint32_t x) {
+        const uint64_t* f = reinterpret_cast<const uint64_t*>(t);
+        const uint64_t f0 = f[pair_lo(x) * kCopies];
+        const uint64_t f1 = f[pair_hi(x) * kCopies];
         s = (uint32_t)(f0 >> (s << 2)) & 15u;
         return (uint32_t)(f1 >> (s << 2)) & 15u;
     }
@@ -98,7 +137,7 @@ __device__ __forceinline__ uint32_t k1_vec(const uint8_t* __restrict__ t1, uint3
     return s;
 }
 
-template <class Step, int PF>
+template <class Step, int PF, bool IL>
 __device__ __forceinline__ void window_kernel(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
                                               const int64_t* __restrict__ endp, int64_t n_strings,
                                               const uint8_t* __restrict__ tpg, const uint8_t* __restrict__ t1g,
@@ -109,7 +148,7 @@ __device__ __forceinline__ void window_kernel(const uint8_t* __restrict__ bytes,
     __shared__ __attribute__((aligned(16))) uint8_t tp[Step::kTableBytes];
     __shared__ __attribute__((aligned(16))) uint8_t t1[256 * 16];
     __shared__ __attribute__((aligned(16))) uint8_t acc[16];
-    stage_table(tp, tpg, Step::kTableBytes);
+    Step::stage(tp, tpg);
     stage_table(t1, t1g, 256 * 16);
     if (threadIdx.x < 16) acc[threadIdx.x] = accg[threadIdx.x];
     __syncthreads();
@@ -119,23 +158,32 @@ __device__ __forceinline__ void window_kernel(const uint8_t* __restrict__ bytes,
     int64_t p = begin[sid];
     const int64_t end = endp[sid];
     uint32_t s = start;
-
-    const int64_t head_end = min(end, p + head_len(bytes, p));
-    for (; p < head_end; ++p) s = t1[((uint32_t)bytes[p] << 4) | s];
-
+    const uint8_t* tl = Step::lane_table(tp);
     WinCheck w; w.init();
-    const int64_t nvec = (end - p) >> 4;
-    stream_body<PF>(reinterpret_cast<const uint4*>(bytes + p), nvec,
-                    [&](uint4 x) { s = window_vec<Step>(tp, s, x, w); });
-    p += nvec << 4;
-    for (; p < end; ++p) s = t1[((uint32_t)bytes[p] << 4) | s];
+
+    if constexpr (IL) {
+        const int64_t len = end - p;
+        const int64_t nvec = len >> 4;
+        stream_body<PF, kIlGroup>(reinterpret_cast<const uint4*>(bytes + p), nvec,
+                                  [&](uint4 x) { s = window_vec<Step>(tl, s, x, w); });
+        const uint8_t* tail = bytes + p + nvec * (16 * kIlGroup);
+        for (int t = 0; t < (int)(len & 15); ++t) s = t1[((uint32_t)tail[t] << 4) | s];
+    } else {
+        const int64_t head_end = min(end, p + head_len(bytes, p));
+        for (; p < head_end; ++p) s = t1[((uint32_t)bytes[p] << 4) | s];
+        const int64_t nvec = (end - p) >> 4;
+        stream_body<PF>(reinterpret_cast<const uint4*>(bytes + p), nvec,
+                        [&](uint4 x) { s = window_vec<Step>(tl, s, x, w); });
+        p += nvec << 4;
+        for (; p < end; ++p) s = t1[((uint32_t)bytes[p] << 4) | s];
+    }
     if (w.outside(win4)) s = dead;
 
     out_state[sid] = (uint8_t)s;
     out_accept[sid] = acc[s];
 }
 
-template <int STRIDE_LOG2>
+template <int STRIDE_LOG2, bool IL>
 __device__ __forceinline__ void k1_kernel(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
                                           const int64_t* __restrict__ endp, int64_t n_strings,
                                           const uint8_t* __restrict__ t1g, const uint8_t* __restrict__ accg,
@@ -155,13 +203,22 @@ __device__ __forceinline__ void k1_kernel(const uint8_t* __restrict__ bytes, con
     const int64_t end = endp[sid];
     uint32_t s = start;
 
-    const int64_t head_end = min(end, p + head_len(bytes, p));
-    for (; p < head_end; ++p) s = k1_byte<STRIDE_LOG2>(t1, s, bytes[p], stride);
-    const int64_t nvec = (end - p) >> 4;
-    stream_body<8>(reinterpret_cast<const uint4*>(bytes + p), nvec,
-                   [&](uint4 x) { s = k1_vec<STRIDE_LOG2>(t1, s, x, stride); });
-    p += nvec << 4;
-    for (; p < end; ++p) s = k1_byte<STRIDE_LOG2>(t1, s, bytes[p], stride);
+    if constexpr (IL) {
+        const int64_t len = end - p;
+        const int64_t nvec = len >> 4;
+        stream_body<8, kIlGroup>(reinterpret_cast<const uint4*>(bytes + p), nvec,
+                                 [&](uint4 x) { s = k1_vec<STRIDE_LOG2>(t1, s, x, stride); });
+        const uint8_t* tail = bytes + p + nvec * (16 * kIlGroup);
+        for (int t = 0; t < (int)(len & 15); ++t) s = k1_byte<STRIDE_LOG2>(t1, s, tail[t], stride);
+    } else {
+        const int64_t head_end = min(end, p + head_len(bytes, p));
+        for (; p < head_end; ++p) s = k1_byte<STRIDE_LOG2>(t1, s, bytes[p], stride);
+        const int64_t nvec = (end - p) >> 4;
+        stream_body<8>(reinterpret_cast<const uint4*>(bytes + p), nvec,
+                       [&](uint4 x) { s = k1_vec<STRIDE_LOG2>(t1, s, x, stride); });
+        p += nvec << 4;
+        for (; p < end; ++p) s = k1_byte<STRIDE_LOG2>(t1, s, bytes[p], stride);
+    }
 
     out_state[sid] = (uint8_t)s;
     out_accept[sid] = acc[s];
@@ -171,47 +228,48 @@ __device__ __forceinline__ void k1_kernel(const uint8_t* __restrict__ bytes, con
 
 // ---------------------------------------------------------------------------
 // Window kernels: 16-state table, two bytes per step.
-//   tpg: pair table (k2w: 16 KB uint8, k2f: 8 KB uint64)
+//   tpg: pair table (k2w: 16 KB uint8, k2f / k2fr: 8 KB uint64)
 //   t1g [256 * 16] byte table, accg [16]
 //   win4 = window base byte replicated in all four bytes (base % 32 == 0)
 // ---------------------------------------------------------------------------
-#define TREX_WINDOW_KERNEL(NAME, STEP, PF, ATTR)                                                              \
-    extern "C" __global__ void __launch_bounds__(TREX_BLOCK) ATTR                                            \
+#define TREX_WINDOW_KERNEL(NAME, STEP, PF, IL)                                                                \
+    extern "C" __global__ void __launch_bounds__(TREX_BLOCK)                                                  \
     NAME(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin, const int64_t* __restrict__ end, \
          int64_t n_strings, const uint8_t* __restrict__ tpg, const uint8_t* __restrict__ t1g,                \
          const uint8_t* __restrict__ accg, uint32_t win4, uint32_t start, uint32_t dead,                     \
          uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept) {                                \
-        window_kernel<STEP, PF>(bytes, begin, end, n_strings, tpg, t1g, accg, win4, start, dead, out_state,  \
-                                out_accept);                                                                  \
+        window_kernel<STEP, PF, IL>(bytes, begin, end, n_strings, tpg, t1g, accg, win4, start, dead,         \
+                                    out_state, out_accept);                                                   \
     }
 
-#define TREX_W4 __attribute__((amdgpu_waves_per_eu(4)))
-#define TREX_NOATTR
-TREX_WINDOW_KERNEL(dfa16_k2w_pf8, StepK2W, 8, TREX_NOATTR)
-TREX_WINDOW_KERNEL(dfa16_k2w_pf16, StepK2W, 16, TREX_NOATTR)
-TREX_WINDOW_KERNEL(dfa16_k2f_pf8, StepK2F, 8, TREX_NOATTR)
-TREX_WINDOW_KERNEL(dfa16_k2f_pf16, StepK2F, 16, TREX_NOATTR)
+#define TREX_WINDOW_KERNELS(BASE, STEP)                        \
+    TREX_WINDOW_KERNEL(BASE##_pf8, STEP, 8, false)             \
+    TREX_WINDOW_KERNEL(BASE##_pf16, STEP, 16, false)           \
+    TREX_WINDOW_KERNEL(BASE##_il_pf8, STEP, 8, true)           \
+    TREX_WINDOW_KERNEL(BASE##_il_pf16, STEP, 16, true)
+
+TREX_WINDOW_KERNELS(dfa16_k2w, StepK2W)
+TREX_WINDOW_KERNELS(dfa16_k2f, StepK2F)
+TREX_WINDOW_KERNELS(dfa16_k2fr, StepK2FR)
 
 // ---------------------------------------------------------------------------
 // General byte walk.
 //   t1g [256 * stride] with t1[byte * stride + s]; accg [stride]
 //   Dynamic LDS: 256 * stride + stride bytes (stride a multiple of 16).
 // ---------------------------------------------------------------------------
-extern "C" TREX_STRING_KERNEL
-dfa16_k1(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin, const int64_t* __restrict__ end,
-         int64_t n_strings, const uint8_t* __restrict__ t1g, const uint8_t* __restrict__ accg, uint32_t stride,
-         uint32_t start, uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
-{
-    k1_kernel<4>(bytes, begin, end, n_strings, t1g, accg, 16u, start, out_state, out_accept);
-}
+#define TREX_K1_KERNEL(NAME, STRIDE_LOG2, IL)                                                                 \
+    extern "C" TREX_STRING_KERNEL                                                                             \
+    NAME(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin, const int64_t* __restrict__ end, \
+         int64_t n_strings, const uint8_t* __restrict__ t1g, const uint8_t* __restrict__ accg, uint32_t stride, \
+         uint32_t start, uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept) {                 \
+        k1_kernel<STRIDE_LOG2, IL>(bytes, begin, end, n_strings, t1g, accg, STRIDE_LOG2 >= 0 ? 16u : stride,  \
+                                   start, out_state, out_accept);                                             \
+    }
 
-extern "C" TREX_STRING_KERNEL
-dfa_k1(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin, const int64_t* __restrict__ end,
-       int64_t n_strings, const uint8_t* __restrict__ t1g, const uint8_t* __restrict__ accg, uint32_t stride,
-       uint32_t start, uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
-{
-    k1_kernel<-1>(bytes, begin, end, n_strings, t1g, accg, stride, start, out_state, out_accept);
-}
+TREX_K1_KERNEL(dfa16_k1, 4, false)
+TREX_K1_KERNEL(dfa16_k1_il, 4, true)
+TREX_K1_KERNEL(dfa_k1, -1, false)
+TREX_K1_KERNEL(dfa_k1_il, -1, true)
 
 // ---------------------------------------------------------------------------
 // Read probes: the HBM read rate of (a) a fully coalesced grid-stride stream
@@ -259,6 +317,21 @@ read_probe_strings_pf16(const uint8_t* __restrict__ bytes, const int64_t* __rest
                         const int64_t* __restrict__ end, int64_t n_strings, uint32_t* __restrict__ sink)
 {
     read_probe_strings_body<16>(bytes, begin, end, n_strings, sink);
+}
+
+// The interleaved layout's access pattern (one 1 KiB row per wave load).
+extern "C" TREX_STRING_KERNEL
+read_probe_il_pf8(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
+                  const int64_t* __restrict__ endp, int64_t n_strings, uint32_t* __restrict__ sink)
+{
+    const int64_t sid = (int64_t)blockIdx.x * TREX_BLOCK + threadIdx.x;
+    if (sid >= n_strings) return;
+    const int64_t p = begin[sid];
+    const int64_t nvec = (endp[sid] - p) >> 4;
+    uint32_t x = 0;
+    stream_body<8, kIlGroup>(reinterpret_cast<const uint4*>(bytes + p), nvec,
+                             [&](uint4 v) { x ^= v.x ^ v.y ^ v.z ^ v.w; });
+    sink[sid] = x;
 }
 
 // ---------------------------------------------------------------------------

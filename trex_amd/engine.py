@@ -21,7 +21,7 @@ from trex_amd import _lib
 from trex_amd.device import DeviceBuffer, Stream
 from trex_amd.tables import DfaByteTable
 
-VARIANTS = {"auto": 0, "k1_generic": 1, "k1_16": 2, "k2_window": 3, "k2_window_f": 5}
+VARIANTS = {"auto": 0, "k1_generic": 1, "k1_16": 2, "k2_window": 3, "k2_window_f": 5, "k2_window_fr": 6}
 
 # Staggered HBM layout: strings are kept in groups of one wave (64 lanes); group g
 # is moved STAGGER_GAP * g bytes further than in the packed CSR layout, so that
@@ -29,6 +29,11 @@ VARIANTS = {"auto": 0, "k1_generic": 1, "k1_16": 2, "k2_window": 3, "k2_window_f
 # HBM channel interleave instead of all hitting the same channels.
 STAGGER_GROUP = 64
 STAGGER_GAP = 256
+# Interleaved HBM layout: the 64 strings of a wave are stored as rows of
+# 64 x 16 bytes (vector k of lane j at group base + (64 k + j) * 16), so every
+# wave load reads one contiguous 1 KiB row; groups are IL_GAP bytes apart.
+IL_GROUP = 64
+IL_GAP = 256
 VARIANT_NAMES = {v: k for k, v in VARIANTS.items()}
 
 
@@ -88,7 +93,8 @@ class DfaEngine:
     def match_batch_device(self, batch: "DeviceBatch", d_final: int, d_accept: int,
                            stream: Stream | None = None) -> None:
         """Asynchronous dispatch on a :class:`DeviceBatch` (any layout)."""
-        _lib.call("trex_dfa_match_ranges", self.handle, batch.d_bytes.ptr, batch.d_begin.ptr, batch.d_end.ptr,
+        fn = "trex_dfa_match_interleaved" if batch.layout == "interleaved" else "trex_dfa_match_ranges"
+        _lib.call(fn, self.handle, batch.d_bytes.ptr, batch.d_begin.ptr, batch.d_end.ptr,
                   batch.n_strings, d_final, d_accept, stream.handle if stream else None)
 
     def match_plan(self, plan: "ChunkPlan", d_bytes: int, d_final: int, d_accept: int,
@@ -139,13 +145,63 @@ def staggered_ranges(offsets: np.ndarray) -> tuple[np.ndarray, np.ndarray, int]:
     return begin, end, size
 
 
+def interleaved_image(data: np.ndarray, offsets: np.ndarray) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """(device image, begin, end) of a CSR batch in the interleaved layout:
+    group g of IL_GROUP consecutive strings holds rows of IL_GROUP x 16 bytes,
+    vector k of its lane j at base_g + (IL_GROUP * k + j) * 16; a group has as
+    many rows as its longest string has (partly filled) vectors; begin[i] is
+    the address of string i's vector 0 and end[i] - begin[i] its length."""
+    data, offsets = _as_csr(data=data, offsets=offsets)
+    n = len(offsets) - 1
+    lengths = offsets[1:] - offsets[:-1]
+    n_groups = -(-n // IL_GROUP)
+    vecs = -(-lengths // 16)
+    gvecs = np.zeros(n_groups, np.int64)
+    if n:
+        np.maximum.at(gvecs, np.arange(n) // IL_GROUP, vecs)
+    block = gvecs * (16 * IL_GROUP)
+    base = np.zeros(n_groups, np.int64)
+    if n_groups > 1:
+        np.cumsum(block[:-1] + IL_GAP, out=base[1:])
+    size = int(base[-1] + block[-1]) if n_groups else 0
+    image = np.zeros(size + IL_GAP, np.uint8)   # a trailing gap keeps every group's block + gap in range
+    lane = np.arange(n, dtype=np.int64) % IL_GROUP
+    begin = base[np.arange(n) // IL_GROUP] + 16 * lane if n else np.zeros(0, np.int64)
+    end = begin + lengths
+    uniform = n and n % IL_GROUP == 0 and np.all(lengths == lengths[0]) and lengths[0] % 16 == 0
+    if uniform:   # one strided copy: [group][lane][vector][16] -> [group][vector][lane][16]
+        L = int(lengths[0])
+        if L:
+            blk = IL_GROUP * L
+            img = image[:n_groups * (blk + IL_GAP)].reshape(n_groups, blk + IL_GAP)
+            src = data[offsets[0]:offsets[0] + n * L].reshape(n_groups, IL_GROUP, L // 16, 16)
+            img[:, :blk] = src.transpose(0, 2, 1, 3).reshape(n_groups, blk)
+    else:
+        for g in range(n_groups):
+            gv = int(gvecs[g])
+            if gv == 0:
+                continue
+            rows = np.zeros((gv, IL_GROUP, 16), np.uint8)
+            for j in range(min(IL_GROUP, n - g * IL_GROUP)):
+                i = g * IL_GROUP + j
+                L = int(lengths[i])
+                if L:
+                    pad = np.zeros(int(vecs[i]) * 16, np.uint8)
+                    pad[:L] = data[offsets[i]:offsets[i + 1]]
+                    rows[:int(vecs[i]), j, :] = pad.reshape(-1, 16)
+            image[base[g]:base[g] + block[g]] = rows.reshape(-1)
+    return image, np.ascontiguousarray(begin, np.int64), np.ascontiguousarray(end, np.int64)
+
+
 class DeviceBatch:
     """A string batch resident in HBM: bytes plus per-string [begin, end) ranges.
 
     ``layout="csr"`` keeps the packed CSR bytes (begin = offsets[:-1],
     end = offsets[1:]); ``layout="staggered"`` moves each group of 64 strings
     256 bytes further than the previous one (at most 0.4% more memory for
-    4 KiB strings), see STAGGER_GAP.
+    4 KiB strings), see STAGGER_GAP; ``layout="interleaved"`` stores each
+    group of 64 strings row by row (:func:`interleaved_image`; DFA kernels
+    only, the groups' strings padded to the longest one).
     """
 
     def __init__(self, data: np.ndarray, offsets: np.ndarray, layout: str = "staggered"):
@@ -167,6 +223,11 @@ class DeviceBatch:
                 lo, hi = int(offsets[g0]), int(offsets[g1])
                 if hi > lo:
                     self.d_bytes.upload(data[lo:hi], offset=int(begin[g0]))
+        elif layout == "interleaved":
+            image, begin, end = interleaved_image(data, offsets)
+            self.d_bytes = DeviceBuffer(-(-len(image) // 16) * 16)
+            self.d_bytes.upload(image)
+            del image
         else:
             raise ValueError(f"unknown layout {layout!r}")
         self.begin = np.ascontiguousarray(begin, dtype=np.int64)
@@ -226,6 +287,8 @@ class NfaEngine:
 
     def match_batch_device(self, batch: "DeviceBatch", d_mask: int, d_accept: int,
                            stream: Stream | None = None) -> None:
+        if batch.layout == "interleaved":
+            raise ValueError("the NFA kernels take CSR or staggered batches, not the interleaved layout")
         _lib.call("trex_nfa_match_ranges", self.handle, batch.d_bytes.ptr, batch.d_begin.ptr, batch.d_end.ptr,
                   batch.n_strings, d_mask, d_accept, stream.handle if stream else None)
 
