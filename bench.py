@@ -56,7 +56,7 @@ HBM_PEAK_GBS = 8000.0
 # counter runs (FETCH_SIZE x 2.000 + WRITE_SIZE, both calibrated against a plain
 # 1 GiB copy); the command lines and counts are in the profile named with each.
 PMC_TRAFFIC = {
-    ("dfa16_k2f_il_pf8", "interleaved"): (1_078_770_000, "profiles/dfa16_k2f_il_pf8_1GiB_8patterns.md"),
+    ("dfa16_k2f_il_pf8", "interleaved"): (1_078_713_000, "profiles/dfa16_k2f_il_pf8_1GiB_8patterns.md"),
     ("dfa16_k2f_pf8", "staggered"): (1_092_300_000, "profiles/dfa16_k2f_pf8_1GiB_8patterns.md"),
 }
 MFMA_F16_PEAK_TFLOPS = 2500.0
@@ -800,14 +800,15 @@ def sweep(args, data, offsets, batch, engines, outs, stream) -> int:
         eng.match_batch_device(batch, outs[i][0].ptr, outs[i][1].ptr, stream)
         ref.append(outs[i][0].download(np.uint8, n, stream))
     sink = DeviceBuffer(max(8192 * 256, n) * 4)
-    ms = time_launches(lambda: _lib.call("trex_read_probe", batch.d_bytes.ptr, data.nbytes, None, None, 0, 0,
-                                         sink.ptr, stream.handle), stream, 5)
-    log(f"read probe coalesced stream      : {ms:.4f} ms = {data.nbytes / (ms * 1e-3) / 1e9:.0f} GB/s")
+    for mode, tag in ((0, ""), (4, " nt")):
+        ms = time_launches(lambda: _lib.call("trex_read_probe", batch.d_bytes.ptr, data.nbytes, None, None, 0, mode,
+                                             sink.ptr, stream.handle), stream, 5)
+        log(f"read probe coalesced stream{tag:3s}   : {ms:.4f} ms = {data.nbytes / (ms * 1e-3) / 1e9:.0f} GB/s")
     for name, b in layouts.items():
-        for mode in ((3,) if name == "interleaved" else (1, 2)):
+        for mode in ((3, 5) if name == "interleaved" else (1, 2)):
             ms = time_launches(lambda: _lib.call("trex_read_probe", b.d_bytes.ptr, 16, b.d_begin.ptr, b.d_end.ptr,
                                                  n, mode, sink.ptr, stream.handle), stream, 5)
-            log(f"read probe {name:11s} pf{8 if mode == 3 else 8 * mode:2d}        : {ms:.4f} ms = "
+            log(f"read probe {name:11s} pf{16 if mode == 2 else 8:2d}{' nt' if mode == 5 else '   '}     : {ms:.4f} ms = "
                 f"{data.nbytes / (ms * 1e-3) / 1e9:.0f} GB/s")
     for name, b in layouts.items():
         for v, pf in [("k1_16", 16), ("k2_window", 8), ("k2_window", 16), ("k2_window_f", 8), ("k2_window_f", 16),

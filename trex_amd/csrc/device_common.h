@@ -43,6 +43,37 @@ __device__ __forceinline__ void stream_body(const uint4* __restrict__ v, int64_t
     for (; i < nvec; ++i) consume(v[i * VSTRIDE]);
 }
 
+// v[i] by a non-temporal load (data read once: no reuse in the caches).
+__device__ __forceinline__ uint4 load16_nt(const uint4* __restrict__ v, int64_t i) {
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    const v4u x = __builtin_nontemporal_load(reinterpret_cast<const v4u*>(v) + i);
+    return make_uint4(x.x, x.y, x.z, x.w);
+}
+
+// stream_body with non-temporal loads, for access patterns where one wave
+// load covers whole cache lines that are not read again.
+template <int PF, int VSTRIDE = 1, class F>
+__device__ __forceinline__ void stream_body_nt(const uint4* __restrict__ v, int64_t nvec, F&& consume) {
+    int64_t i = 0;
+    if (nvec >= PF) {
+        uint4 a[PF];
+#pragma unroll
+        for (int k = 0; k < PF; ++k) a[k] = load16_nt(v, (int64_t)k * VSTRIDE);
+        for (i = PF; i + PF <= nvec; i += PF) {
+            uint4 b[PF];
+#pragma unroll
+            for (int k = 0; k < PF; ++k) b[k] = load16_nt(v, (i + k) * VSTRIDE);
+#pragma unroll
+            for (int k = 0; k < PF; ++k) consume(a[k]);
+#pragma unroll
+            for (int k = 0; k < PF; ++k) a[k] = b[k];
+        }
+#pragma unroll
+        for (int k = 0; k < PF; ++k) consume(a[k]);
+    }
+    for (; i < nvec; ++i) consume(load16_nt(v, i * VSTRIDE));
+}
+
 // Copy `n` bytes (multiple of 16) from global to LDS with the whole block.
 __device__ __forceinline__ void stage_table(uint8_t* __restrict__ dst, const uint8_t* __restrict__ src, int n) {
     const uint4* s4 = reinterpret_cast<const uint4*>(src);

@@ -10,9 +10,12 @@
 //    begin = offsets, end = offsets + 1, and the staggered HBM layout (strings
 //    grouped per wave, groups separated by a gap so that concurrent waves read
 //    different HBM channels) passes its own begin/end arrays;
-//  * the body of each string is read with 16-byte global loads through a
-//    rotating buffer of PF vectors per lane: every consumed slot is refilled
-//    at once with the vector PF positions ahead, so PF loads stay in flight;
+//  * the body of each string is read with 16-byte global loads, PF vectors
+//    in flight per lane (two blocks of PF, double-buffered). The _il kernels
+//    issue them non-temporal: a wave load there covers whole cache lines that
+//    are never read again (measured: +12-14 % HBM read rate). Contiguous
+//    strings keep ordinary loads: a lane's next loads hit the cache line its
+//    last one brought in (non-temporal loads cost 2.5x there);
 //  * the "window" kernels fold two bytes per table step ("K=4": one 32-bit
 //    word = two pair steps) when every byte that does not lead straight to
 //    the dead state lies in one aligned 32-byte window [base, base+32); the
@@ -164,8 +167,8 @@ __device__ __forceinline__ void window_kernel(const uint8_t* __restrict__ bytes,
     if constexpr (IL) {
         const int64_t len = end - p;
         const int64_t nvec = len >> 4;
-        stream_body<PF, kIlGroup>(reinterpret_cast<const uint4*>(bytes + p), nvec,
-                                  [&](uint4 x) { s = window_vec<Step>(tl, s, x, w); });
+        stream_body_nt<PF, kIlGroup>(reinterpret_cast<const uint4*>(bytes + p), nvec,
+                                        [&](uint4 x) { s = window_vec<Step>(tl, s, x, w); });
         const uint8_t* tail = bytes + p + nvec * (16 * kIlGroup);
         for (int t = 0; t < (int)(len & 15); ++t) s = t1[((uint32_t)tail[t] << 4) | s];
     } else {
@@ -206,8 +209,8 @@ __device__ __forceinline__ void k1_kernel(const uint8_t* __restrict__ bytes, con
     if constexpr (IL) {
         const int64_t len = end - p;
         const int64_t nvec = len >> 4;
-        stream_body<8, kIlGroup>(reinterpret_cast<const uint4*>(bytes + p), nvec,
-                                 [&](uint4 x) { s = k1_vec<STRIDE_LOG2>(t1, s, x, stride); });
+        stream_body_nt<8, kIlGroup>(reinterpret_cast<const uint4*>(bytes + p), nvec,
+                                       [&](uint4 x) { s = k1_vec<STRIDE_LOG2>(t1, s, x, stride); });
         const uint8_t* tail = bytes + p + nvec * (16 * kIlGroup);
         for (int t = 0; t < (int)(len & 15); ++t) s = k1_byte<STRIDE_LOG2>(t1, s, tail[t], stride);
     } else {
@@ -277,16 +280,29 @@ TREX_K1_KERNEL(dfa_k1_il, -1, true)
 // work (XOR of every word), over the same begin/end ranges the DFA kernels
 // take. Each lane writes one word so the loads are kept.
 // ---------------------------------------------------------------------------
-extern "C" __global__ void __launch_bounds__(TREX_BLOCK)
-read_probe_stream(const uint4* __restrict__ src, int64_t n16, uint32_t* __restrict__ sink)
+template <bool NT>
+__device__ __forceinline__ void read_probe_stream_body(const uint4* __restrict__ src, int64_t n16,
+                                                       uint32_t* __restrict__ sink)
 {
     const int64_t stride = (int64_t)gridDim.x * TREX_BLOCK;
     uint32_t x = 0;
     for (int64_t i = (int64_t)blockIdx.x * TREX_BLOCK + threadIdx.x; i < n16; i += stride) {
-        const uint4 v = src[i];
+        const uint4 v = NT ? load16_nt(src, i) : src[i];
         x ^= v.x ^ v.y ^ v.z ^ v.w;
     }
     sink[(int64_t)blockIdx.x * TREX_BLOCK + threadIdx.x] = x;
+}
+
+extern "C" __global__ void __launch_bounds__(TREX_BLOCK)
+read_probe_stream(const uint4* __restrict__ src, int64_t n16, uint32_t* __restrict__ sink)
+{
+    read_probe_stream_body<false>(src, n16, sink);
+}
+
+extern "C" __global__ void __launch_bounds__(TREX_BLOCK)
+read_probe_stream_nt(const uint4* __restrict__ src, int64_t n16, uint32_t* __restrict__ sink)
+{
+    read_probe_stream_body<true>(src, n16, sink);
 }
 
 template <int PF>
@@ -319,19 +335,36 @@ read_probe_strings_pf16(const uint8_t* __restrict__ bytes, const int64_t* __rest
     read_probe_strings_body<16>(bytes, begin, end, n_strings, sink);
 }
 
-// The interleaved layout's access pattern (one 1 KiB row per wave load).
-extern "C" TREX_STRING_KERNEL
-read_probe_il_pf8(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
-                  const int64_t* __restrict__ endp, int64_t n_strings, uint32_t* __restrict__ sink)
+// The interleaved layout's access pattern (one 1 KiB row per wave load),
+// with ordinary or non-temporal loads.
+template <bool NT>
+__device__ __forceinline__ void read_probe_il_body(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
+                                                   const int64_t* __restrict__ endp, int64_t n_strings,
+                                                   uint32_t* __restrict__ sink)
 {
     const int64_t sid = (int64_t)blockIdx.x * TREX_BLOCK + threadIdx.x;
     if (sid >= n_strings) return;
     const int64_t p = begin[sid];
     const int64_t nvec = (endp[sid] - p) >> 4;
     uint32_t x = 0;
-    stream_body<8, kIlGroup>(reinterpret_cast<const uint4*>(bytes + p), nvec,
-                             [&](uint4 v) { x ^= v.x ^ v.y ^ v.z ^ v.w; });
+    auto eat = [&](uint4 v) { x ^= v.x ^ v.y ^ v.z ^ v.w; };
+    if constexpr (NT) stream_body_nt<8, kIlGroup>(reinterpret_cast<const uint4*>(bytes + p), nvec, eat);
+    else stream_body<8, kIlGroup>(reinterpret_cast<const uint4*>(bytes + p), nvec, eat);
     sink[sid] = x;
+}
+
+extern "C" TREX_STRING_KERNEL
+read_probe_il_pf8(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
+                  const int64_t* __restrict__ endp, int64_t n_strings, uint32_t* __restrict__ sink)
+{
+    read_probe_il_body<false>(bytes, begin, endp, n_strings, sink);
+}
+
+extern "C" TREX_STRING_KERNEL
+read_probe_il_nt_pf8(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
+                     const int64_t* __restrict__ endp, int64_t n_strings, uint32_t* __restrict__ sink)
+{
+    read_probe_il_body<true>(bytes, begin, endp, n_strings, sink);
 }
 
 // ---------------------------------------------------------------------------
