@@ -200,8 +200,10 @@ void nfa_tc_engine_destroy(void);
  *   logmass = log(1 . T[c_L] ... T[c_1] s_0)
  * in FP64, from FP16/BF16 MFMA products with FP32 accumulation, a power-of-two
  * rescale per step into an integer log2-scale, and the last step recomputed in
- * FP32 from the unrounded T. A symbol >= n_symbols empties the state
- * (LL = -inf).
+ * FP32 from the unrounded T. For a nonnegative model every product's mass is
+ * checked against sum_k colsum(round T)_k x_k computed on the vector ALUs and
+ * restored (the matrix cores' FP32 accumulation drifts one way). A symbol
+ * >= n_symbols empties the state (LL = -inf).
  * ------------------------------------------------------------------------ */
 typedef struct trex_pnfa trex_pnfa;
 
@@ -227,9 +229,10 @@ int trex_pnfa_set_waves(trex_pnfa* h, int waves);
  * reference (it has no probabilistic kernel). */
 int trex_pnfa_set_grid(trex_pnfa* h, int64_t workgroups);
 /* The operator a mode applies on every step but the last: the rounded T as FP32
- * [n_symbols][64][64] and the column mass factors [n_symbols][64], so that
+ * [n_symbols][64][64] and the column mass factors [n_symbols][64] (FP64: the
+ * kernels apply each as an FP32 high part plus its FP32 rest), so that
  * s' = T_rounded (col_scale[c] * s). Either output may be null. */
-int trex_pnfa_rounded(const trex_pnfa* h, int mode, float* t_rounded, float* col_scale);
+int trex_pnfa_rounded(const trex_pnfa* h, int mode, float* t_rounded, double* col_scale);
 /* d_symbols: [n_seq][length] uint8 symbol indices; d_ll, d_logmass: [n_seq]
  * FP64, only written (pinned host memory works too). Asynchronous on
  * `stream`. length <= 2^31 - 1. */
@@ -241,6 +244,18 @@ int trex_pnfa_loglik(trex_pnfa* h, const uint8_t* d_symbols, int64_t n_seq, int6
 int trex_pnfa_loglik_ranges(trex_pnfa* h, const uint8_t* d_symbols, const int64_t* d_begin, const int64_t* d_end,
                             int64_t n_seq, double* d_ll, double* d_logmass, void* stream);
 /* Host-pointer form (synchronous), with kernel and end-to-end times. */
+/* Parallel in time, for long sequences: sequence j = d_symbols[begin[j] ..
+ * end[j]) with begin / end in HOST memory (the chunk table is built on the
+ * host). Each sequence's rounded steps are cut into chunks of chunk_len
+ * symbols (0: about sixteen chunks per compute unit over the call) whose
+ * 64 x 64 products run concurrently on MFMA, then chained per sequence in
+ * FP64; the same operator and last FP32 step as trex_pnfa_loglik_ranges.
+ * About 64 times the arithmetic of the batch kernel, in exchange for a
+ * latency set by chunk_len rather than by the sequence length. Workspace
+ * (16.3 KB per chunk) is allocated and freed in stream order on `stream`.
+ * No reference counterpart. */
+int trex_pnfa_loglik_chunked(trex_pnfa* h, const uint8_t* d_symbols, const int64_t* begin, const int64_t* end,
+                             int64_t n_seq, int64_t chunk_len, double* d_ll, double* d_logmass, void* stream);
 int trex_pnfa_loglik_host(trex_pnfa* h, const uint8_t* symbols, int64_t n_seq, int64_t length, double* ll,
                           double* logmass, float* kernel_ms, float* total_ms);
 

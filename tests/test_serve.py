@@ -61,17 +61,31 @@ def test_gpu_served_equals_alone(gpu_lib):
     engines += [ProbNfaEngine(*dirichlet_pnfa(rng, 32))]
     kinds = ["mix"] * 5 + ["prob"]
     reqs = _requests(rng, len(engines), kinds, 60, 64, 512, 2.0, 300_000)
-    server = Server(engines, hbm_budget=256 << 20, small_bytes=64 << 10)
+    server = Server(engines, hbm_budget=256 << 20, small_bytes=64 << 10, prob_chunk_min=1024, prob_chunk_len=128)
     with server:
         for r in reqs:
             server.submit(r)
         for r in reqs:
             assert r.wait(60), "request not served"
     assert server.ticks["small"] >= 1 and server.ticks["bulk"] >= 1
+    assert any(r.lane == "prob_small" and np.any(np.diff(r.offsets) > server.prob_chunk_min) for r in reqs), \
+        "no small probabilistic request exercises the parallel-in-time path"
     for r in reqs:
         e = engines[r.pattern]
         if isinstance(e, ProbNfaEngine):
-            ll, lm = e.loglik_ragged(r.data, r.offsets)
+            # alone: the server's routing rule per sequence (each sequence's result
+            # depends only on itself and the fixed chunk length)
+            lengths = np.diff(r.offsets)
+            chunked = (lengths > server.prob_chunk_min) if r.lane == "prob_small" else np.zeros(len(lengths), bool)
+            ll = np.empty(len(lengths))
+            lm = np.empty(len(lengths))
+            for sel, kw in ((chunked, {"chunk_len": server.prob_chunk_len}), (~chunked, {})):
+                idx = np.flatnonzero(sel)
+                if len(idx):
+                    parts = [r.data[r.offsets[i]:r.offsets[i + 1]] for i in idx]
+                    offs = np.concatenate([[0], np.cumsum([len(x) for x in parts])]).astype(np.int64)
+                    a, b = e.loglik_ragged(np.concatenate(parts) if offs[-1] else np.zeros(0, np.uint8), offs, **kw)
+                    ll[idx], lm[idx] = a, b
             assert np.array_equal(r.result[0], ll) and np.array_equal(r.result[1], lm)
         else:
             final, acc, _, _ = e.match_arrays(r.data, r.offsets)

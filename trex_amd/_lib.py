@@ -86,6 +86,8 @@ _SIGNATURES = {
     "trex_pnfa_loglik": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
     "trex_pnfa_loglik_ranges": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
                                         c_void_p]),
+    "trex_pnfa_loglik_chunked": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p,
+                                         c_void_p]),
     "trex_pnfa_loglik_host": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, f32p, f32p]),
     # mixed batches
     "trex_mix_create": (c_int, [c_void_p, c_void_p, c_int, vpp]),

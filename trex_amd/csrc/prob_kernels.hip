@@ -34,38 +34,17 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
+#include "prob_common.h"
 
 namespace {
+
+using namespace pnfadev;
 
 constexpr int kStates = 64;
 constexpr int kRow = kStates + 4;   // padded LDS row (floats) against bank conflicts
 constexpr int kSeqPerGroup = 64;
 constexpr int kMaxSymbols = 32;
 constexpr int kChunk = 64;          // symbols staged per sequence at a time
-
-template <bool BF16>
-struct Lp;
-
-template <>
-struct Lp<false> {
-    typedef f16x8 vec;
-    typedef _Float16 scalar;
-    __device__ __forceinline__ static f32x4 mfma(vec a, vec b, f32x4 c) {
-        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-    }
-};
-
-template <>
-struct Lp<true> {
-    typedef bf16x8 vec;
-    typedef __bf16 scalar;
-    __device__ __forceinline__ static f32x4 mfma(vec a, vec b, f32x4 c) {
-        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-    }
-};
 
 // Position of the n-th set bit of m (n < popcount(m)).
 __device__ __forceinline__ int nth_set_bit(uint64_t m, int n) {
@@ -92,35 +71,6 @@ __device__ __forceinline__ int nth_set_bit(uint64_t m, int n) {
     return base;
 }
 
-// Largest value over the four lanes l, l^16, l^32, l^48 (the lanes that hold
-// one column of an MFMA result), with the gfx950 lane-swap instructions.
-__device__ __forceinline__ float column_max(float v) {
-    unsigned u = __float_as_uint(v);
-    const auto a = __builtin_amdgcn_permlane32_swap(u, u, false, false);
-    v = fmaxf(__uint_as_float(a[0]), __uint_as_float(a[1]));
-    u = __float_as_uint(v);
-    const auto b = __builtin_amdgcn_permlane16_swap(u, u, false, false);
-    return fmaxf(__uint_as_float(b[0]), __uint_as_float(b[1]));
-}
-
-// Eight FP32 values -> MFMA operand: the high half (FP32 truncated to the
-// operand's mantissa, converted exactly) and, when split, the rounded rest.
-template <bool BF16, bool SPLIT>
-__device__ __forceinline__ void split8(const float (&x)[8], typename Lp<BF16>::vec& hi, typename Lp<BF16>::vec& lo) {
-    typedef typename Lp<BF16>::scalar scalar;
-    constexpr unsigned kMask = BF16 ? 0xffff0000u : 0xffffe000u;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        if constexpr (SPLIT) {
-            const float h = __uint_as_float(__float_as_uint(x[e]) & kMask);
-            hi[e] = (scalar)h;
-            lo[e] = (scalar)(x[e] - h);
-        } else {
-            hi[e] = (scalar)x[e];
-        }
-    }
-}
-
 template <bool BF16, bool SPLIT, int SPW>
 __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __restrict__ colscale,
                              const float* __restrict__ t32, int n_sym, const float* __restrict__ s0, int e0,
@@ -137,7 +87,10 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
     // One state row per sequence, updated in place: a row is read and then
     // rewritten by the one wave that owns its sequence's symbol at that step.
     __shared__ __attribute__((aligned(16))) float state[kSeqPerGroup][kRow];
-    __shared__ __attribute__((aligned(16))) float dscale[kMaxSymbols][kStates];
+    __shared__ __attribute__((aligned(16))) float dscale[kMaxSymbols][kStates];      // column factors, high part
+    __shared__ __attribute__((aligned(16))) float dscale_lo[kMaxSymbols][kStates];   // and the FP32 rest
+    __shared__ __attribute__((aligned(16))) float dcq[kMaxSymbols][kStates];         // colsum(round T), high
+    __shared__ __attribute__((aligned(16))) float dcq_lo[kMaxSymbols][kStates];      // and rest
     __shared__ uint8_t syms[kSeqPerGroup][kChunk + 4];
     __shared__ int32_t estep[2][kSeqPerGroup];   // exponent taken out at a step, by step parity
     __shared__ int32_t expo[kSeqPerGroup];
@@ -196,8 +149,13 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
             syms[s][k] = v;
         }
     };
-    for (int i = tid; i < kMaxSymbols * kStates; i += kBlock)
-        dscale[i / kStates][i % kStates] = i < n_sym * kStates ? colscale[i] : 1.f;
+    for (int i = tid; i < kMaxSymbols * kStates; i += kBlock) {
+        const bool in = i < n_sym * kStates;
+        dscale[i / kStates][i % kStates] = in ? colscale[i] : 1.f;
+        dscale_lo[i / kStates][i % kStates] = in ? colscale[n_sym * kStates + i] : 0.f;
+        dcq[i / kStates][i % kStates] = in ? colscale[2 * n_sym * kStates + i] : 0.f;
+        dcq_lo[i / kStates][i % kStates] = in ? colscale[3 * n_sym * kStates + i] : 0.f;
+    }
     if (tid < 2 * kSeqPerGroup) estep[tid / kSeqPerGroup][tid % kSeqPerGroup] = 0;
     __syncthreads();
     stage(0);
@@ -207,7 +165,7 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
     for (int i = tid; i < kSeqPerGroup * kStates; i += kBlock) {
         const int s = i / kStates, k = i % kStates;
         const int c0 = syms[s][0];
-        state[s][k] = s0[k] * (c0 < n_sym ? dscale[c0][k] : 1.f);
+        state[s][k] = c0 < n_sym ? fmaf(s0[k], dscale[c0][k], s0[k] * dscale_lo[c0][k]) : s0[k];
     }
     int expo_home = e0;   // wave 0, lane j: log2-scale of sequence j
     const int steps = max_len - 1;
@@ -242,21 +200,21 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
             const int cn = syms[seq][tc + 1];
             const int myq = valid ? c - wave * SPW : -1;
             vec bhi[2], blo[2];
+            float mass_in = 0.f;   // sum_k colsum(round T[c])_k x_k over this lane's 16 k
 #pragma unroll
             for (int kb = 0; kb < 2; ++kb) {
                 const float4* src = reinterpret_cast<const float4*>(&state[seq][kb * 32 + koff]);
                 const float4 x0 = src[0], x1 = src[1];
                 const float x[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+                const float4* qh = reinterpret_cast<const float4*>(&dcq[c][kb * 32 + koff]);
+                const float4* ql = reinterpret_cast<const float4*>(&dcq_lo[c][kb * 32 + koff]);
+                const float4 qa = qh[0], qb = qh[1], ra = ql[0], rb2 = ql[1];
+                const float q[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+                const float r[8] = {ra.x, ra.y, ra.z, ra.w, rb2.x, rb2.y, rb2.z, rb2.w};
+#pragma unroll
+                for (int e = 0; e < 8; ++e) mass_in = fmaf(x[e], q[e], fmaf(x[e], r[e], mass_in));
                 split8<BF16, SPLIT>(x, bhi[kb], blo[kb]);
             }
-            // column factors of the next symbol for this lane's 16 result rows
-            // (none when the next step is the last one or the sequence ends)
-            float4 f[4];
-            const bool use_f = cn < n_sym;
-#pragma unroll
-            for (int rb = 0; rb < 4; ++rb)
-                f[rb] = use_f ? *reinterpret_cast<const float4*>(&dscale[cn][rb * 16 + rsub])
-                              : make_float4(1.f, 1.f, 1.f, 1.f);
             f32x4 d[4];
 #pragma unroll
             for (int rb = 0; rb < 4; ++rb) d[rb] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -279,17 +237,38 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
                         if constexpr (SPLIT) d[rb] = Lp<BF16>::mfma(A[q][rb][kb], bl[kb], d[rb]);
                     }
             }
-            float mx = 0.f;
+            // column factors of the next symbol for this lane's 16 result rows
+            // (none when the next step is the last one or the sequence ends)
+            float4 f[4], fl[4];
+            const bool use_f = cn < n_sym;
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb)
+            for (int rb = 0; rb < 4; ++rb) {
+                f[rb] = use_f ? *reinterpret_cast<const float4*>(&dscale[cn][rb * 16 + rsub])
+                              : make_float4(1.f, 1.f, 1.f, 1.f);
+                fl[rb] = use_f ? *reinterpret_cast<const float4*>(&dscale_lo[cn][rb * 16 + rsub])
+                               : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            float mx = 0.f, mass_out = 0.f;
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
                 mx = fmaxf(mx, fmaxf(fmaxf(fabsf(d[rb][0]), fabsf(d[rb][1])), fmaxf(fabsf(d[rb][2]), fabsf(d[rb][3]))));
-            mx = column_max(mx);
+                mass_out += (d[rb][0] + d[rb][1]) + (d[rb][2] + d[rb][3]);
+            }
+            // the product's mass back to what its operands carry: the matrix cores'
+            // FP32 accumulation loses a little mass on every step, in one direction
+            mass_in = column_sum(mass_in);
+            mass_out = column_sum(mass_out);
+            const float fix = mass_out > 0.f && mass_in > 0.f ? mass_in / mass_out : 1.f;
+            mx = column_max(mx) * fix;
             const int e2 = mx > 0.f ? __builtin_amdgcn_frexp_expf(mx) : 0;   // mx = f * 2^e2, f in [0.5, 1)
             if (valid) {
 #pragma unroll
                 for (int rb = 0; rb < 4; ++rb) {
-                    const float4 out = make_float4(ldexpf(d[rb][0] * f[rb].x, -e2), ldexpf(d[rb][1] * f[rb].y, -e2),
-                                                   ldexpf(d[rb][2] * f[rb].z, -e2), ldexpf(d[rb][3] * f[rb].w, -e2));
+                    const f32x4 dv = d[rb] * fix;
+                    const float4 out = make_float4(ldexpf(fmaf(dv[0], f[rb].x, dv[0] * fl[rb].x), -e2),
+                                                   ldexpf(fmaf(dv[1], f[rb].y, dv[1] * fl[rb].y), -e2),
+                                                   ldexpf(fmaf(dv[2], f[rb].z, dv[2] * fl[rb].z), -e2),
+                                                   ldexpf(fmaf(dv[3], f[rb].w, dv[3] * fl[rb].w), -e2));
                     *reinterpret_cast<float4*>(&state[seq][rb * 16 + rsub]) = out;
                 }
                 if ((lane >> 4) == 0) estep[par][seq] = e2;

@@ -369,9 +369,10 @@ class ProbNfaEngine:
         _lib.call("trex_pnfa_set_mode", self.handle, PNFA_MODES[mode])
 
     def rounded(self, mode: str):
-        """(T_rounded [σ,64,64] float32, col_scale [σ,64] float32) of ``mode``."""
+        """(T_rounded [σ,64,64] float32, col_scale [σ,64] float64) of ``mode``: the
+        operator s' = T_rounded (col_scale[c] * s) the kernels apply on every step but the last."""
         t = np.empty_like(self.T)
-        sc = np.empty((self.n_symbols, self.N), dtype=np.float32)
+        sc = np.empty((self.n_symbols, self.N), dtype=np.float64)
         _lib.call("trex_pnfa_rounded", self.handle, PNFA_MODES[mode], t.ctypes.data, sc.ctypes.data)
         return t, sc
 
@@ -386,14 +387,37 @@ class ProbNfaEngine:
         _lib.call("trex_pnfa_loglik_ranges", self.handle, d_symbols, d_begin, d_end, int(n_seq), d_ll, d_logmass,
                   stream.handle if stream else None)
 
-    def loglik_ragged(self, data: np.ndarray, offsets: np.ndarray):
+    def loglik_chunked_device(self, d_symbols: int, begin: np.ndarray, end: np.ndarray, d_ll: int, d_logmass: int,
+                              chunk_len: int = 0, stream: Stream | None = None) -> None:
+        """Parallel in time: sequence j = symbols[begin[j]:end[j]] with begin/end host arrays;
+        chunk products of chunk_len symbols (0: about sixteen chunks per compute unit) run
+        concurrently and are chained per sequence (latency set by chunk_len, not the length)."""
+        begin = np.ascontiguousarray(begin, dtype=np.int64)
+        end = np.ascontiguousarray(end, dtype=np.int64)
+        if begin.shape != end.shape:
+            raise ValueError("begin and end differ in length")
+        _lib.call("trex_pnfa_loglik_chunked", self.handle, d_symbols, begin.ctypes.data, end.ctypes.data, len(begin),
+                  int(chunk_len), d_ll, d_logmass, stream.handle if stream else None)
+
+    def loglik_ragged(self, data: np.ndarray, offsets: np.ndarray, chunk_len: int | None = None):
         """Host CSR batch of symbol sequences -> (ll float64[n], logmass float64[n]). Sequences are
-        grouped by length (longest first) for the launch and returned in input order."""
+        grouped by length (longest first) for the launch and returned in input order. With
+        ``chunk_len`` (0: automatic) the parallel-in-time path runs instead of the batch kernel."""
         data, offsets = _as_csr(data=data, offsets=offsets)
         n = len(offsets) - 1
         ll = np.empty(n, np.float64)
         lm = np.empty(n, np.float64)
         if n == 0:
+            return ll, lm
+        if chunk_len is not None:
+            stream = Stream()
+            d_sym = DeviceBuffer.from_array(data if len(data) else np.zeros(16, np.uint8), stream)
+            d_ll, d_lm = DeviceBuffer(8 * n), DeviceBuffer(8 * n)
+            self.loglik_chunked_device(d_sym.ptr, offsets[:-1], offsets[1:], d_ll.ptr, d_lm.ptr, chunk_len, stream)
+            ll[:] = d_ll.download(np.float64, n, stream)
+            lm[:] = d_lm.download(np.float64, n, stream)
+            for buf in (d_sym, d_ll, d_lm):
+                buf.free()
             return ll, lm
         order = np.argsort(-(offsets[1:] - offsets[:-1]), kind="stable")
         stream = Stream()

@@ -27,7 +27,10 @@ Boolean NFA-64 and probabilistic NFA-64 engines) and serves match requests
   ``prob_grid`` workgroups per kernel launch, set on every probabilistic
   engine the server holds); each probabilistic lane has ``prob_slots`` slots
   with a stream each, so a few long sequential launches overlap one another
-  while the compute units they hold stay bounded;
+  while the compute units they hold stay bounded. In small probabilistic
+  requests, sequences longer than ``prob_chunk_min`` symbols take the
+  parallel-in-time path (chunks of ``prob_chunk_len`` multiplied concurrently,
+  then chained), whose latency does not grow with the sequence length;
 * no call that synchronises the whole device (``hipFree``, a synchronous
   copy) runs while serving: tick buffers are reserved up front and grow
   geometrically, and mixed-queue plans are copied asynchronously;
@@ -163,7 +166,8 @@ class Server:
     def __init__(self, engines: list, hbm_budget: int = 8 << 30, small_bytes: int = 1 << 20,
                  small_params: MixParams | None = None, bulk_params: MixParams | None = None,
                  bulk_grid: int | None = None, small_budget: int | None = None, prob_slots: int = 2,
-                 bulk_slots: int = 1, prob_grid: int = 32, reserve_strings: int = 1 << 18):
+                 bulk_slots: int = 1, prob_grid: int = 32, reserve_strings: int = 1 << 18,
+                 prob_chunk_min: int = 1024, prob_chunk_len: int = 256):
         hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4)   # HIP's default is 4
         if hwq < 8:
             warnings.warn(f"GPU_MAX_HW_QUEUES is {hwq}: the server's seven streams share hardware queues, so "
@@ -184,6 +188,8 @@ class Server:
                 raise TypeError(f"pattern {i}: unsupported engine {type(e).__name__}")
         self.mix_index = mix_index
         self.small_bytes = int(small_bytes)
+        self.prob_chunk_min = int(prob_chunk_min)
+        self.prob_chunk_len = int(prob_chunk_len)   # fixed, so a sequence's result does not depend on its neighbours
         self.arena = Arena(hbm_budget)
         self.small_arena = Arena(small_budget if small_budget is not None else max(64 * self.small_bytes, 256 << 20))
         self.small_stage = PinnedBuffer(self.small_arena.size)   # host mirror of the small arena
@@ -341,7 +347,7 @@ class Server:
                         _lib.call("trex_memcpy_kernel", r._arena_obj.buf.ptr + r._arena,
                                   self.small_stage.ptr + r._arena, r.nbytes, stream.handle)
                 if prob:
-                    self._launch_prob(slot, reqs, stream)
+                    self._launch_prob(slot, reqs, stream, self.prob_chunk_min if name == "prob_small" else 0)
                 else:
                     self._launch_mix(slot, reqs, stream, lane["params"])
                 slot.event.record(stream)
@@ -372,7 +378,10 @@ class Server:
         if n:   # results written by the kernel into pinned host memory
             slot.mix.run(base, slot.h_mask.ptr, slot.h_acc.ptr, stream)
 
-    def _launch_prob(self, slot: _Slot, reqs: list[Request], stream: Stream) -> None:
+    def _launch_prob(self, slot: _Slot, reqs: list[Request], stream: Stream, chunk_min: int = 0) -> None:
+        """One ragged launch per request; with chunk_min > 0, the request's sequences
+        longer than chunk_min symbols (a prefix of the longest-first order) take the
+        parallel-in-time path, whose latency does not grow with their length."""
         base = reqs[0]._arena_obj.buf.ptr
         n = sum(len(r.offsets) - 1 for r in reqs)
         slot.reserve(n)
@@ -398,11 +407,21 @@ class Server:
         pin[n:2 * n] = end
         _lib.call("trex_memcpy_kernel", slot.d_begin.ptr, slot.h_f64.ptr, 16 * n, stream.handle)
         d_end = slot.d_begin.ptr + 8 * n
-        for r, (k0, m, _) in zip(reqs, slot.order):
-            if m:
-                self.engines[r.pattern].loglik_ranges_device(base, slot.d_begin.ptr + 8 * k0, d_end + 8 * k0, m,
-                                                             slot.h_f64.ptr + 8 * k0, slot.h_f64.ptr + 8 * (n + k0),
-                                                             stream)
+        for r, (k0, m, order) in zip(reqs, slot.order):
+            if not m:
+                continue
+            eng = self.engines[r.pattern]
+            n_long = 0
+            if chunk_min > 0:
+                lengths = end[k0:k0 + m] - begin[k0:k0 + m]
+                n_long = int(np.count_nonzero(lengths > chunk_min))
+            if n_long:
+                eng.loglik_chunked_device(base, begin[k0:k0 + n_long], end[k0:k0 + n_long], slot.h_f64.ptr + 8 * k0,
+                                          slot.h_f64.ptr + 8 * (n + k0), self.prob_chunk_len, stream)
+            if n_long < m:
+                k1 = k0 + n_long
+                eng.loglik_ranges_device(base, slot.d_begin.ptr + 8 * k1, d_end + 8 * k1, m - n_long,
+                                         slot.h_f64.ptr + 8 * k1, slot.h_f64.ptr + 8 * (n + k1), stream)
 
     def _complete(self, name: str, slot: _Slot) -> None:
         now = time.perf_counter()
