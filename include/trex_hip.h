@@ -247,7 +247,8 @@ int trex_pnfa_loglik_ranges(trex_pnfa* h, const uint8_t* d_symbols, const int64_
 /* Parallel in time, for long sequences: sequence j = d_symbols[begin[j] ..
  * end[j]) with begin / end in HOST memory (the chunk table is built on the
  * host). Each sequence's rounded steps are cut into chunks of chunk_len
- * symbols (0: about sixteen chunks per compute unit over the call) whose
+ * symbols (0: about four chunks per compute unit over the call, at least 256
+ * symbols each: the error grows with the number of chunks) whose
  * 64 x 64 products run concurrently on MFMA, then chained per sequence in
  * FP64; the same operator and last FP32 step as trex_pnfa_loglik_ranges.
  * About 64 times the arithmetic of the batch kernel, in exchange for a

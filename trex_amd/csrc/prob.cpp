@@ -380,8 +380,11 @@ int trex_pnfa_loglik_chunked(trex_pnfa* h, const uint8_t* d_symbols, const int64
         if (begin[j] < 0 || end[j] < begin[j]) return fail(TREX_ERR_INVALID, "sequence %lld: bad range", (long long)j);
         steps += std::max<int64_t>(end[j] - begin[j] - 1, 0);
     }
-    if (chunk_len == 0)   // about sixteen chunks per compute unit
-        chunk_len = std::max<int64_t>(64, (steps + 16 * device_cu_count() - 1) / (16 * device_cu_count()));
+    // auto: about four chunks per compute unit, at least 256 symbols each (the
+    // chained result's error grows with the number of chunks, ~1e-7 to 5e-7
+    // relative LL per chunk measured, so chunks stay long)
+    if (chunk_len == 0)
+        chunk_len = std::max<int64_t>(256, (steps + 4 * device_cu_count() - 1) / (4 * device_cu_count()));
     if (chunk_len > 0x7fffffff) return fail(TREX_ERR_INVALID, "chunk_len beyond 2^31 - 1");
     int64_t n_chunks = 0;
     for (int64_t j = 0; j < n_seq; ++j)

@@ -243,20 +243,28 @@ pnfa64_combine(const float* __restrict__ chunk_p, const int32_t* __restrict__ ch
     int64_t scale = e0;
     const int64_t c0 = first[j];
     const int nc = count[j];
+    // software pipeline: chunk k + 1's row of P and exponent load while chunk k is applied
+    float4 row[kStates / 4], row_n[kStates / 4];
+    int ek = 0, ek_n = 0;
+    auto fetch = [&](int k, float4 (&dst)[kStates / 4], int& e) {
+        const float4* src = reinterpret_cast<const float4*>(chunk_p + (c0 + k) * (kStates * kStates) + r * kStates);
+#pragma unroll
+        for (int q = 0; q < kStates / 4; ++q) dst[q] = src[q];
+        e = chunk_e[(c0 + k) * kStates + r];
+    };
+    if (nc > 0) fetch(0, row, ek);
     for (int k = 0; k < nc; ++k) {
-        const int32_t* E = chunk_e + (c0 + k) * kStates;
-        const float* P = chunk_p + (c0 + k) * (kStates * kStates);
+        if (k + 1 < nc) fetch(k + 1, row_n, ek_n);
         // v = s * 2^E, aligned to the largest exponent so no entry overflows
         int xs = 0;
         const double m = frexp(s, &xs);
-        const int tot = s != 0.0 ? xs + E[r] : INT32_MIN;
+        const int tot = s != 0.0 ? xs + ek : INT32_MIN;
         const int top = wave_max_int(tot);
         if (top == INT32_MIN) break;   // the state is empty: it stays empty
         vs[w][r] = s != 0.0 ? ldexp(m, tot - top) : 0.0;
         __builtin_amdgcn_wave_barrier();
         double acc = 0.0;
-        const float4* row = reinterpret_cast<const float4*>(P + r * kStates);
-#pragma unroll 4
+#pragma unroll
         for (int q = 0; q < kStates / 4; ++q) {
             const float4 p = row[q];
             acc = fma((double)p.x, vs[w][4 * q], acc);
@@ -265,6 +273,9 @@ pnfa64_combine(const float* __restrict__ chunk_p, const int32_t* __restrict__ ch
             acc = fma((double)p.w, vs[w][4 * q + 3], acc);
         }
         __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int q = 0; q < kStates / 4; ++q) row[q] = row_n[q];
+        ek = ek_n;
         scale += top;
         const double mx = wave_max(fabs(acc));
         int e = 0;

@@ -390,8 +390,9 @@ class ProbNfaEngine:
     def loglik_chunked_device(self, d_symbols: int, begin: np.ndarray, end: np.ndarray, d_ll: int, d_logmass: int,
                               chunk_len: int = 0, stream: Stream | None = None) -> None:
         """Parallel in time: sequence j = symbols[begin[j]:end[j]] with begin/end host arrays;
-        chunk products of chunk_len symbols (0: about sixteen chunks per compute unit) run
-        concurrently and are chained per sequence (latency set by chunk_len, not the length)."""
+        chunk products of chunk_len symbols (0: about four chunks per compute unit, at least
+        256 symbols) run concurrently and are chained per sequence (latency set by chunk_len,
+        not the length; the error grows with the number of chunks)."""
         begin = np.ascontiguousarray(begin, dtype=np.int64)
         end = np.ascontiguousarray(end, dtype=np.int64)
         if begin.shape != end.shape:
