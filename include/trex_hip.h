@@ -306,6 +306,12 @@ int trex_mix_run(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, uint
 /* Persistent workgroups of later runs (0: two per compute unit). Fewer leave
  * compute units to kernels on other streams. */
 int trex_mix_set_grid(trex_mix* m, int64_t workgroups);
+/* Enqueues on `stream`, behind the last run, a copy of the run's first four
+ * queue words (ticket, error word, converged count, one flag) to dst:
+ * 16 bytes, 16-byte aligned, device or pinned host memory, copied by a kernel
+ * so the check never waits behind long copies on the copy engines. With no
+ * planned work nothing is enqueued. No reference counterpart. */
+int trex_mix_status(trex_mix* m, void* dst, void* stream);
 /* Error word of the last run (0: no wait expired). Synchronises `stream`. */
 int trex_mix_error(trex_mix* m, uint32_t* error, void* stream);
 /* Speculative NFA chunks of the last run whose fix-up met a checkpoint (the

@@ -95,6 +95,38 @@ def test_gpu_served_equals_alone(gpu_lib):
 
 
 @pytest.mark.gpu
+def test_gpu_expired_waits_fail_the_tick_not_the_server(gpu_lib):
+    """A spin bound of one makes the fix-up of a long NFA string give up
+    waiting for its speculative chunks (the kernel's own bounded wait: it sets
+    the error word and every wave still exits). The server reads the error
+    word behind the launch and completes that request with an error instead
+    of a wrong result; requests whose launches need no wait are served."""
+    from trex_amd.engine import NfaEngine
+    from trex_amd.serve import Request, Server
+    from trex_amd.workloads import random_nfa64
+
+    rng = np.random.default_rng(29)
+    engines = [NfaEngine.from_matrices(random_nfa64(rng))]
+    long_req = Request(0, rng.integers(0x40, 0x60, size=4 << 20, dtype=np.uint8), np.array([0, 4 << 20]))
+    small = []
+    for _ in range(4):   # short strings: walked whole, no waits
+        lengths = rng.integers(0, 1500, size=20)
+        offsets = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+        small.append(Request(0, rng.integers(0x40, 0x60, size=int(offsets[-1]), dtype=np.uint8), offsets))
+    server = Server(engines, hbm_budget=64 << 20, small_bytes=64 << 10, mix_max_spins=1)
+    with server:
+        for r in [long_req] + small:
+            server.submit(r)
+        for r in [long_req] + small:
+            assert r.wait(60), "request not completed"
+    assert long_req.result is None and "error word" in (long_req.error or "")
+    for r in small:
+        assert r.error is None
+        final, acc, _, _ = engines[0].match_arrays(r.data, r.offsets)
+        assert np.array_equal(r.result[0], final) and np.array_equal(r.result[1], acc)
+
+
+@pytest.mark.gpu
 def test_gpu_kernel_copy_pinned_and_device(gpu_lib):
     """trex_memcpy_kernel: pinned host -> device and device -> device copies of
     sizes around the 16-byte vector and the grid limit are exact; unaligned or

@@ -97,6 +97,7 @@ _SIGNATURES = {
     "trex_mix_info": (c_int, [c_void_p, i64p, i64p, i64p, i64p]),
     "trex_mix_layout": (c_int, [c_void_p] + [c_void_p] * 9),
     "trex_mix_run": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_uint32, c_void_p]),
+    "trex_mix_status": (c_int, [c_void_p, c_void_p, c_void_p]),
     "trex_mix_error": (c_int, [c_void_p, POINTER(ctypes.c_uint32), c_void_p]),
     "trex_mix_set_grid": (c_int, [c_void_p, c_int64]),
     "trex_mix_stats": (c_int, [c_void_p, POINTER(ctypes.c_uint32), c_void_p]),

@@ -506,6 +506,12 @@ int trex_mix_error(trex_mix* m, uint32_t* error, void* stream) {
     return TREX_OK;
 }
 
+int trex_mix_status(trex_mix* m, void* dst, void* stream) {
+    if (!m || !dst) return fail(TREX_ERR_INVALID, "null argument");
+    if (!m->has_device || m->plan.bfirst.empty()) return TREX_OK;   // nothing ran: dst untouched
+    return trex_memcpy_kernel(dst, m->d_words.ptr, 16, stream);
+}
+
 int trex_mix_stats(trex_mix* m, uint32_t* converged, void* stream) {
     if (!m || !converged) return fail(TREX_ERR_INVALID, "null argument");
     *converged = 0;

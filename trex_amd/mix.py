@@ -104,6 +104,11 @@ class MixEngine:
         """Persistent workgroups of later runs (0: two per compute unit)."""
         _lib.call("trex_mix_set_grid", self.handle, int(workgroups))
 
+    def status_to(self, dst: int, stream: Stream | None = None) -> None:
+        """Enqueue a copy of the last run's queue words (ticket, error word, converged,
+        flag; 16 bytes) to dst (device or pinned host memory) behind the run."""
+        _lib.call("trex_mix_status", self.handle, dst, stream.handle if stream else None)
+
     def error(self, stream: Stream | None = None) -> int:
         e = ctypes.c_uint32()
         _lib.call("trex_mix_error", self.handle, ctypes.byref(e), stream.handle if stream else None)

@@ -71,7 +71,9 @@ ALIGN = 256
 
 @dataclass(eq=False)
 class Request:
-    """One match request: ``data[offsets[i]:offsets[i+1]]`` is string i."""
+    """One match request: ``data[offsets[i]:offsets[i+1]]`` is string i. Once
+    ``wait()`` returns True, ``result`` holds the answer, or ``error`` says why
+    there is none (``result`` is then None)."""
     pattern: int
     data: np.ndarray
     offsets: np.ndarray
@@ -140,6 +142,7 @@ class _Slot:
     def __init__(self, engines_dfa_nfa, stream: Stream):
         self.stream = stream
         self.mix = MixEngine(engines_dfa_nfa) if engines_dfa_nfa else None
+        self.h_status = PinnedBuffer(16) if self.mix is not None else None   # queue words of the last run
         self.cap = 0
         self.d_begin = None
         self.h_mask = self.h_acc = None
@@ -167,7 +170,7 @@ class Server:
                  small_params: MixParams | None = None, bulk_params: MixParams | None = None,
                  bulk_grid: int | None = None, small_budget: int | None = None, prob_slots: int = 2,
                  bulk_slots: int = 1, prob_grid: int = 32, reserve_strings: int = 1 << 18,
-                 prob_chunk_min: int = 1024, prob_chunk_len: int = 256):
+                 prob_chunk_min: int = 1024, prob_chunk_len: int = 256, mix_max_spins: int = 0):
         hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4)   # HIP's default is 4
         if hwq < 8:
             warnings.warn(f"GPU_MAX_HW_QUEUES is {hwq}: the server's seven streams share hardware queues, so "
@@ -190,6 +193,7 @@ class Server:
         self.small_bytes = int(small_bytes)
         self.prob_chunk_min = int(prob_chunk_min)
         self.prob_chunk_len = int(prob_chunk_len)   # fixed, so a sequence's result does not depend on its neighbours
+        self.mix_max_spins = int(mix_max_spins)     # bound of every wait in a mixed launch (0: the library's)
         self.arena = Arena(hbm_budget)
         self.small_arena = Arena(small_budget if small_budget is not None else max(64 * self.small_bytes, 256 << 20))
         self.small_stage = PinnedBuffer(self.small_arena.size)   # host mirror of the small arena
@@ -342,15 +346,19 @@ class Server:
                         q.remove(r)
             if reqs:
                 stream = slot.stream
-                for r in reqs:
-                    if r._copied is None and r.nbytes:   # small: staged bytes, copied on this stream
-                        _lib.call("trex_memcpy_kernel", r._arena_obj.buf.ptr + r._arena,
-                                  self.small_stage.ptr + r._arena, r.nbytes, stream.handle)
-                if prob:
-                    self._launch_prob(slot, reqs, stream, self.prob_chunk_min if name == "prob_small" else 0)
-                else:
-                    self._launch_mix(slot, reqs, stream, lane["params"])
-                slot.event.record(stream)
+                try:
+                    for r in reqs:
+                        if r._copied is None and r.nbytes:   # small: staged bytes, copied on this stream
+                            _lib.call("trex_memcpy_kernel", r._arena_obj.buf.ptr + r._arena,
+                                      self.small_stage.ptr + r._arena, r.nbytes, stream.handle)
+                    if prob:
+                        self._launch_prob(slot, reqs, stream, self.prob_chunk_min if name == "prob_small" else 0)
+                    else:
+                        self._launch_mix(slot, reqs, stream, lane["params"])
+                    slot.event.record(stream)
+                except Exception as e:   # the tick's requests fail; the lane goes on
+                    self._fail(slot, reqs, f"launch failed: {e}")
+                    continue
                 slot.requests = reqs
                 slot.busy = True
                 now = time.perf_counter()
@@ -362,7 +370,11 @@ class Server:
             if inflight and (len(inflight) == len(lane["slots"]) and not lane["queue"]):
                 inflight[0].event.synchronize()   # every slot busy, nothing else to watch
             if inflight and inflight[0].event.query():
-                self._complete(name, inflight.popleft())
+                done = inflight.popleft()
+                try:
+                    self._complete(name, done)
+                except Exception as e:
+                    self._fail(done, done.requests, f"completion failed: {e}")
                 continue
             with self._lock:   # a tick or a copy is still running: look again shortly
                 lane["cv"].wait(0.0002)
@@ -375,8 +387,10 @@ class Server:
         n = len(begin)
         slot.reserve(n)
         slot.mix.plan(begin, end, pat, params)
+        slot.h_status.array[:] = 0
         if n:   # results written by the kernel into pinned host memory
-            slot.mix.run(base, slot.h_mask.ptr, slot.h_acc.ptr, stream)
+            slot.mix.run(base, slot.h_mask.ptr, slot.h_acc.ptr, stream, self.mix_max_spins)
+            slot.mix.status_to(slot.h_status.ptr, stream)
 
     def _launch_prob(self, slot: _Slot, reqs: list[Request], stream: Stream, chunk_min: int = 0) -> None:
         """One ragged launch per request; with chunk_min > 0, the request's sequences
@@ -423,6 +437,26 @@ class Server:
                 eng.loglik_ranges_device(base, slot.d_begin.ptr + 8 * k1, d_end + 8 * k1, m - n_long,
                                          slot.h_f64.ptr + 8 * k1, slot.h_f64.ptr + 8 * (n + k1), stream)
 
+    def _fail(self, slot: _Slot, reqs: list[Request], message: str) -> None:
+        """Completes `reqs` with `error` set and no result, once the slot's stream
+        has drained (its kernels may still read the requests' arena space)."""
+        try:
+            slot.stream.synchronize()
+        except Exception:
+            pass
+        now = time.perf_counter()
+        slot.requests = []
+        with self._lock:
+            self._release(reqs)
+            slot.busy = False
+            self.completed.extend(reqs)
+        for r in reqs:
+            r.result = None
+            r.error = message
+            r.launched = r.launched or now
+            r.done = now
+            r._event.set()
+
     def _complete(self, name: str, slot: _Slot) -> None:
         now = time.perf_counter()
         if name.startswith("prob"):
@@ -435,6 +469,10 @@ class Server:
                 lm[order] = f64[n + k0:n + k0 + m]
                 r.result = (ll, lm)
         else:
+            err = int(slot.h_status.array.view(np.uint32)[1])
+            if err:   # a bounded wait expired: the tick's results are not to be trusted
+                self._fail(slot, slot.requests, f"mixed launch reported error word {err:#x} (a bounded wait expired)")
+                return
             k = 0
             mask = slot.h_mask.array.view(np.uint64)
             acc = slot.h_acc.array
