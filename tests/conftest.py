@@ -7,6 +7,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# tests compile every pattern they construct: a table cache directory from the
+# environment would let one test's compiled tables stand in for another's
+os.environ.pop("TREX_AMD_TABLE_CACHE", None)
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) and the built HIP library")

@@ -152,3 +152,36 @@ def test_gpu_kernel_copy_pinned_and_device(gpu_lib):
     if pageable.ctypes.data % 16 == 0:
         with pytest.raises(TrexError):
             _lib.call("trex_memcpy_kernel", d.ptr, pageable.ctypes.data, 4096, stream.handle)
+
+
+@pytest.mark.gpu
+def test_gpu_restart_from_table_file(gpu_lib, tmp_path):
+    """A server restarted from its table file serves the same results as the
+    engines it was saved from (no pattern compiled at the restart)."""
+    from trex_amd.engine import DfaEngine, NfaEngine, ProbNfaEngine
+    from trex_amd.serve import Server
+    from trex_amd.tables import dfa_byte_table
+    from trex_amd.workloads import ALPHABET_32, dirichlet_pnfa, random_dfa_pattern, random_nfa64
+
+    rng = np.random.default_rng(23)
+    engines = [DfaEngine(dfa_byte_table(random_dfa_pattern(rng, ALPHABET_32)[1])),
+               NfaEngine.from_matrices(random_nfa64(rng)), ProbNfaEngine(*dirichlet_pnfa(rng, 32))]
+    first = Server(engines, hbm_budget=64 << 20, small_bytes=64 << 10)
+    first.save_tables(tmp_path / "serving.trxt")
+    # sequences <= 1000 symbols: every probabilistic one takes the sequential path
+    reqs = _requests(rng, len(engines), ["mix", "mix", "prob"], 12, 32, 256, 1.0, 1000)
+    with Server.from_tables(tmp_path / "serving.trxt", hbm_budget=64 << 20, small_bytes=64 << 10) as restarted:
+        assert [type(e) for e in restarted.engines] == [type(e) for e in engines]
+        for r in reqs:
+            restarted.submit(r)
+        for r in reqs:
+            assert r.wait(60) and r.error is None
+    for r in reqs:
+        e = engines[r.pattern]
+        if isinstance(e, ProbNfaEngine):
+            ll, lm = e.loglik_ragged(r.data, r.offsets)
+            assert np.array_equal(r.result[0], ll) and np.array_equal(r.result[1], lm)
+        else:
+            final, acc, _, _ = e.match_arrays(r.data, r.offsets)
+            assert np.array_equal(r.result[0].astype(final.dtype), final)
+            assert np.array_equal(r.result[1], acc)

@@ -1,0 +1,280 @@
+"""Compiled-table cache and table files: a cached pattern loads the same tables
+the compiler makes, those tables decide strings as the oracle does, and every
+malformed file is refused (a cache miss, never a wrong table)."""
+
+import threading
+
+import numpy as np
+import pytest
+
+import trex_amd.optimized_engine as oe
+from oracle.dfa import dfa_simulate, simulate_monoid
+from oracle.nfa import simulate_nfa
+from trex_amd.table_cache import (CompiledPattern, ProbModel, TableCache, TableFileError, _pack, _unpack,
+                                  default_cache, load_tables, pattern_key, save_tables)
+from trex_amd.workloads import dirichlet_pnfa
+
+PATTERNS = ["(a|b)*abb", "(b*ab*ab*)*b*", "[a-c]*abc", "(ab|cd|ef)*(x|y)(ab|cd)*z"]
+
+
+def _strings(rng, alpha="abcdefxyz", n=200, max_len=24):
+    return ["".join(alpha[i] for i in rng.integers(0, len(alpha), int(rng.integers(0, max_len))))
+            for _ in range(n)]
+
+
+def _walk_table(cp, s):
+    t = cp.table
+    q = t.start
+    for ch in s.encode("latin-1"):
+        q = int(t.delta[q, ch])
+    return bool(t.accept[q])
+
+
+def _walk_masks(cp, s):
+    m = cp.start_mask
+    for ch in s.encode("latin-1"):
+        c = int(cp.byte_to_sym[ch])
+        if c == -1:
+            return False
+        if c == -2:
+            continue
+        nxt = 0
+        for src in range(64):
+            if m >> src & 1:
+                nxt |= int(cp.succ[c, src])
+        m = nxt
+    return bool(m & cp.accept_mask)
+
+
+@pytest.mark.parametrize("config", oe.CONFIGS)
+@pytest.mark.parametrize("pattern", PATTERNS)
+def test_cache_hit_equals_fresh_compile(tmp_path, monkeypatch, pattern, config):
+    try:
+        fresh = oe.OptimizedEngine(pattern, config=config, cache=False)
+    except RuntimeError:
+        pytest.skip("configuration refuses this pattern")
+    first = oe.OptimizedEngine(pattern, config=config, cache=tmp_path)
+    assert first.compiled.same_tables(fresh.compiled)
+
+    def refuse(*a, **k):
+        raise AssertionError("compiled on a cache hit")
+    monkeypatch.setattr(oe, "compile_regex", refuse)
+    monkeypatch.setattr(oe, "compile_nfa_matrices", refuse)
+    cache = TableCache(tmp_path)                # a new process: empty memory, same directory
+    hit = oe.OptimizedEngine(pattern, config=config, cache=cache)
+    assert cache.hits == 1 and cache.misses == 0
+    assert hit.compiled.same_tables(fresh.compiled)
+    assert hit.config_info == fresh.config_info
+
+
+@pytest.mark.parametrize("config", oe.CONFIGS)
+@pytest.mark.parametrize("pattern", PATTERNS)
+def test_cached_tables_decide_as_the_oracle(tmp_path, pattern, config):
+    try:
+        fresh = oe.OptimizedEngine(pattern, config=config, cache=False)
+    except RuntimeError:
+        pytest.skip("configuration refuses this pattern")
+    oe.OptimizedEngine(pattern, config=config, cache=tmp_path)
+    cp = oe.OptimizedEngine(pattern, config=config, cache=TableCache(tmp_path)).compiled
+    rng = np.random.default_rng(len(pattern) * 31 + oe.CONFIGS.index(config))
+    strings = _strings(rng)
+    if config in ("monoid", "monoid+kgram", "monoid+gpu", "monoid_batch+gpu"):
+        want = [simulate_monoid(fresh._md, s) for s in strings]
+    elif config == "prefix+gpu":
+        dfa = fresh._dfa
+        want = [dfa_simulate(dfa, "".join(c for c in s if c in dfa.alphabet)) for s in strings]
+    elif cp.representation == "nfa":
+        want = [simulate_nfa(fresh._nm, s) for s in strings]
+    else:
+        want = [dfa_simulate(fresh._dfa, s) for s in strings]
+    walk = _walk_masks if cp.representation == "nfa" else _walk_table
+    assert [walk(cp, s) for s in strings] == want
+
+
+def test_memory_cache_and_key_fields(tmp_path):
+    cache = TableCache(None)
+    a = oe.OptimizedEngine("(a|b)*abb", cache=cache)
+    b = oe.OptimizedEngine("(a|b)*abb", cache=cache)
+    assert cache.hits == 1 and b.compiled is a.compiled
+    # every option that changes the compiled table is part of the key
+    c = oe.OptimizedEngine("(a|b)*abb", dfa_state_cap=2, cache=cache)
+    assert c.compiled.representation == "nfa" and cache.misses == 2
+    assert pattern_key("x", None, 64, 65536) != pattern_key("x", None, 64, 1024)
+    assert pattern_key("x", None, 64, 65536) != pattern_key("x", "nfa", 64, 65536)
+
+
+def test_default_cache_from_environment(tmp_path, monkeypatch):
+    monkeypatch.delenv("TREX_AMD_TABLE_CACHE", raising=False)
+    assert default_cache() is None
+    monkeypatch.setenv("TREX_AMD_TABLE_CACHE", str(tmp_path))
+    cache = default_cache()
+    assert cache is not None and cache.root == tmp_path
+    oe.OptimizedEngine("[a-c]*abc")
+    assert len(list(tmp_path.glob("*.trxt"))) == 1
+    oe.OptimizedEngine("[a-c]*abc", cache=False)   # False bypasses the cache
+    assert cache.hits == 0
+    oe.OptimizedEngine("[a-c]*abc")
+    assert cache.hits == 1
+
+
+def _records():
+    rng = np.random.default_rng(5)
+    T, s, a = dirichlet_pnfa(rng, 8)
+    return [oe.OptimizedEngine("(a|b)*abb", cache=False).compiled,
+            oe.OptimizedEngine("(ab|cd|ef)*(x|y)(ab|cd)*z", config="nfa", cache=False).compiled,
+            oe.OptimizedEngine("[a-c]*abc", config="prefix+gpu", cache=False).compiled,
+            ProbModel(T, s, a, mode="bf16x2")]
+
+
+def test_table_file_round_trip(tmp_path):
+    recs = _records()
+    save_tables(tmp_path / "set.trxt", recs)
+    back = load_tables(tmp_path / "set.trxt")
+    assert len(back) == len(recs)
+    for r, b in zip(recs, back):
+        if isinstance(r, CompiledPattern):
+            assert b.same_tables(r)
+            assert (b.scan_backend, b.kgram_k, b.info) == (r.scan_backend, r.kgram_k, r.info)
+        else:
+            assert np.array_equal(b.T, r.T) and np.array_equal(b.start, r.start)
+            assert np.array_equal(b.accept, r.accept) and b.mode == r.mode
+    # an empty set is a valid file
+    save_tables(tmp_path / "empty.trxt", [])
+    assert load_tables(tmp_path / "empty.trxt") == []
+
+
+def _rewrite(src, dst, edit):
+    meta, arrays = _unpack(src.read_bytes())
+    arrays = {k: np.array(v) for k, v in arrays.items()}     # writable copies
+    edit(meta, arrays)
+    dst.write_bytes(_pack(meta, arrays))
+
+
+def _set_row(meta, arrays, name, index, col, value):
+    a = arrays[name]
+    a[index, col] = value
+
+
+BAD_CASES = {
+    "truncated": lambda good: good.read_bytes()[:300],
+    "magic": lambda good: b"TREXTBX\0" + good.read_bytes()[8:],
+    "version": lambda good: good.read_bytes()[:8] + np.array([99], "<u4").tobytes() + good.read_bytes()[12:],
+    "header_length": lambda good: good.read_bytes()[:12] + np.array([1 << 30], "<u4").tobytes() + good.read_bytes()[16:],
+    "header_json": lambda good: good.read_bytes()[:16] + b"x" + good.read_bytes()[17:],
+    "dtype": lambda good: good.read_bytes().replace(b'"|u1"', b'"|O8"', 1),
+}
+
+
+@pytest.mark.parametrize("case", ["truncated", "magic", "version", "header_length", "header_json", "dtype",
+                                  "delta_range", "delta_shape", "rows", "dead", "nfa_symbol", "prob_digest",
+                                  "missing", "kind"])
+def test_bad_files_are_refused(tmp_path, case):
+    good = tmp_path / "good.trxt"
+    save_tables(good, _records())
+    assert len(load_tables(good)) == 4
+    bad = tmp_path / "bad.trxt"
+    if case in BAD_CASES:
+        bad.write_bytes(BAD_CASES[case](good))
+    elif case == "delta_range":
+        n0 = _records()[0].table.n_states
+        _rewrite(good, bad, lambda m, a: _set_row(m, a, "dfa.delta", 0, 5, n0))   # a state past record 0's table
+    elif case == "delta_shape":
+        _rewrite(good, bad, lambda m, a: a.__setitem__("dfa.delta", a["dfa.delta"][:, :128].copy()))
+    elif case == "rows":
+        _rewrite(good, bad, lambda m, a: m["records"][2].__setitem__("row", 10 ** 6))
+    elif case == "dead":
+        _rewrite(good, bad, lambda m, a: m["records"][0].__setitem__("dead", 200))
+    elif case == "nfa_symbol":
+        _rewrite(good, bad, lambda m, a: _set_row(m, a, "nfa.byte_to_sym", 0, ord("q"), 63))
+    elif case == "prob_digest":
+        def edit(m, a):
+            a["pnfa.T"][0, 0, 0] += 0.25
+        _rewrite(good, bad, edit)
+    elif case == "missing":
+        _rewrite(good, bad, lambda m, a: a.pop("dfa.accept"))
+    elif case == "kind":
+        _rewrite(good, bad, lambda m, a: m["records"][1].__setitem__("kind", "script"))
+    with pytest.raises(TableFileError):
+        load_tables(bad)
+
+
+def test_cache_treats_bad_files_and_collisions_as_misses(tmp_path):
+    cache = TableCache(tmp_path, memory=False)
+    key = pattern_key("(a|b)*abb", None, 64, 65536)
+    fresh = oe.OptimizedEngine("(a|b)*abb", cache=cache).compiled
+    path = cache.path_of(key)
+    assert path.exists()
+    # a corrupt file: miss, recompiled, rewritten
+    path.write_bytes(b"garbage")
+    again = oe.OptimizedEngine("(a|b)*abb", cache=cache).compiled
+    assert cache.bad_files == 1 and again.same_tables(fresh)
+    assert load_tables(path)[0].same_tables(fresh)
+    # a file stored under another key at this path (a hash collision): miss
+    other = oe.OptimizedEngine("[a-c]*abc", cache=False).compiled
+    other.key = pattern_key("[a-c]*abc", None, 64, 65536)
+    save_tables(path, [other])
+    misses = cache.misses
+    got = oe.OptimizedEngine("(a|b)*abb", cache=cache).compiled
+    assert cache.misses == misses + 1 and got.same_tables(fresh)
+
+
+def test_concurrent_writers_leave_a_valid_file(tmp_path):
+    cache = TableCache(tmp_path, memory=False)
+    errors = []
+
+    def work():
+        try:
+            for _ in range(20):
+                oe.OptimizedEngine("(ab|cd|ef)*(x|y)(ab|cd)*z", config="monoid+gpu", cache=cache)
+        except Exception as e:      # pragma: no cover - reported below
+            errors.append(e)
+    threads = [threading.Thread(target=work) for _ in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors
+    files = list(tmp_path.iterdir())
+    assert len(files) == 1, files   # no temporary files left behind
+    fresh = oe.OptimizedEngine("(ab|cd|ef)*(x|y)(ab|cd)*z", config="monoid+gpu", cache=False).compiled
+    assert load_tables(files[0])[0].same_tables(fresh)
+
+
+# ---------------------------------------------------------------------------
+# GPU: engines restored from a table file match as the originals do
+# ---------------------------------------------------------------------------
+@pytest.mark.gpu
+def test_gpu_restored_engines_match_originals(gpu_lib, tmp_path):
+    from trex_amd.engine import DfaEngine, NfaEngine, ProbNfaEngine, pack_strings
+    from trex_amd.table_cache import engines_of
+    recs = _records()
+    originals = engines_of(recs)
+    save_tables(tmp_path / "serving.trxt", originals)
+    restored = engines_of(load_tables(tmp_path / "serving.trxt"))
+    rng = np.random.default_rng(9)
+    data, offsets = pack_strings(_strings(rng, n=500, max_len=200))
+    sym = (data % 8).astype(np.uint8)
+    for a, b in zip(originals, restored):
+        assert type(a) is type(b)
+        if isinstance(a, (DfaEngine, NfaEngine)):
+            fa, aa = a.match_arrays(data, offsets)[:2]
+            fb, ab = b.match_arrays(data, offsets)[:2]
+            assert np.array_equal(fa, fb) and np.array_equal(aa, ab)
+        else:
+            assert isinstance(a, ProbNfaEngine) and b.info()["mode"] == "bf16x2"
+            la, ma = a.loglik_ragged(sym, offsets)
+            lb, mb = b.loglik_ragged(sym, offsets)
+            assert np.array_equal(la, lb) and np.array_equal(ma, mb)
+    for e in originals + restored:
+        e.close()
+
+
+@pytest.mark.gpu
+def test_gpu_cache_hit_engine_matches_fresh(gpu_lib, tmp_path):
+    rng = np.random.default_rng(3)
+    strings = _strings(rng, n=400, max_len=60)
+    for config in (None, "nfa", "monoid_batch+gpu", "prefix+gpu", "baseline"):
+        fresh = oe.OptimizedEngine("(ab|cd|ef)*(x|y)(ab|cd)*z", config=config, cache=False)
+        oe.OptimizedEngine("(ab|cd|ef)*(x|y)(ab|cd)*z", config=config, cache=tmp_path)
+        hit = oe.OptimizedEngine("(ab|cd|ef)*(x|y)(ab|cd)*z", config=config, cache=TableCache(tmp_path))
+        assert hit.match(strings) == fresh.match(strings)

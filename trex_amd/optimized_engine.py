@@ -22,6 +22,12 @@ outside the pattern's alphabet:
 
 GPU engines are created on first use, so construction and ``config_info``
 work on a host without a GPU.
+
+The compiled result (byte table or NFA successor masks plus the selection
+fields) is a :class:`trex_amd.table_cache.CompiledPattern`. With a table
+cache (the ``cache`` argument, or the directory named by
+``$TREX_AMD_TABLE_CACHE``) a pattern compiled once under the same options is
+loaded instead of compiled again.
 """
 
 from __future__ import annotations
@@ -36,6 +42,7 @@ from trex_amd.monoid import compute_monoid, monoid_byte_table
 from trex_amd.nfa_matrices import compile_nfa_matrices
 from trex_amd.regex_to_dfa import compile_regex
 from trex_amd.simulation import DFAMatrices, precompute_tmap
+from trex_amd.table_cache import CompiledPattern, TableCache, default_cache, pattern_key
 from trex_amd.tables import MAX_TABLE_STATES, dfa_byte_table, tmap_byte_table
 
 CONFIGS = (None, "monoid", "monoid+kgram", "baseline", "nfa", "monoid+gpu", "batched+gpu", "kgram+gpu",
@@ -48,7 +55,9 @@ CHUNK_LEN = 1 << 16
 
 class OptimizedEngine:
     def __init__(self, regex: str, config: Optional[str] = None, dfa_state_cap: int = 64,
-                 monoid_cap: int = 65536):
+                 monoid_cap: int = 65536, cache=None):
+        """``cache``: a :class:`TableCache`, a directory for one, ``False`` for
+        none, or ``None`` for the default (``$TREX_AMD_TABLE_CACHE`` if set)."""
         if config not in CONFIGS:
             raise ValueError(f"Unknown config: {config!r}. Choose from None, " +
                              ", ".join(repr(c) for c in CONFIGS[1:]) + ".")
@@ -66,6 +75,27 @@ class OptimizedEngine:
         self._scan_backend = None
         self._selection_reason = None
         self._kgram_k = None
+        if cache is None:
+            cache = default_cache()
+        elif cache is False:
+            cache = None
+        elif not isinstance(cache, TableCache):
+            cache = TableCache(cache)
+        key = pattern_key(regex, config, dfa_state_cap, monoid_cap)
+        cp = cache.get(key) if cache is not None else None
+        if cp is None:
+            self._select(config)
+            cp = self._record()
+            if cache is not None:
+                cache.put(key, cp)
+        self._compiled = cp
+        self._table = cp.table
+        self._representation = cp.representation
+        self._scan_backend = cp.scan_backend
+        self._selection_reason = cp.selection_reason
+        self._kgram_k = cp.kgram_k
+
+    def _select(self, config):
         {
             None: self._auto_select,
             "monoid": lambda: self._select_monoid("monoid", "forced config='monoid'"),
@@ -161,11 +191,9 @@ class OptimizedEngine:
             self._selection_reason = (f"DFA has {n_states} states ({table.n_states} with its dead state) "
                                       f"> 16; auto-selected dfa+gpu (byte table walk)")
 
-    # -- public API -------------------------------------------------------------
-    @property
-    def config_info(self) -> dict:
-        info = {"representation": self._representation, "scan_backend": self._scan_backend,
-                "kgram_k": self._kgram_k, "selection_reason": self._selection_reason}
+    def _record(self) -> CompiledPattern:
+        """The selection's compiled form (what the table cache stores)."""
+        info = {}
         if self._dfa is not None:
             info["alphabet_size"] = len(self._dfa.alphabet)
             info["dfa_states"] = self._dfa.n_states
@@ -176,17 +204,31 @@ class OptimizedEngine:
             info["monoid_size"] = self._md.size
         if self._nm is not None and "nfa_states" not in info:
             info["nfa_states"] = self._nm.n_states_raw
+        common = dict(representation=self._representation, scan_backend=self._scan_backend,
+                      selection_reason=self._selection_reason, kgram_k=self._kgram_k, info=info)
+        if self._representation == "nfa":
+            from trex_amd.engine import nfa_byte_map
+            nm = self._nm
+            return CompiledPattern(succ=nm.successor_masks(), byte_to_sym=nfa_byte_map(nm),
+                                   start_mask=nm.start_mask(), accept_mask=nm.accept_bits(), **common)
+        return CompiledPattern(table=self._table, **common)
+
+    # -- public API -------------------------------------------------------------
+    @property
+    def compiled(self) -> CompiledPattern:
+        return self._compiled
+
+    @property
+    def config_info(self) -> dict:
+        cp = self._compiled
+        info = {"representation": cp.representation, "scan_backend": cp.scan_backend,
+                "kgram_k": cp.kgram_k, "selection_reason": cp.selection_reason}
+        info.update(cp.info)
         return info
 
     def _gpu(self):
         if self._engine is None:
-            from trex_amd.engine import DfaEngine, NfaEngine
-            if self._representation == "nfa":
-                self._engine = NfaEngine.from_matrices(self._nm)
-            else:
-                self._engine = DfaEngine(self._table)
-                if self._scan_backend == "sequential":
-                    self._engine.set_variant("k1_generic" if self._table.n_states > 16 else "k1_16")
+            self._engine = self._compiled.engine()
         return self._engine
 
     def match_arrays(self, data: np.ndarray, offsets: np.ndarray):

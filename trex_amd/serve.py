@@ -228,6 +228,20 @@ class Server:
         self.completed: list[Request] = []
         self.ticks = collections.Counter()
 
+    # -- resident tables ---------------------------------------------------------------
+    def save_tables(self, path) -> None:
+        """Write every pattern's tables (pattern index = record index) to one
+        table file, so a restarted server loads them instead of compiling."""
+        from trex_amd.table_cache import save_tables
+        save_tables(path, self.engines)
+
+    @classmethod
+    def from_tables(cls, path, **kwargs) -> "Server":
+        """A server over the patterns of a table file (written by
+        :meth:`save_tables` or :func:`trex_amd.table_cache.save_tables`)."""
+        from trex_amd.table_cache import engines_of, load_tables
+        return cls(engines_of(load_tables(path)), **kwargs)
+
     # -- client side ---------------------------------------------------------------
     def submit(self, req: Request) -> Request:
         """Admits the request at once when the arena has room (its bytes are
