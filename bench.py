@@ -19,6 +19,10 @@ launch stream. The results of the last step are checked bit-exact against the
 multithreaded C oracle over the whole batch; the oracle's time on that check
 is the CPU baseline.
 
+Beside the headline, "multi_pattern" times the same 8 patterns over the same
+batch in one launch of the multi-pattern kernel (one read of the bytes
+advances all 8), checked against the same oracle results.
+
 Secondary legs (reported under "secondary"):
   * nfa64: one Boolean NFA-64 (64 states, each per-symbol edge present with
     p = 0.06, 32 symbols, state 0 start, 4 accept states) over the same 1 GiB
@@ -130,6 +134,8 @@ def main() -> int:
     ap.add_argument("--serve-prob-cap", type=int, default=1 << 16,
                     help="longest probabilistic sequence in serving requests (each symbol is a sequential step)")
     ap.add_argument("--sweep", action="store_true", help="time every kernel variant and layout, then exit")
+    ap.add_argument("--multi-sweep", action="store_true",
+                    help="also time the multi-pattern pass for every table form and patterns-per-launch")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also time a plain 1 GiB device copy (HBM traffic counter calibration)")
     args = ap.parse_args()
@@ -263,6 +269,8 @@ def dfa_leg(args, data, offsets, batch, stream, result: dict, copy_gbs) -> int:
     log(f"dfa step {ms_per_step:.3f} ms (wall {wall / args.steps * 1e3:.3f} ms), kernel avg {avg_kernel_ms:.4f} ms "
         f"min {min(kernel_ms):.4f} max {max(kernel_ms):.4f}, {gchars:.1f} Gchars/s, {achieved:.0f} GB/s")
 
+    multi_res, multi_outs = multi_leg(args, engines, batch, stream, ms_per_step, data.nbytes)
+
     cpu = None
     verified = None
     if not args.no_verify:
@@ -280,6 +288,14 @@ def dfa_leg(args, data, offsets, batch, stream, result: dict, copy_gbs) -> int:
             same = bool(np.array_equal(gf, rf) and np.array_equal(ga, ra))
             ok &= same
             log(f"pattern {i}: bit-exact={same} accept-rate={float(ra.mean()):.4f}")
+            if multi_outs is not None:
+                mf = multi_outs[0].download(np.uint8, n, offset=i * n)
+                ma = multi_outs[1].download(np.uint8, n, offset=i * n)
+                m_same = bool(np.array_equal(mf, rf) and np.array_equal(ma, ra))
+                multi_res["verified_bit_exact"] = multi_res.get("verified_bit_exact", True) and m_same
+                ok &= m_same
+                if not m_same:
+                    log(f"pattern {i}: multi-pattern pass differs from the oracle")
         verified = ok
         # pure-Python restatement of DFA.simulate on a 256-string sample (1 MiB)
         sample = [bytes(data[offsets[j]:offsets[j + 1]]).decode("latin-1") for j in range(256)]
@@ -294,7 +310,7 @@ def dfa_leg(args, data, offsets, batch, stream, result: dict, copy_gbs) -> int:
                "python_dfa_simulate_mchars_s": round(len(sample) * LENGTH / t_py / 1e6, 2)}
         if not ok:
             log("PARITY FAILURE (dfa)")
-    for a, b in outs:
+    for a, b in outs + ([multi_outs] if multi_outs is not None else []):
         a.free()
         b.free()
 
@@ -325,8 +341,65 @@ def dfa_leg(args, data, offsets, batch, stream, result: dict, copy_gbs) -> int:
                      "alg_bytes_per_launch": alg_bytes, "copy_probe_gbs": copy_gbs},
         "cpu_baseline": cpu,
         "verified_bit_exact": verified,
+        "multi_pattern": multi_res,
     })
     return 0 if verified in (None, True) else 1
+
+
+def multi_leg(args, engines, batch, stream, per_pattern_ms: float, nbytes: int):
+    """The same 8 patterns over the same batch in one pass (trex_dfa_multi):
+    one read of the bytes advances all patterns. Returns its summary and its
+    output buffers (checked against the oracle with the per-pattern ones)."""
+    from trex_amd.device import DeviceBuffer, Event
+    from trex_amd.engine import DfaMultiEngine
+
+    if any(e.info()["window_base"] < 0 for e in engines):
+        log("multi-pattern pass skipped: a pattern has no 32-byte window")
+        return None, None
+    n = batch.n_strings
+    P = len(engines)
+    multi = DfaMultiEngine(engines)
+    d_f, d_a = DeviceBuffer(P * n), DeviceBuffer(P * n)
+
+    def timed(reps: int) -> list:
+        launch = lambda: multi.match_batch_device(batch, d_f.ptr, d_a.ptr, stream)
+        for _ in range(args.warmup):
+            launch()
+        evs = [(Event(), Event()) for _ in range(reps)]
+        for a, b in evs:
+            a.record(stream)
+            launch()
+            b.record(stream)
+        stream.synchronize()
+        return [a.elapsed_ms(b) for a, b in evs]
+
+    if args.multi_sweep:
+        for form in ("k2f", "k2w"):
+            multi.set_form(form)
+            for group in (1, 2, 3, 4, 5, 6, 7, 8):
+                multi.set_group(group)
+                ms = float(np.median(timed(max(3, args.steps))))
+                log(f"multi sweep: form {form} group {group}: {ms:.4f} ms per 8 patterns, "
+                    f"{P * nbytes / (ms * 1e-3) / 1e9:.0f} Gchars/s, {per_pattern_ms / ms:.2f}x per-pattern launches")
+        multi.set_form("k2f")
+        multi.set_group(4)
+    ms = timed(max(3, args.steps))
+    avg = float(np.mean(ms))
+    info = multi.info()
+    alg_bytes = nbytes + 16 * n + 2 * P * n
+    launches = -(-P // info["group"])
+    per_launch = -(-P // launches)
+    res = {"kernel": f"dfa16_multi_{info['form']}_p{per_launch}{'_il' if batch.layout == 'interleaved' else ''}",
+           "patterns_per_launch": per_launch, "launches_per_step": launches,
+           "ms_per_step": round(avg, 4), "ms_min": round(min(ms), 4), "ms_max": round(max(ms), 4),
+           "gchars_s": round(P * nbytes / (avg * 1e-3) / 1e9, 1),
+           "speedup_vs_per_pattern_launches": round(per_pattern_ms / avg, 2),
+           "hbm_gbs": round(alg_bytes / (avg * 1e-3) / 1e9, 1),
+           "hbm_frac": round(alg_bytes / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"multi-pattern pass ({res['kernel']}): {avg:.4f} ms for {P} patterns = {res['gchars_s']} Gchars/s, "
+        f"{res['speedup_vs_per_pattern_launches']}x the per-pattern launches, {res['hbm_gbs']} GB/s of HBM")
+    multi.close()
+    return res, (d_f, d_a)
 
 
 def nfa_leg(args, data, offsets, batch, stream) -> dict:

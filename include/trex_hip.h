@@ -150,6 +150,43 @@ int trex_dfa_match_host(trex_dfa* h, const uint8_t* bytes, const int64_t* offset
                         uint8_t* final_state, uint8_t* accept, float* kernel_ms, float* total_ms);
 
 /* ------------------------------------------------------------------------
+ * Several DFA-16 patterns in one pass over the bytes. The reference packs
+ * several DFAs block-diagonally into one state vector and evolves it through
+ * its batched matrix kernels (PackedEngine, src/packed_engine.py:25-126,
+ * batched_engine_dispatch_multi); these entry points replace that with
+ * per-pattern pair tables in LDS. Each launch reads the batch once and advances up to `group`
+ * patterns, so matching P patterns reads the bytes ceil(P / group) times
+ * instead of P.
+ * ------------------------------------------------------------------------ */
+typedef struct trex_dfa_multi trex_dfa_multi;
+
+/* patterns: n_patterns (1..256) handles of DFA-16 tables whose live bytes lie
+ * in one aligned 32-byte window each (trex_dfa_info reports window_base >= 0;
+ * the windows may differ). The tables are copied: the handles may be
+ * destroyed afterwards. Any other table is refused (TREX_ERR_INVALID, naming
+ * the pattern). */
+int trex_dfa_multi_create(trex_dfa* const* patterns, int n_patterns, trex_dfa_multi** out);
+int trex_dfa_multi_destroy(trex_dfa_multi* m);
+/* Table form: TREX_DFA_K2_WINDOW_F (default, 8 KB of LDS per pattern) or
+ * TREX_DFA_K2_WINDOW (16 KB per pattern). */
+int trex_dfa_multi_set_form(trex_dfa_multi* m, int form);
+/* Patterns advanced per launch at most, 1..8 (default 4); the patterns are
+ * split evenly over ceil(n_patterns / group) launches. */
+int trex_dfa_multi_set_group(trex_dfa_multi* m, int group);
+int trex_dfa_multi_info(const trex_dfa_multi* m, int* n_patterns, int* form, int* group);
+/* Final state and accept bit of every (pattern, string), pattern-major:
+ * d_final_state[p * n_strings + i], d_accept[p * n_strings + i] (each
+ * n_patterns * n_strings bytes). Strings as in trex_dfa_match_ranges /
+ * trex_dfa_match_interleaved; results equal those of each pattern's own
+ * handle. Asynchronous on `stream`. */
+int trex_dfa_multi_match_ranges(trex_dfa_multi* m, const uint8_t* d_bytes, const int64_t* d_begin,
+                                const int64_t* d_end, int64_t n_strings, uint8_t* d_final_state, uint8_t* d_accept,
+                                void* stream);
+int trex_dfa_multi_match_interleaved(trex_dfa_multi* m, const uint8_t* d_bytes, const int64_t* d_begin,
+                                     const int64_t* d_end, int64_t n_strings, uint8_t* d_final_state,
+                                     uint8_t* d_accept, void* stream);
+
+/* ------------------------------------------------------------------------
  * Boolean NFA-64, bit-parallel (hot path (c)).
  * Replaces nfa_tc_engine_init/dispatch with the V4 bit-parallel kernel
  * (cuda/nfa_tc_evolution.cu:683-745, 943-1013), which handles two symbols

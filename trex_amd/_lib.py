@@ -61,6 +61,15 @@ _SIGNATURES = {
     "trex_dfa_set_prefetch": (c_int, [c_void_p, c_int]),
     "trex_dfa_match_ranges": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
                                       c_void_p]),
+    "trex_dfa_multi_create": (c_int, [c_void_p, c_int, vpp]),
+    "trex_dfa_multi_destroy": (c_int, [c_void_p]),
+    "trex_dfa_multi_set_form": (c_int, [c_void_p, c_int]),
+    "trex_dfa_multi_set_group": (c_int, [c_void_p, c_int]),
+    "trex_dfa_multi_info": (c_int, [c_void_p, intp, intp, intp]),
+    "trex_dfa_multi_match_ranges": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
+                                            c_void_p]),
+    "trex_dfa_multi_match_interleaved": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
+                                                 c_void_p, c_void_p]),
     "trex_dfa_match_interleaved": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
                                       c_void_p]),
     # NFA

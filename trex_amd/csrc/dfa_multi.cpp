@@ -1,0 +1,192 @@
+// Multi-pattern DFA-16 handle: several resident DFA-16 tables matched in one
+// pass over the bytes (kernels in dfa_multi_kernels.hip).
+//
+// The handle copies each member's pair tables (device to device) into one
+// pattern-major array and derives the single-symbol tables from the member's
+// byte table, so it does not depend on the member handles after creation.
+// A dispatch launches ceil(n_patterns / group) kernels, the patterns split
+// as evenly as possible among them; each reads the batch once and advances
+// its patterns. The default group of 4 is the measured optimum on gfx950:
+// the kernels are bound by LDS bank conflicts of the table reads, and four
+// patterns (33 KB of tables) still leave room for four workgroups per CU.
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <vector>
+
+#include "runtime_internal.h"
+#include "trex_hip.h"
+
+using namespace trex;
+
+namespace {
+
+constexpr int kMultiMax = 8;          // patterns per launch (kernels exist for 1..8)
+constexpr int kMultiGroup = 4;        // default patterns per launch
+constexpr int kMultiPatterns = 256;   // patterns per handle
+
+// Kernel argument block; the layout of MultiParams in dfa_multi_kernels.hip.
+struct MultiParams {
+    uint32_t win4[kMultiMax];
+    uint32_t start[kMultiMax];
+    uint32_t dead[kMultiMax];
+    uint32_t accept[kMultiMax];
+};
+
+}  // namespace
+
+struct trex_dfa_multi {
+    int n = 0;
+    int form = TREX_DFA_K2_WINDOW_F;
+    int group = kMultiGroup;
+    uint8_t* d_f2 = nullptr;   // [n][1024] uint64 pair functions
+    uint8_t* d_f1 = nullptr;   // [n][32] uint64 single-symbol functions
+    uint8_t* d_t2 = nullptr;   // [n][1024][16] pair walk
+    uint8_t* d_t1 = nullptr;   // [n][32][16] single-symbol walk
+    std::vector<uint32_t> win4, start, dead, accept;
+};
+
+extern "C" {
+
+int trex_dfa_multi_destroy(trex_dfa_multi* m) {
+    if (!m) return TREX_OK;
+    hipFree(m->d_f2);
+    hipFree(m->d_f1);
+    hipFree(m->d_t2);
+    hipFree(m->d_t1);
+    delete m;
+    return TREX_OK;
+}
+
+int trex_dfa_multi_create(trex_dfa* const* patterns, int n_patterns, trex_dfa_multi** out) {
+    if (!patterns || !out) return fail(TREX_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (n_patterns < 1 || n_patterns > kMultiPatterns)
+        return fail(TREX_ERR_INVALID, "n_patterns=%d outside 1..%d", n_patterns, kMultiPatterns);
+    for (int i = 0; i < n_patterns; ++i) {
+        const trex_dfa* h = patterns[i];
+        if (!h) return fail(TREX_ERR_INVALID, "pattern %d: null handle", i);
+        if (h->n_states > 16 || h->window_base < 0 || h->dead < 0 || !h->d_f2 || !h->d_t2 || h->stride != 16)
+            return fail(TREX_ERR_INVALID,
+                        "pattern %d: not a DFA-16 whose live bytes lie in one 32-byte window "
+                        "(n_states=%d, window=%d); match it with its own handle", i, h->n_states, h->window_base);
+    }
+    TREX_TRY(ensure_module());
+    trex_dfa_multi* m = new trex_dfa_multi();
+    m->n = n_patterns;
+    const size_t n = (size_t)n_patterns;
+    int rc = TREX_OK;
+    hipError_t e = hipMalloc((void**)&m->d_f2, n * 1024 * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&m->d_t2, n * 1024 * 16);
+    if (e != hipSuccess) rc = hip_fail(e, "hipMalloc(multi-pattern tables)");
+    std::vector<uint64_t> f1(n * 32, 0);
+    std::vector<uint8_t> t1w(n * 32 * 16, 0), t1(256 * 16), acc(16);
+    for (int i = 0; i < n_patterns && !rc; ++i) {
+        const trex_dfa* h = patterns[i];
+        e = hipMemcpy(m->d_f2 + (size_t)i * 1024 * 8, h->d_f2, 1024 * 8, hipMemcpyDeviceToDevice);
+        if (e == hipSuccess) e = hipMemcpy(m->d_t2 + (size_t)i * 1024 * 16, h->d_t2, 1024 * 16, hipMemcpyDeviceToDevice);
+        if (e == hipSuccess) e = hipMemcpy(t1.data(), h->d_t1, t1.size(), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(acc.data(), h->d_acc, acc.size(), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { rc = hip_fail(e, "hipMemcpy(multi-pattern tables)"); break; }
+        const int base = h->window_base;
+        for (int c = 0; c < 32; ++c)
+            for (int s = 0; s < 16; ++s) {
+                const uint8_t nxt = t1[(size_t)(base + c) * 16 + s];   // identity beyond n_states, as in t1
+                t1w[((size_t)i * 32 + c) * 16 + s] = nxt;
+                f1[(size_t)i * 32 + c] |= (uint64_t)nxt << (4 * s);
+            }
+        uint32_t mask = 0;
+        for (int s = 0; s < 16; ++s) mask |= (acc[s] ? 1u : 0u) << s;
+        m->win4.push_back((uint32_t)base * 0x01010101u);
+        m->start.push_back((uint32_t)h->start);
+        m->dead.push_back((uint32_t)h->dead);
+        m->accept.push_back(mask);
+    }
+    if (!rc) rc = upload((void**)&m->d_f1, f1.data(), f1.size() * sizeof(uint64_t));
+    if (!rc) rc = upload((void**)&m->d_t1, t1w.data(), t1w.size());
+    if (rc) { trex_dfa_multi_destroy(m); return rc; }
+    *out = m;
+    return TREX_OK;
+}
+
+int trex_dfa_multi_set_form(trex_dfa_multi* m, int form) {
+    if (!m) return fail(TREX_ERR_INVALID, "null handle");
+    if (form == TREX_DFA_AUTO) form = TREX_DFA_K2_WINDOW_F;
+    if (form != TREX_DFA_K2_WINDOW_F && form != TREX_DFA_K2_WINDOW)
+        return fail(TREX_ERR_INVALID, "form %d: use TREX_DFA_K2_WINDOW_F or TREX_DFA_K2_WINDOW", form);
+    m->form = form;
+    return TREX_OK;
+}
+
+int trex_dfa_multi_set_group(trex_dfa_multi* m, int group) {
+    if (!m) return fail(TREX_ERR_INVALID, "null handle");
+    if (group < 1 || group > kMultiMax) return fail(TREX_ERR_INVALID, "group=%d outside 1..%d", group, kMultiMax);
+    m->group = group;
+    return TREX_OK;
+}
+
+int trex_dfa_multi_info(const trex_dfa_multi* m, int* n_patterns, int* form, int* group) {
+    if (!m) return fail(TREX_ERR_INVALID, "null handle");
+    if (n_patterns) *n_patterns = m->n;
+    if (form) *form = m->form;
+    if (group) *group = m->group;
+    return TREX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int multi_dispatch(trex_dfa_multi* m, const uint8_t* d_bytes, const int64_t* d_begin, const int64_t* d_end,
+                   int64_t n_strings, uint8_t* d_final_state, uint8_t* d_accept, void* stream, bool il) {
+    if (!m) return fail(TREX_ERR_INVALID, "null handle");
+    if (n_strings < 0) return fail(TREX_ERR_INVALID, "n_strings < 0");
+    if (n_strings == 0) return TREX_OK;
+    if (!d_bytes || !d_begin || !d_end || !d_final_state || !d_accept) return fail(TREX_ERR_INVALID, "null buffer");
+    const int64_t grid64 = (n_strings + kBlock - 1) / kBlock;
+    if (grid64 > 0x7fffffff) return fail(TREX_ERR_CAPACITY, "too many strings");
+    const unsigned grid = (unsigned)grid64;
+    const bool f = m->form == TREX_DFA_K2_WINDOW_F;
+    const size_t pair_bytes = f ? 1024 * 8 : 1024 * 16, sym_bytes = f ? 32 * 8 : 32 * 16;
+    const int launches = (m->n + m->group - 1) / m->group;
+    for (int l = 0, p0 = 0; l < launches; ++l) {
+        const int P = (m->n - p0) / (launches - l);   // even split: sizes differ by at most one
+        MultiParams prm{};
+        for (int k = 0; k < P; ++k) {
+            prm.win4[k] = m->win4[p0 + k];
+            prm.start[k] = m->start[p0 + k];
+            prm.dead[k] = m->dead[p0 + k];
+            prm.accept[k] = m->accept[p0 + k];
+        }
+        const uint8_t* pairs = (f ? m->d_f2 : m->d_t2) + (size_t)p0 * pair_bytes;
+        const uint8_t* syms = (f ? m->d_f1 : m->d_t1) + (size_t)p0 * sym_bytes;
+        uint8_t* out_s = d_final_state + (size_t)p0 * (size_t)n_strings;
+        uint8_t* out_a = d_accept + (size_t)p0 * (size_t)n_strings;
+        void* args[] = {&d_bytes, &d_begin, &d_end, &n_strings, &pairs, &syms, &prm, &out_s, &out_a};
+        char name[48];
+        std::snprintf(name, sizeof(name), "dfa16_multi_%s_p%d%s", f ? "k2f" : "k2w", P, il ? "_il" : "");
+        TREX_TRY(launch(name, grid, kBlock, 0, (hipStream_t)stream, args));
+        p0 += P;
+    }
+    return TREX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int trex_dfa_multi_match_ranges(trex_dfa_multi* m, const uint8_t* d_bytes, const int64_t* d_begin,
+                                const int64_t* d_end, int64_t n_strings, uint8_t* d_final_state, uint8_t* d_accept,
+                                void* stream) {
+    return multi_dispatch(m, d_bytes, d_begin, d_end, n_strings, d_final_state, d_accept, stream, false);
+}
+
+int trex_dfa_multi_match_interleaved(trex_dfa_multi* m, const uint8_t* d_bytes, const int64_t* d_begin,
+                                     const int64_t* d_end, int64_t n_strings, uint8_t* d_final_state,
+                                     uint8_t* d_accept, void* stream) {
+    return multi_dispatch(m, d_bytes, d_begin, d_end, n_strings, d_final_state, d_accept, stream, true);
+}
+
+}  // extern "C"

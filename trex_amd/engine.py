@@ -193,6 +193,82 @@ def interleaved_image(data: np.ndarray, offsets: np.ndarray) -> tuple[np.ndarray
     return image, np.ascontiguousarray(begin, np.int64), np.ascontiguousarray(end, np.int64)
 
 
+MULTI_FORMS = {"k2f": 5, "k2w": 3}
+MULTI_FORM_NAMES = {v: k for k, v in MULTI_FORMS.items()}
+
+
+class DfaMultiEngine:
+    """Several DFA-16 patterns matched in one pass over the bytes
+    (``trex_dfa_multi_*``): each launch reads the batch once and advances up
+    to ``group`` (default 4) patterns. Every pattern must take the window
+    kernels (``DfaEngine.info()["window_base"] >= 0``). Results are
+    pattern-major arrays ``[n_patterns, n_strings]`` equal to each pattern's
+    own :class:`DfaEngine` results."""
+
+    def __init__(self, engines: Sequence[DfaEngine]):
+        engines = list(engines)
+        if not engines:
+            raise ValueError("no patterns")
+        handles = (ctypes.c_void_p * len(engines))(*[e.handle for e in engines])
+        h = ctypes.c_void_p()
+        _lib.call("trex_dfa_multi_create", handles, len(engines), ctypes.byref(h))
+        self.handle = h.value
+        self.n_patterns = len(engines)
+
+    def info(self) -> dict:
+        n, f, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.call("trex_dfa_multi_info", self.handle, ctypes.byref(n), ctypes.byref(f), ctypes.byref(g))
+        return {"n_patterns": n.value, "form": MULTI_FORM_NAMES.get(f.value, f.value), "group": g.value}
+
+    def set_form(self, name: str) -> None:
+        _lib.call("trex_dfa_multi_set_form", self.handle, MULTI_FORMS[name])
+
+    def set_group(self, group: int) -> None:
+        _lib.call("trex_dfa_multi_set_group", self.handle, int(group))
+
+    def match_batch_device(self, batch: "DeviceBatch", d_final: int, d_accept: int,
+                           stream: Stream | None = None) -> None:
+        """Asynchronous dispatch on a :class:`DeviceBatch` (any layout); the
+        outputs hold ``n_patterns * n_strings`` bytes each."""
+        fn = "trex_dfa_multi_match_interleaved" if batch.layout == "interleaved" else "trex_dfa_multi_match_ranges"
+        _lib.call(fn, self.handle, batch.d_bytes.ptr, batch.d_begin.ptr, batch.d_end.ptr, batch.n_strings,
+                  d_final, d_accept, stream.handle if stream else None)
+
+    def match_arrays(self, data: np.ndarray, offsets: np.ndarray, layout: str = "csr"):
+        """Host CSR batch -> (final_state uint8[P, n], accept uint8[P, n])."""
+        data, offsets = _as_csr(data=data, offsets=offsets)
+        n = len(offsets) - 1
+        final = np.empty((self.n_patterns, n), dtype=np.uint8)
+        acc = np.empty((self.n_patterns, n), dtype=np.uint8)
+        if n == 0:
+            return final, acc
+        batch = DeviceBatch(data, offsets, layout=layout)
+        d_f, d_a = DeviceBuffer(final.size), DeviceBuffer(acc.size)
+        try:
+            self.match_batch_device(batch, d_f.ptr, d_a.ptr)   # null stream; the downloads synchronise
+            final[:] = d_f.download(np.uint8, final.size).reshape(final.shape)
+            acc[:] = d_a.download(np.uint8, acc.size).reshape(acc.shape)
+        finally:
+            batch.free()
+            d_f.free()
+            d_a.free()
+        return final, acc
+
+    def match_strings(self, strings: Sequence):
+        return self.match_arrays(*pack_strings(strings))
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.lib().trex_dfa_multi_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceBatch:
     """A string batch resident in HBM: bytes plus per-string [begin, end) ranges.
 
