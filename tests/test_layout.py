@@ -65,3 +65,26 @@ def test_staggered_ranges_shift_per_group():
     assert np.array_equal(begin - offsets[:-1], groups * STAGGER_GAP)
     assert np.array_equal(end - begin, np.diff(offsets))
     assert size == 2000 + (-(-200 // STAGGER_GROUP)) * STAGGER_GAP
+
+
+@pytest.mark.parametrize("items", [
+    ["", "a", "\xff\x00b", "xyz" * 100],
+    [b"", b"x", bytearray(b"yz"), b"\x00\xff"],
+    ["ab", b"cd", memoryview(b"ef"), bytearray(b"")],
+    [],
+    ("t", "u", ""),
+])
+def test_pack_strings_fast_paths_equal_the_per_item_packing(items):
+    """Joined-once packing (all str, all bytes) equals encoding item by item."""
+    from trex_amd.engine import pack_strings
+    data, offsets = pack_strings(items)
+    parts = [s.encode("latin-1") if isinstance(s, str) else bytes(s) for s in items]
+    assert bytes(data) == b"".join(parts)
+    assert offsets.tolist() == np.concatenate([[0], np.cumsum([len(p) for p in parts], dtype=np.int64)]).tolist()
+    assert data.dtype == np.uint8 and offsets.dtype == np.int64
+
+
+def test_pack_strings_rejects_characters_beyond_latin1():
+    from trex_amd.engine import pack_strings
+    with pytest.raises(UnicodeEncodeError):
+        pack_strings(["ok", "Ā"])
