@@ -192,23 +192,46 @@ __device__ __forceinline__ void store_release(uint32_t* p, uint32_t v) {
 // h = (k >> 4) & 1, x = k & 15, so that lanes looking up the same nibble value
 // in different classes hit different banks. XOR-ing every nibble of the state
 // with x gives the slot directly; rows of even and odd p move by +h and -h.
+// Nibble n of w as one v_bfe_u32 (the compiler splits (w >> 4n) & 15 into a
+// shift and an and).
+template <int N>
+__device__ __forceinline__ uint32_t nibble(uint32_t w) {
+    if constexpr (N == 0) {
+        return w & 15u;
+    } else {
+        uint32_t r;
+        asm("v_bfe_u32 %0, %1, %2, 4" : "=v"(r) : "v"(w), "n"(4 * N));
+        return r;
+    }
+}
+
 __device__ __forceinline__ uint64_t nfa_step(const uint64_t* __restrict__ tab, uint32_t k, uint64_t s) {
     const uint32_t x = (k & 15u) * 0x11111111u;
     const uint32_t lo = (uint32_t)s ^ x, hi = (uint32_t)(s >> 32) ^ x;
-    const int h16 = (int)((k >> 4) & 1u) * 16;
-    const uint64_t* te = tab + k * 256u + h16;   // even p: row p + h
-    const uint64_t* to = tab + k * 256u - h16;   // odd p: row p - h
+    // Row 0 of the even rows (p + h) and row 1 of the odd rows (p + 1 - h):
+    // both at or after the class's table, so row p's offset p * 16 entries
+    // goes into the read's immediate and a lookup is an extract and a
+    // shift-add.
+    const uint32_t h = ((k >> 4) & 1u) * 16u;
+    const uint64_t* te = tab + k * 256u + h;
+    const uint64_t* to = tab + k * 256u + (16u - h);
     uint64_t r0 = 0, r1 = 0;
-#pragma unroll
-    for (int p = 0; p < 8; p += 2) {
-        r0 |= te[p * 16 + ((lo >> (4 * p)) & 15u)];
-        r1 |= to[(p + 1) * 16 + ((lo >> (4 * p + 4)) & 15u)];
-    }
-#pragma unroll
-    for (int p = 0; p < 8; p += 2) {
-        r0 |= te[(8 + p) * 16 + ((hi >> (4 * p)) & 15u)];
-        r1 |= to[(9 + p) * 16 + ((hi >> (4 * p + 4)) & 15u)];
-    }
+    r0 |= te[0 * 16 + nibble<0>(lo)];
+    r1 |= to[0 * 16 + nibble<1>(lo)];
+    r0 |= te[2 * 16 + nibble<2>(lo)];
+    r1 |= to[2 * 16 + nibble<3>(lo)];
+    r0 |= te[4 * 16 + nibble<4>(lo)];
+    r1 |= to[4 * 16 + nibble<5>(lo)];
+    r0 |= te[6 * 16 + nibble<6>(lo)];
+    r1 |= to[6 * 16 + nibble<7>(lo)];
+    r0 |= te[8 * 16 + nibble<0>(hi)];
+    r1 |= to[8 * 16 + nibble<1>(hi)];
+    r0 |= te[10 * 16 + nibble<2>(hi)];
+    r1 |= to[10 * 16 + nibble<3>(hi)];
+    r0 |= te[12 * 16 + nibble<4>(hi)];
+    r1 |= to[12 * 16 + nibble<5>(hi)];
+    r0 |= te[14 * 16 + nibble<6>(hi)];
+    r1 |= to[14 * 16 + nibble<7>(hi)];
     return r0 | r1;
 }
 

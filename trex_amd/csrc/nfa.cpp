@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -14,6 +15,7 @@ using namespace trex;
 
 namespace {
 constexpr int kNfaBlock = 512;
+constexpr int kNfa5Block = 1024;   // nfa64_window5: one 1024-thread workgroup per CU holds the 104 KB of tables
 constexpr int kMaxSymbols = 62;   // (62 + 2) classes x 2 KB of nibble tables fit one workgroup's LDS
 }
 
@@ -41,6 +43,27 @@ std::vector<uint64_t> swizzled(const std::vector<uint64_t>& tab) {
         const unsigned h = (unsigned)(k >> 4) & 1u, x = (unsigned)k & 15u;
         for (unsigned p = 0; p < 16; ++p)
             for (unsigned v = 0; v < 16; ++v) out[k * 256 + (p ^ h) * 16 + (v ^ x)] = tab[k * 256 + p * 16 + v];
+    }
+    return out;
+}
+
+// 5-bit slice tables of the 32 window classes (layout of nfa64_window5):
+// class k at byte k * kClass5, entry (r, v) = union of succ[5r + i] over the
+// bits i of v at byte r * 256 + v * 8 of it; row 12 holds states 60..63 (its
+// entries v >= 16 repeat v & 15 and are never read).
+constexpr size_t kClass5 = 13 * 256 + 8;
+
+std::vector<uint8_t> slice5_tables(const std::vector<const uint64_t*>& succ_of_class) {
+    std::vector<uint8_t> out(32 * kClass5, 0);
+    for (int k = 0; k < 32; ++k) {
+        const uint64_t* succ64 = succ_of_class[k];
+        for (int r = 0; r < 13; ++r)
+            for (int v = 0; v < 32; ++v) {
+                uint64_t m = 0;
+                for (int i = 0; i < 5; ++i)
+                    if ((v & (1 << i)) && 5 * r + i < 64) m |= succ64[5 * r + i];
+                std::memcpy(out.data() + k * kClass5 + r * 256 + v * 8, &m, 8);
+            }
     }
     return out;
 }
@@ -104,9 +127,18 @@ int trex_nfa_create(const uint64_t* succ, int n_symbols, const int16_t* byte_to_
         const std::vector<uint64_t> tw_lds = swizzled(tw);
         rc = upload((void**)&h->d_tab_win, tw_lds.data(), tw_lds.size() * 8);
         if (!rc) rc = upload((void**)&h->d_cls_win, cw.data(), cw.size());
+        if (!rc) {
+            std::vector<const uint64_t*> src(32);
+            for (int k = 0; k < 32; ++k) {
+                const int c = cls[h->window_base + k];
+                src[k] = c < n_symbols ? succ + (size_t)c * 64 : c == reject ? none.data() : ident.data();
+            }
+            const std::vector<uint8_t> t5 = slice5_tables(src);
+            rc = upload((void**)&h->d_tab5, t5.data(), t5.size());
+        }
     }
     if (rc) { trex_nfa_destroy(h); return rc; }
-    h->variant = h->window_base >= 0 ? TREX_NFA_WINDOW : TREX_NFA_BYTES;
+    h->variant = h->d_tab5 ? TREX_NFA_WINDOW5 : h->window_base >= 0 ? TREX_NFA_WINDOW : TREX_NFA_BYTES;
     *out = h;
     return TREX_OK;
 }
@@ -118,6 +150,7 @@ int trex_nfa_destroy(trex_nfa* h) {
     hipFree(h->d_cls);
     hipFree(h->d_tab_win);
     hipFree(h->d_cls_win);
+    hipFree(h->d_tab5);
     h->in_bytes.release();
     h->in_offsets.release();
     h->out_mask.release();
@@ -137,7 +170,8 @@ int trex_nfa_info(const trex_nfa* h, int* variant, int* window_base) {
 
 int trex_nfa_set_variant(trex_nfa* h, int variant) {
     if (!h) return fail(TREX_ERR_INVALID, "null handle");
-    if (variant == TREX_NFA_BYTES || (variant == TREX_NFA_WINDOW && h->window_base >= 0)) {
+    if (variant == TREX_NFA_BYTES || (variant == TREX_NFA_WINDOW && h->window_base >= 0) ||
+        (variant == TREX_NFA_WINDOW5 && h->d_tab5)) {
         h->variant = variant;
         return TREX_OK;
     }
@@ -150,6 +184,15 @@ int trex_nfa_match_ranges(trex_nfa* h, const uint8_t* d_bytes, const int64_t* d_
     if (n_strings < 0) return fail(TREX_ERR_INVALID, "n_strings < 0");
     if (n_strings == 0) return TREX_OK;
     if (!d_bytes || !d_begin || !d_end || !d_final_mask || !d_accept) return fail(TREX_ERR_INVALID, "null buffer");
+    if (h->variant == TREX_NFA_WINDOW5) {
+        const int64_t grid5 = (n_strings + kNfa5Block - 1) / kNfa5Block;
+        if (grid5 > 0x7fffffff) return fail(TREX_ERR_CAPACITY, "too many strings");
+        uint8_t* tab5 = h->d_tab5;
+        uint32_t win4 = (uint32_t)h->window_base * 0x01010101u;
+        uint64_t start = h->start, accept = h->accept;
+        void* args[] = {&d_bytes, &d_begin, &d_end, &n_strings, &tab5, &win4, &start, &accept, &d_final_mask, &d_accept};
+        return launch("nfa64_window5", (unsigned)grid5, kNfa5Block, 0, (hipStream_t)stream, args);
+    }
     const int64_t grid64 = (n_strings + kNfaBlock - 1) / kNfaBlock;
     if (grid64 > 0x7fffffff) return fail(TREX_ERR_CAPACITY, "too many strings");
     const bool win = h->variant == TREX_NFA_WINDOW;
