@@ -733,6 +733,23 @@ def mix_leg(args, stream) -> dict:
     log(f"mix: one launch {avg:.2f} ms (graph replay {np.median(ms_graph):.2f} ms), {gchars:.1f} Gchars/s, "
         f"plan {info}, error word {err}")
 
+    # the same plan parameters with the other NFA table form
+    other = "nibble" if params.form == "slice5" else "slice5"
+    mix5 = MixEngine(engines)
+    mix5.plan(begin, end, pattern_of, MixParams(**{**params.__dict__, "form": other}))
+    mix5.run(d_bytes.ptr, d_mask.ptr, d_acc.ptr, stream, args.mix_max_spins)
+    ms5 = timed(lambda: mix5.run(d_bytes.ptr, d_mask.ptr, d_acc.ptr, stream, args.mix_max_spins),
+                max(2, args.steps // 10))
+    err5 = mix5.error(stream)
+    same5 = bool(np.array_equal(d_mask.download(np.uint64, n, stream), full_mask)
+                 and np.array_equal(d_acc.download(np.uint8, n, stream), full_acc))
+    avg5 = float(np.median(ms5))
+    log(f"mix ({other} NFA tables): one launch {avg5:.2f} ms, {total / (avg5 * 1e-3) / 1e9:.1f} Gchars/s, "
+        f"error word {err5}, results equal to the tuned form's: {same5}")
+    slice5 = {"form": other, "kernel_ms": round(avg5, 3), "gchars_s": round(total / (avg5 * 1e-3) / 1e9, 1),
+              "error_word": int(err5), "same_results": same5}
+    del mix5
+
     # batched vs plain on the subset
     sub = np.flatnonzero(np.isin(pattern_of, MIX_SUBSET))
     sub_bytes = int(lengths[sub].sum())
@@ -806,7 +823,7 @@ def mix_leg(args, stream) -> dict:
             t_cpu += time.perf_counter() - t0
             n_checked += int(sub_off[-1])
             ok &= bool(np.array_equal(full_mask[idx], rf.astype(np.uint64)) and np.array_equal(full_acc[idx], ra))
-        verified = ok and same_sub and err == 0
+        verified = ok and same_sub and err == 0 and slice5["same_results"] and slice5["error_word"] == 0
         cpu = {"value": round(n_checked / t_cpu / 1e9, 3), "unit": "Gchars/s", "cores": ocpu.default_threads(),
                "kind": "port", "sample": f"{len(pick)} strings ({n_checked / 1e9:.2f} GB: 4096 <= 64 KiB and 24 "
                                          f"> 1 MiB) through the C oracle walks, {t_cpu:.2f} s"}
@@ -815,7 +832,8 @@ def mix_leg(args, stream) -> dict:
         b.free()
     return {"workload": "uneven_mix_64_patterns", "strings": n, "bytes": total, "patterns": "32 DFA-16 + 32 NFA-64",
             "value": round(gchars, 2), "unit": "Gchars/s", "kernel_ms": round(avg, 3),
-            "graph_replay_ms": round(float(np.median(ms_graph)), 3), "plan": info, "params": params.__dict__,
+            "graph_replay_ms": round(float(np.median(ms_graph)), 3), "other_form": slice5,
+            "plan": info, "params": params.__dict__,
             "tuning": tuning,
             "roofline": {"bound": "hbm (DFA strings) / lds (NFA strings)",
                          "achieved": round(total / (avg * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

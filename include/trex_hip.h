@@ -347,6 +347,15 @@ int trex_mix_run(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, uint
 /* Persistent workgroups of later runs (0: two per compute unit). Fewer leave
  * compute units to kernels on other streams. */
 int trex_mix_set_grid(trex_mix* m, int64_t workgroups);
+/* NFA table form of the launch: TREX_MIX_FORM_NIBBLE (default; 512-thread
+ * workgroups, two per CU) or TREX_MIX_FORM_SLICE5 (5-bit state slices, 13 LDS
+ * reads per NFA byte instead of 16; 1024-thread workgroups, one per CU; every
+ * NFA pattern must have a window). Batches are cut to the workgroup size, so
+ * call it before trex_mix_plan; a run whose plan was made for the other form
+ * fails. Results are the same in both forms. */
+#define TREX_MIX_FORM_NIBBLE 0
+#define TREX_MIX_FORM_SLICE5 1
+int trex_mix_set_form(trex_mix* m, int form);
 /* Enqueues on `stream`, behind the last run, a copy of the run's first four
  * queue words (ticket, error word, converged count, one flag) to dst:
  * 16 bytes, 16-byte aligned, device or pinned host memory, copied by a kernel

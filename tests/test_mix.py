@@ -27,7 +27,7 @@ def _batch(rng, n, lo, hi, n_patterns):
     return offsets, pattern_of
 
 
-def check_plan(m: MixEngine, begin, end, pattern_of, kinds, params: MixParams):
+def check_plan(m: MixEngine, begin, end, pattern_of, kinds, params: MixParams, block: int = BLOCK):
     L = m.layout()
     nb = len(L["bfirst"])
     roles = L["brole"]
@@ -37,7 +37,7 @@ def check_plan(m: MixEngine, begin, end, pattern_of, kinds, params: MixParams):
     item_batch = np.full(len(L["ibeg"]), -1)
     for b in range(nb):
         f, c, p, r = L["bfirst"][b], L["bcount"][b], L["bpattern"][b], roles[b]
-        assert 1 <= c <= BLOCK
+        assert 1 <= c <= block
         assert (kinds[p] == DFA) == (r in (0, 1))
         assert np.all(item_batch[f:f + c] == -1)
         item_batch[f:f + c] = b
@@ -89,14 +89,17 @@ def check_plan(m: MixEngine, begin, end, pattern_of, kinds, params: MixParams):
         assert (i in whole_strings) == (end[i] - begin[i] <= split)
 
 
-def test_plan_invariants():
+@pytest.mark.parametrize("form,block", [("nibble", BLOCK), ("slice5", 2 * BLOCK)])
+def test_plan_invariants(form, block):
     rng = np.random.default_rng(0)
     kinds = [DFA, NFA, DFA, NFA, NFA]
     offsets, pattern_of = _batch(rng, 3000, 1, 200_000, len(kinds))
     m = MixEngine(kinds=kinds)
+    m.set_form(form)                      # batches are cut to the form's workgroup size
     info = m.plan(offsets[:-1], offsets[1:], pattern_of, SMALL)
     assert info["chunks"] > 0 and info["fixups"] > 0
-    check_plan(m, offsets[:-1], offsets[1:], pattern_of, kinds, SMALL)
+    check_plan(m, offsets[:-1], offsets[1:], pattern_of, kinds, SMALL, block)
+    assert max(m.layout()["bcount"]) == block
 
 
 def test_plan_edge_cases():
@@ -173,9 +176,10 @@ def _oracle(refs, data, offsets, pattern_of):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("form", ["nibble", "slice5"])
 @pytest.mark.parametrize("params", [SMALL, MixParams(dfa_split=1 << 20, dfa_chunk=1 << 16, nfa_split=1 << 20,
                                                      nfa_chunk=1 << 16, ck=256)], ids=["small", "large"])
-def test_gpu_mix_matches_oracle(gpu_lib, params):
+def test_gpu_mix_matches_oracle(gpu_lib, params, form):
     rng = np.random.default_rng(1)
     pats, refs = _patterns(rng, 5, 5, permutation_nfa=1)
     offsets, pattern_of = _batch(rng, 2500, 1, 300_000, len(pats))
@@ -184,6 +188,7 @@ def test_gpu_mix_matches_oracle(gpu_lib, params):
     bad = rng.choice(len(data), size=20, replace=False)
     data[bad] = rng.integers(0, 0x40, size=20, dtype=np.uint8)
     m = MixEngine(pats)
+    m.set_form(form)
     mask, acc = m.match_arrays(data, offsets, pattern_of, params)
     ref_mask, ref_acc = _oracle(refs, data, offsets, pattern_of)
     bad_idx = np.flatnonzero((mask != ref_mask) | (acc != ref_acc))
@@ -191,7 +196,8 @@ def test_gpu_mix_matches_oracle(gpu_lib, params):
 
 
 @pytest.mark.gpu
-def test_gpu_mix_graph_replay(gpu_lib):
+@pytest.mark.parametrize("form", ["nibble", "slice5"])
+def test_gpu_mix_graph_replay(gpu_lib, form):
     from trex_amd.device import DeviceBuffer, Stream
     rng = np.random.default_rng(2)
     pats, refs = _patterns(rng, 3, 3)
@@ -200,6 +206,7 @@ def test_gpu_mix_graph_replay(gpu_lib):
     ref_mask, ref_acc = _oracle(refs, data, offsets, pattern_of)
     n = len(offsets) - 1
     m = MixEngine(pats)
+    m.set_form(form)
     m.plan(offsets[:-1], offsets[1:], pattern_of, SMALL)
     stream = Stream()
     d_bytes = DeviceBuffer.from_array(data, stream)
@@ -223,6 +230,24 @@ def test_gpu_mix_rejects_unsupported_pattern(gpu_lib):
     e = DfaEngine(dfa_byte_table(compile_regex("(a|A)*b")))   # live bytes 0x41 and 0x61: two windows
     with pytest.raises(TrexError):
         MixEngine([e])
+
+
+@pytest.mark.gpu
+def test_gpu_mix_run_refuses_a_plan_for_the_other_form(gpu_lib):
+    from trex_amd._lib import TrexError
+    from trex_amd.device import DeviceBuffer
+    rng = np.random.default_rng(4)
+    pats, _ = _patterns(rng, 1, 1)
+    offsets, pattern_of = _batch(rng, 50, 1, 5000, len(pats))
+    data = rng.integers(0x40, 0x60, size=int(offsets[-1]), dtype=np.uint8)
+    m = MixEngine(pats)
+    m.plan(offsets[:-1], offsets[1:], pattern_of, SMALL)
+    m.set_form("slice5")
+    d_bytes = DeviceBuffer.from_array(data)
+    n = len(offsets) - 1
+    d_mask, d_acc = DeviceBuffer(8 * n), DeviceBuffer(n)
+    with pytest.raises(TrexError, match="plan again"):
+        m.run(d_bytes.ptr, d_mask.ptr, d_acc.ptr)
 
 
 # ---------------------------------------------------------------------------
@@ -263,3 +288,28 @@ def test_gpu_generator_matches_oracle(gpu_lib):
     assert np.array_equal(got, ocpu.gen_symbols(0, n, 99))
     with pytest.raises(_lib.TrexError):
         _lib.call("trex_gen_symbols", d.ptr, 100, 99, 0x40, 32, stream.handle)
+
+
+@pytest.mark.gpu
+def test_gpu_tune_mix_chooses_a_form_and_the_choice_matches(gpu_lib, tmp_path, monkeypatch):
+    """The autotuner times both NFA table forms (two workgroup shapes); the
+    chosen plan gives the oracle's results, and a second call is a cache hit."""
+    monkeypatch.setenv("TREX_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    from trex_amd.autotune import tune_mix
+    from trex_amd.device import DeviceBuffer, Stream
+    rng = np.random.default_rng(6)
+    pats, refs = _patterns(rng, 2, 2)
+    offsets, pattern_of = _batch(rng, 600, 1, 60_000, len(pats))
+    data = rng.integers(0x40, 0x60, size=int(offsets[-1]), dtype=np.uint8)
+    stream = Stream()
+    d_bytes = DeviceBuffer.from_array(data, stream)
+    grid = {"dfa_split": (4096,), "dfa_chunk": (1024,), "nfa_split": (2048,), "nfa_chunk": (1024,), "ck": (64,),
+            "form": ("nibble", "slice5")}
+    params, info = tune_mix(pats, offsets[:-1], offsets[1:], pattern_of, d_bytes.ptr, stream, sample=300, grid=grid)
+    assert info["source"] == "tuned" and info["candidates"] == 2 and params.form in ("nibble", "slice5")
+    m = MixEngine(pats)
+    mask, acc = m.match_arrays(data, offsets, pattern_of, params)
+    ref_mask, ref_acc = _oracle(refs, data, offsets, pattern_of)
+    assert np.array_equal(mask, ref_mask) and np.array_equal(acc, ref_acc)
+    again, info2 = tune_mix(pats, offsets[:-1], offsets[1:], pattern_of, d_bytes.ptr, stream, sample=300, grid=grid)
+    assert info2["source"] == "cache" and again == params

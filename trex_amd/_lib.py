@@ -108,6 +108,7 @@ _SIGNATURES = {
     "trex_mix_run": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_uint32, c_void_p]),
     "trex_mix_status": (c_int, [c_void_p, c_void_p, c_void_p]),
     "trex_mix_error": (c_int, [c_void_p, POINTER(ctypes.c_uint32), c_void_p]),
+    "trex_mix_set_form": (c_int, [c_void_p, c_int]),
     "trex_mix_set_grid": (c_int, [c_void_p, c_int64]),
     "trex_mix_stats": (c_int, [c_void_p, POINTER(ctypes.c_uint32), c_void_p]),
     "trex_mix_capture": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_uint32, c_void_p]),

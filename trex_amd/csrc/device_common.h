@@ -235,4 +235,57 @@ __device__ __forceinline__ uint64_t nfa_step(const uint64_t* __restrict__ tab, u
     return r0 | r1;
 }
 
+// ---------------------------------------------------------------------------
+// 5-bit slices: 13 lookups per byte instead of 16.
+//
+//     F[c][r][v] = union of succ[c][5r + i] over the bits i of v   (uint64)
+//
+// for rows r = 0..12 (row 12 holds states 60..63, four bits). Only the 32
+// window classes are held (106,752 bytes of LDS): head and tail bytes enter
+// the window check like the body and step with class byte & 31. Class k's
+// table starts at byte k * kClass5 with kClass5 = 13 * 256 + 8, so entry
+// (r, v) of class k falls in bank pair (v + k) mod 32: lanes reading the same
+// slot value in different classes hit different banks, with no per-lookup
+// cost. Row r's offset r * 256 goes into the read's immediate.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kClass5 = 13u * 256u + 8u;
+constexpr uint32_t kTab5Bytes = 32u * kClass5;
+// Bits [OFF, OFF + 5) of w as one v_bfe_u32 (bits beyond 31 read as zero).
+template <int OFF>
+__device__ __forceinline__ uint32_t field5(uint32_t w) {
+    if constexpr (OFF == 0) {
+        return w & 31u;
+    } else {
+        uint32_t r;
+        asm("v_bfe_u32 %0, %1, %2, 5" : "=v"(r) : "v"(w), "n"(OFF));
+        return r;
+    }
+}
+
+__device__ __forceinline__ uint64_t nfa5_step(const char* __restrict__ tb, uint32_t k, uint64_t s) {
+    const uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32);
+    // the class's byte offset, computed once per step: left visible, the
+    // compiler folds k * kClass5 into every lookup's address (one more
+    // instruction per lookup)
+    uint32_t base = k * kClass5;
+    asm("" : "+v"(base));
+    const char* t = tb + base;
+    auto ld = [t](uint32_t off) { return *reinterpret_cast<const uint64_t*>(t + off); };
+    uint64_t r0 = 0, r1 = 0;
+    r0 |= ld(0 * 256 + field5<0>(lo) * 8u);
+    r1 |= ld(1 * 256 + field5<5>(lo) * 8u);
+    r0 |= ld(2 * 256 + field5<10>(lo) * 8u);
+    r1 |= ld(3 * 256 + field5<15>(lo) * 8u);
+    r0 |= ld(4 * 256 + field5<20>(lo) * 8u);
+    r1 |= ld(5 * 256 + field5<25>(lo) * 8u);
+    r0 |= ld(6 * 256 + (__builtin_amdgcn_alignbit(hi, lo, 30) & 31u) * 8u);   // bits 30..34
+    r1 |= ld(7 * 256 + field5<3>(hi) * 8u);
+    r0 |= ld(8 * 256 + field5<8>(hi) * 8u);
+    r1 |= ld(9 * 256 + field5<13>(hi) * 8u);
+    r0 |= ld(10 * 256 + field5<18>(hi) * 8u);
+    r1 |= ld(11 * 256 + field5<23>(hi) * 8u);
+    r0 |= ld(12 * 256 + field5<28>(hi) * 8u);                                   // bits 60..63
+    return r0 | r1;
+}
+
 }  // namespace trexdev

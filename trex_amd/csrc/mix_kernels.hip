@@ -36,6 +36,7 @@
 #include "device_common.h"
 
 #define MIX_BLOCK 512
+#define MIX_BLOCK5 1024
 
 namespace {
 
@@ -57,7 +58,7 @@ struct MixPattern {
     const uint8_t* t2;    // DFA pair rows [1024][16]
     const uint8_t* f2;    // DFA pair functions [1024] x uint64
     const uint8_t* acc;   // DFA accept [16]
-    const uint8_t* ntab;  // NFA window tables [34][256] x uint64 (bank-swizzled)
+    const uint8_t* ntab;  // NFA window tables: [34][256] x uint64 (bank-swizzled) or 5-bit slices
     const uint8_t* ncls;  // NFA byte -> class [256]
 };
 
@@ -244,30 +245,49 @@ __device__ void dfa_chunk(const MixArgs& A, const MixPattern& P, const DfaLds& L
 // NFA roles
 // ---------------------------------------------------------------------------
 
+// The NFA step over the tables of the launch's form: nibble tables (34
+// classes, reject and skip included, bank-swizzled) with a byte -> class map,
+// or 5-bit slice tables of the 32 window classes, where every byte enters the
+// window check and steps with class byte & 31.
+template <bool S5>
 struct NfaWalker {
-    const uint64_t* tab;
+    const void* tab;
     const uint8_t* cls;
     uint64_t s;
     WinCheck w;
-    __device__ __forceinline__ void byte(uint32_t b) { s = nfa_step(tab, cls[b], s); }
+    __device__ __forceinline__ uint64_t step(uint32_t k, uint64_t x) const {
+        if constexpr (S5) return nfa5_step(static_cast<const char*>(tab), k, x);
+        else return nfa_step(static_cast<const uint64_t*>(tab), k, x);
+    }
+    __device__ __forceinline__ void byte(uint32_t b) {
+        if constexpr (S5) {
+            w.add(b * 0x01010101u);
+            s = step(b & 31u, s);
+        } else {
+            s = step(cls[b], s);
+        }
+    }
     __device__ __forceinline__ void word(uint32_t x) {
         w.add(x);
-        s = nfa_step(tab, x & 31u, s);
-        s = nfa_step(tab, (x >> 8) & 31u, s);
-        s = nfa_step(tab, (x >> 16) & 31u, s);
-        s = nfa_step(tab, (x >> 24) & 31u, s);
+        s = step(x & 31u, s);
+        s = step((x >> 8) & 31u, s);
+        s = step((x >> 16) & 31u, s);
+        s = step((x >> 24) & 31u, s);
     }
     __device__ __forceinline__ void vec(uint4 v) { word(v.x); word(v.y); word(v.z); word(v.w); }
     // A byte outside the window rejects: once seen, the set is empty for good.
+    // Applied before every set that is compared or stored, so both forms give
+    // the same sets there.
     __device__ __forceinline__ void settle(uint32_t win4) {
         if (w.outside(win4)) s = 0;
         w.init();
     }
 };
 
+template <bool S5>
 __device__ uint64_t nfa_walk(const uint8_t* __restrict__ bytes, int64_t p, int64_t end, uint64_t s,
-                             const uint64_t* tab, const uint8_t* cls, uint32_t win4) {
-    NfaWalker nw{tab, cls, s, {}};
+                             const void* tab, const uint8_t* cls, uint32_t win4) {
+    NfaWalker<S5> nw{tab, cls, s, {}};
     nw.w.init();
     const int64_t head_end = min(end, p + head_len(bytes, p));
     for (; p < head_end; ++p) nw.byte(bytes[p]);
@@ -285,6 +305,7 @@ __device__ uint64_t nfa_walk(const uint8_t* __restrict__ bytes, int64_t p, int64
     nw.settle(win4);
     p += nvec << 4;
     for (; p < end; ++p) nw.byte(bytes[p]);
+    nw.settle(win4);
     return nw.s;
 }
 
@@ -292,12 +313,12 @@ __device__ uint64_t nfa_walk(const uint8_t* __restrict__ bytes, int64_t p, int64
 // position that is a multiple of ck (ck a multiple of 16) into ck_out[0..n_ck).
 // When `compare`, stops at the first such position whose set equals
 // ck_in[idx] and reports it through `met`.
-template <bool RECORD, bool COMPARE>
+template <bool S5, bool RECORD, bool COMPARE>
 __device__ uint64_t nfa_walk_ck(const uint8_t* __restrict__ bytes, int64_t p, int64_t end, uint64_t s,
-                                const uint64_t* tab, const uint8_t* cls, uint32_t win4, int32_t ck,
+                                const void* tab, const uint8_t* cls, uint32_t win4, int32_t ck,
                                 uint64_t* __restrict__ ck_out, const uint64_t* __restrict__ ck_in, int32_t n_ck,
                                 bool& met) {
-    NfaWalker nw{tab, cls, s, {}};
+    NfaWalker<S5> nw{tab, cls, s, {}};
     nw.w.init();
     met = false;
     const int64_t head_end = min(end, p + head_len(bytes, p));
@@ -323,15 +344,15 @@ __device__ uint64_t nfa_walk_ck(const uint8_t* __restrict__ bytes, int64_t p, in
     nw.settle(win4);
     p += nvec << 4;
     for (; p < end; ++p) nw.byte(bytes[p]);
+    nw.settle(win4);
     return nw.s;
 }
 
-}  // namespace
-
-// Dynamic LDS: kNfaClasses * 2048 + 256 bytes (the NFA tables; the DFA tables
-// use the first 29 KB of the same space).
-extern "C" __global__ void __launch_bounds__(MIX_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
-mix_persistent(MixArgs A)
+// One persistent workgroup's loop. Dynamic LDS: the NFA tables of the form
+// (nibble: kNfaClasses * 2048 + 256 bytes; 5-bit slices: kTab5Bytes); the
+// DFA tables use the first 29 KB of the same space.
+template <bool S5>
+__device__ __forceinline__ void mix_loop(const MixArgs& A)
 {
     extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
     __shared__ int32_t s_batch;
@@ -343,8 +364,8 @@ mix_persistent(MixArgs A)
     dl.t2 = lds + 4096;                                       // 16 KB
     dl.f2 = reinterpret_cast<const uint64_t*>(lds + 20480);   // 8 KB
     dl.acc = lds + 28672;                                     // 16 B
-    const uint64_t* ntab = reinterpret_cast<const uint64_t*>(lds);
-    const uint8_t* ncls = lds + kNfaClasses * 2048;
+    const void* ntab = lds;
+    const uint8_t* ncls = S5 ? nullptr : lds + kNfaClasses * 2048;
 
     while (true) {
         __syncthreads();   // the previous batch is finished with LDS and s_batch
@@ -361,6 +382,8 @@ mix_persistent(MixArgs A)
                 stage_table(lds + 4096, P.t2, 16384);
                 stage_table(lds + 20480, P.f2, 8192);
                 stage_table(lds + 28672, P.acc, 16);
+            } else if constexpr (S5) {
+                stage_table(lds, P.ntab, (int)kTab5Bytes);
             } else {
                 stage_table(lds, P.ntab, kNfaClasses * 2048);
                 stage_table(lds + kNfaClasses * 2048, P.ncls, 256);
@@ -383,16 +406,16 @@ mix_persistent(MixArgs A)
             A.out_mask[A.ib[item]] = s;
             A.out_accept[A.ib[item]] = dl.acc[s];
         } else if (role == kRoleNfaWhole) {
-            const uint64_t s = nfa_walk(A.bytes, A.ibeg[item], A.iend[item], P.nstart, ntab, ncls, P.win4);
+            const uint64_t s = nfa_walk<S5>(A.bytes, A.ibeg[item], A.iend[item], P.nstart, ntab, ncls, P.win4);
             A.out_mask[A.ib[item]] = s;
             A.out_accept[A.ib[item]] = (s & P.naccept) != 0 ? 1 : 0;
         } else if (role == kRoleNfaSpec) {
             const bool first = (A.iflags[item] & 1u) != 0;
             const int32_t o = A.ic[item];
             bool met;
-            const uint64_t s = nfa_walk_ck<true, false>(A.bytes, A.ibeg[item], A.iend[item],
-                                                        first ? P.nstart : ~0ull, ntab, ncls, P.win4, A.ck,
-                                                        A.ckpt + (int64_t)o * A.n_ck, nullptr, A.n_ck, met);
+            const uint64_t s = nfa_walk_ck<S5, true, false>(A.bytes, A.ibeg[item], A.iend[item],
+                                                            first ? P.nstart : ~0ull, ntab, ncls, P.win4, A.ck,
+                                                            A.ckpt + (int64_t)o * A.n_ck, nullptr, A.n_ck, met);
             A.spec_exit[o] = s;
             __threadfence();
             atomicAdd(&A.done[A.ia[item]], 1u);
@@ -417,9 +440,9 @@ mix_persistent(MixArgs A)
                 for (int64_t c = c0 + 1; c < c1 && s != 0; ++c) {
                     const int32_t o = A.ic[c];
                     bool met;
-                    const uint64_t t = nfa_walk_ck<false, true>(A.bytes, A.ibeg[c], A.iend[c], s, ntab, ncls,
-                                                                P.win4, A.ck, nullptr,
-                                                                A.ckpt + (int64_t)o * A.n_ck, A.n_ck, met);
+                    const uint64_t t = nfa_walk_ck<S5, false, true>(A.bytes, A.ibeg[c], A.iend[c], s, ntab, ncls,
+                                                                    P.win4, A.ck, nullptr,
+                                                                    A.ckpt + (int64_t)o * A.n_ck, A.n_ck, met);
                     s = met ? A.spec_exit[o] : t;
                     if (met) atomicAdd(A.converged, 1u);
                 }
@@ -428,4 +451,21 @@ mix_persistent(MixArgs A)
             A.out_accept[A.ib[item]] = (s & P.naccept) != 0 ? 1 : 0;
         }
     }
+}
+
+}  // namespace
+
+// Nibble NFA tables: two 512-thread workgroups per CU (70 KB of LDS each).
+extern "C" __global__ void __launch_bounds__(MIX_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+mix_persistent(MixArgs A)
+{
+    mix_loop<false>(A);
+}
+
+// 5-bit slice NFA tables (13 reads per byte instead of 16): one 1024-thread
+// workgroup per CU (104 KB of LDS).
+extern "C" __global__ void __launch_bounds__(MIX_BLOCK5) __attribute__((amdgpu_waves_per_eu(4)))
+mix_persistent5(MixArgs A)
+{
+    mix_loop<true>(A);
 }

@@ -29,15 +29,20 @@ BLOCK = 512   # lane items per work batch (one workgroup)
 
 @dataclass
 class MixParams:
-    """Split lengths, chunk lengths and checkpoint spacing of a plan."""
+    """Split lengths, chunk lengths and checkpoint spacing of a plan, and the
+    NFA table form ("nibble" / "slice5"; None keeps the engine's form)."""
     dfa_split: int = 256 << 10
     dfa_chunk: int = 64 << 10
     nfa_split: int = 32 << 10
     nfa_chunk: int = 32 << 10
     ck: int = 64
+    form: str | None = None
 
     def astuple(self):
         return (int(self.dfa_split), int(self.dfa_chunk), int(self.nfa_split), int(self.nfa_chunk), int(self.ck))
+
+
+MIX_FORMS = {"nibble": 0, "slice5": 1}
 
 
 class MixEngine:
@@ -76,6 +81,8 @@ class MixEngine:
         n = len(self._begin)
         if len(self._end) != n or len(self._pat) != n:
             raise ValueError("begin, end and pattern_of must have the same length")
+        if params.form is not None:
+            self.set_form(params.form)
         _lib.call("trex_mix_plan", self.handle, self._begin.ctypes.data, self._end.ctypes.data, self._pat.ctypes.data,
                   n, *params.astuple())
         self.n_strings = n
@@ -101,8 +108,14 @@ class MixEngine:
                   stream.handle if stream else None)
 
     def set_grid(self, workgroups: int) -> None:
-        """Persistent workgroups of later runs (0: two per compute unit)."""
+        """Persistent workgroups of later runs (0: two per compute unit, one
+        with the "slice5" form)."""
         _lib.call("trex_mix_set_grid", self.handle, int(workgroups))
+
+    def set_form(self, form: str) -> None:
+        """NFA table form: "nibble" (default, 16 LDS reads per NFA byte) or
+        "slice5" (13 reads, 1024-thread workgroups). Call before :meth:`plan`."""
+        _lib.call("trex_mix_set_form", self.handle, MIX_FORMS[form])
 
     def status_to(self, dst: int, stream: Stream | None = None) -> None:
         """Enqueue a copy of the last run's queue words (ticket, error word, converged,
