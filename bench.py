@@ -478,7 +478,7 @@ def nfa_leg(args, data, offsets, batch, stream) -> dict:
         return [a.elapsed_ms(b) for a, b in evs]
 
     default = eng.info()["variant"]
-    per_variant = {v: timed(v) for v in ("window", "window5")}
+    per_variant = {v: timed(v) for v in ("window", "window5", "window6")}
     for v, t in per_variant.items():
         log(f"nfa64 ({v}): kernel avg {np.mean(t):.4f} ms min {min(t):.4f} max {max(t):.4f}")
     eng.set_variant(default)
@@ -496,7 +496,7 @@ def nfa_leg(args, data, offsets, batch, stream) -> dict:
            "variants_ms": {v: round(float(np.mean(t)), 5) for v, t in per_variant.items()},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "alg_bytes_per_launch": alg_bytes,
-                        "note": "LDS table lookups (13 per byte with 5-bit slices, 16 with nibbles) bound "
+                        "note": "LDS table lookups (12 per byte with 6/5-bit slices, 13 with 5-bit, 16 with nibbles) bound "
                                 "the kernel before HBM does"},
            "verified_bit_exact": None, "cpu_baseline": None}
     if not args.no_verify:
@@ -734,21 +734,21 @@ def mix_leg(args, stream) -> dict:
         f"plan {info}, error word {err}")
 
     # the same plan parameters with the other NFA table form
-    other = "nibble" if params.form == "slice5" else "slice5"
-    mix5 = MixEngine(engines)
-    mix5.plan(begin, end, pattern_of, MixParams(**{**params.__dict__, "form": other}))
-    mix5.run(d_bytes.ptr, d_mask.ptr, d_acc.ptr, stream, args.mix_max_spins)
-    ms5 = timed(lambda: mix5.run(d_bytes.ptr, d_mask.ptr, d_acc.ptr, stream, args.mix_max_spins),
+    other = "nibble" if params.form == "slice" else "slice"
+    mix_o = MixEngine(engines)
+    mix_o.plan(begin, end, pattern_of, MixParams(**{**params.__dict__, "form": other}))
+    mix_o.run(d_bytes.ptr, d_mask.ptr, d_acc.ptr, stream, args.mix_max_spins)
+    ms_o = timed(lambda: mix_o.run(d_bytes.ptr, d_mask.ptr, d_acc.ptr, stream, args.mix_max_spins),
                 max(2, args.steps // 10))
-    err5 = mix5.error(stream)
-    same5 = bool(np.array_equal(d_mask.download(np.uint64, n, stream), full_mask)
+    err_o = mix_o.error(stream)
+    same_o = bool(np.array_equal(d_mask.download(np.uint64, n, stream), full_mask)
                  and np.array_equal(d_acc.download(np.uint8, n, stream), full_acc))
-    avg5 = float(np.median(ms5))
-    log(f"mix ({other} NFA tables): one launch {avg5:.2f} ms, {total / (avg5 * 1e-3) / 1e9:.1f} Gchars/s, "
-        f"error word {err5}, results equal to the tuned form's: {same5}")
-    slice5 = {"form": other, "kernel_ms": round(avg5, 3), "gchars_s": round(total / (avg5 * 1e-3) / 1e9, 1),
-              "error_word": int(err5), "same_results": same5}
-    del mix5
+    avg_o = float(np.median(ms_o))
+    log(f"mix ({other} NFA tables): one launch {avg_o:.2f} ms, {total / (avg_o * 1e-3) / 1e9:.1f} Gchars/s, "
+        f"error word {err_o}, results equal to the tuned form's: {same_o}")
+    other_res = {"form": other, "kernel_ms": round(avg_o, 3), "gchars_s": round(total / (avg_o * 1e-3) / 1e9, 1),
+              "error_word": int(err_o), "same_results": same_o}
+    del mix_o
 
     # batched vs plain on the subset
     sub = np.flatnonzero(np.isin(pattern_of, MIX_SUBSET))
@@ -823,7 +823,7 @@ def mix_leg(args, stream) -> dict:
             t_cpu += time.perf_counter() - t0
             n_checked += int(sub_off[-1])
             ok &= bool(np.array_equal(full_mask[idx], rf.astype(np.uint64)) and np.array_equal(full_acc[idx], ra))
-        verified = ok and same_sub and err == 0 and slice5["same_results"] and slice5["error_word"] == 0
+        verified = ok and same_sub and err == 0 and other_res["same_results"] and other_res["error_word"] == 0
         cpu = {"value": round(n_checked / t_cpu / 1e9, 3), "unit": "Gchars/s", "cores": ocpu.default_threads(),
                "kind": "port", "sample": f"{len(pick)} strings ({n_checked / 1e9:.2f} GB: 4096 <= 64 KiB and 24 "
                                          f"> 1 MiB) through the C oracle walks, {t_cpu:.2f} s"}
@@ -832,7 +832,7 @@ def mix_leg(args, stream) -> dict:
         b.free()
     return {"workload": "uneven_mix_64_patterns", "strings": n, "bytes": total, "patterns": "32 DFA-16 + 32 NFA-64",
             "value": round(gchars, 2), "unit": "Gchars/s", "kernel_ms": round(avg, 3),
-            "graph_replay_ms": round(float(np.median(ms_graph)), 3), "other_form": slice5,
+            "graph_replay_ms": round(float(np.median(ms_graph)), 3), "other_form": other_res,
             "plan": info, "params": params.__dict__,
             "tuning": tuning,
             "roofline": {"bound": "hbm (DFA strings) / lds (NFA strings)",

@@ -198,6 +198,7 @@ typedef struct trex_nfa trex_nfa;
 #define TREX_NFA_BYTES 1   /* any byte -> class map */
 #define TREX_NFA_WINDOW 2  /* every byte outside one aligned 32-byte window rejects */
 #define TREX_NFA_WINDOW5 3 /* as 2, 5-bit state slices: 13 table reads per byte instead of 16 */
+#define TREX_NFA_WINDOW6 4 /* as 2, 6/5-bit state slices: 12 table reads per byte */
 
 /* succ: [n_symbols][64] uint64, bit d of succ[c][s] = edge s -> d on symbol c
  * (epsilon closure already folded in, as in NFAMatrices). byte_to_sym[256]:
@@ -206,7 +207,7 @@ typedef struct trex_nfa trex_nfa;
 int trex_nfa_create(const uint64_t* succ, int n_symbols, const int16_t* byte_to_sym, uint64_t start,
                     uint64_t accept, trex_nfa** out);
 int trex_nfa_destroy(trex_nfa* h);
-/* The variant chosen at create time: TREX_NFA_WINDOW5 when every byte outside
+/* The variant chosen at create time: TREX_NFA_WINDOW6 when every byte outside
  * one aligned 32-byte window rejects, else TREX_NFA_BYTES; set_variant forces
  * another the automaton allows. */
 int trex_nfa_info(const trex_nfa* h, int* variant, int* window_base);
@@ -348,13 +349,13 @@ int trex_mix_run(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, uint
  * compute units to kernels on other streams. */
 int trex_mix_set_grid(trex_mix* m, int64_t workgroups);
 /* NFA table form of the launch: TREX_MIX_FORM_NIBBLE (default; 512-thread
- * workgroups, two per CU) or TREX_MIX_FORM_SLICE5 (5-bit state slices, 13 LDS
+ * workgroups, two per CU) or TREX_MIX_FORM_SLICE (6/5-bit state slices, 12 LDS
  * reads per NFA byte instead of 16; 1024-thread workgroups, one per CU; every
  * NFA pattern must have a window). Batches are cut to the workgroup size, so
  * call it before trex_mix_plan; a run whose plan was made for the other form
  * fails. Results are the same in both forms. */
 #define TREX_MIX_FORM_NIBBLE 0
-#define TREX_MIX_FORM_SLICE5 1
+#define TREX_MIX_FORM_SLICE 1
 int trex_mix_set_form(trex_mix* m, int form);
 /* Enqueues on `stream`, behind the last run, a copy of the run's first four
  * queue words (ticket, error word, converged count, one flag) to dst:

@@ -89,7 +89,7 @@ def check_plan(m: MixEngine, begin, end, pattern_of, kinds, params: MixParams, b
         assert (i in whole_strings) == (end[i] - begin[i] <= split)
 
 
-@pytest.mark.parametrize("form,block", [("nibble", BLOCK), ("slice5", 2 * BLOCK)])
+@pytest.mark.parametrize("form,block", [("nibble", BLOCK), ("slice", 2 * BLOCK)])
 def test_plan_invariants(form, block):
     rng = np.random.default_rng(0)
     kinds = [DFA, NFA, DFA, NFA, NFA]
@@ -176,7 +176,7 @@ def _oracle(refs, data, offsets, pattern_of):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form", ["nibble", "slice5"])
+@pytest.mark.parametrize("form", ["nibble", "slice"])
 @pytest.mark.parametrize("params", [SMALL, MixParams(dfa_split=1 << 20, dfa_chunk=1 << 16, nfa_split=1 << 20,
                                                      nfa_chunk=1 << 16, ck=256)], ids=["small", "large"])
 def test_gpu_mix_matches_oracle(gpu_lib, params, form):
@@ -196,7 +196,7 @@ def test_gpu_mix_matches_oracle(gpu_lib, params, form):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form", ["nibble", "slice5"])
+@pytest.mark.parametrize("form", ["nibble", "slice"])
 def test_gpu_mix_graph_replay(gpu_lib, form):
     from trex_amd.device import DeviceBuffer, Stream
     rng = np.random.default_rng(2)
@@ -242,7 +242,7 @@ def test_gpu_mix_run_refuses_a_plan_for_the_other_form(gpu_lib):
     data = rng.integers(0x40, 0x60, size=int(offsets[-1]), dtype=np.uint8)
     m = MixEngine(pats)
     m.plan(offsets[:-1], offsets[1:], pattern_of, SMALL)
-    m.set_form("slice5")
+    m.set_form("slice")
     d_bytes = DeviceBuffer.from_array(data)
     n = len(offsets) - 1
     d_mask, d_acc = DeviceBuffer(8 * n), DeviceBuffer(n)
@@ -304,9 +304,9 @@ def test_gpu_tune_mix_chooses_a_form_and_the_choice_matches(gpu_lib, tmp_path, m
     stream = Stream()
     d_bytes = DeviceBuffer.from_array(data, stream)
     grid = {"dfa_split": (4096,), "dfa_chunk": (1024,), "nfa_split": (2048,), "nfa_chunk": (1024,), "ck": (64,),
-            "form": ("nibble", "slice5")}
+            "form": ("nibble", "slice")}
     params, info = tune_mix(pats, offsets[:-1], offsets[1:], pattern_of, d_bytes.ptr, stream, sample=300, grid=grid)
-    assert info["source"] == "tuned" and info["candidates"] == 2 and params.form in ("nibble", "slice5")
+    assert info["source"] == "tuned" and info["candidates"] == 2 and params.form in ("nibble", "slice")
     m = MixEngine(pats)
     mask, acc = m.match_arrays(data, offsets, pattern_of, params)
     ref_mask, ref_acc = _oracle(refs, data, offsets, pattern_of)

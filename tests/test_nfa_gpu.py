@@ -64,8 +64,9 @@ def test_regex_nfas_vs_re_and_oracle(gpu_lib, pattern):
                for _ in range(400)]
     data, offs = pack_strings(strings)
     mask, acc = _check(eng, data, offs)
-    eng.set_variant("window5")                     # 5-bit slice tables
-    assert np.array_equal(_check(eng, data, offs)[0], mask)
+    for v in ("window5", "window6"):               # 5-bit and 6/5-bit slice tables
+        eng.set_variant(v)
+        assert np.array_equal(_check(eng, data, offs)[0], mask)
     rx = re.compile(f"(?:{pattern})")
     assert [bool(x) for x in acc] == [rx.fullmatch(s) is not None for s in strings]
     assert [bool(x) for x in acc[:60]] == [simulate_nfa(nm, s) for s in strings[:60]]
@@ -76,7 +77,7 @@ def test_random_nfa64_32_symbols_both_variants(gpu_lib, seed):
     rng = np.random.default_rng(500 + seed)
     nm = random_nfa64(rng)
     eng = NfaEngine.from_matrices(nm)
-    assert eng.info()["variant"] == "window5"
+    assert eng.info()["variant"] == "window6"
     eng.set_variant("window")
     lengths = rng.integers(0, 600, 3000)
     lengths[:6] = [0, 1, 15, 16, 17, 33]
@@ -86,9 +87,9 @@ def test_random_nfa64_32_symbols_both_variants(gpu_lib, seed):
     eng.set_variant("bytes")
     second = _check(eng, data, offs)
     assert np.array_equal(first[0], second[0])
-    eng.set_variant("window5")
-    third = _check(eng, data, offs)
-    assert np.array_equal(first[0], third[0])
+    for v in ("window5", "window6"):
+        eng.set_variant(v)
+        assert np.array_equal(first[0], _check(eng, data, offs)[0])
     # spot-check the oracle against the set-of-states restatement
     for i in range(0, 3000, 500):
         b = bytes(data[offs[i]:offs[i + 1]])
@@ -106,7 +107,7 @@ def test_skip_bytes_general_map(gpu_lib):
     _check(eng, data, offs)
 
 
-@pytest.mark.parametrize("variant", ["window", "window5"])
+@pytest.mark.parametrize("variant", ["window", "window5", "window6"])
 def test_uniform_batch_bit_exact(gpu_lib, variant):
     rng = np.random.default_rng(3)
     nm = random_nfa64(rng)
@@ -125,11 +126,12 @@ def test_window5_skip_and_reject_bytes_inside_the_window(gpu_lib):
     b2s[0x40:0x5a] = np.arange(26)
     b2s[0x5a:0x5e] = -2                            # skip
     eng = NfaEngine(nm.successor_masks(), b2s, nm.start_mask(), nm.accept_bits())
-    assert eng.info()["variant"] == "window5"
+    assert eng.info()["variant"] == "window6"
     lengths = rng.integers(0, 400, 2000)
     lengths[:5] = [0, 1, 15, 16, 31]
     data, offs = ragged_batch(rng, lengths, 0x40, 0x60)
     data[rng.integers(0, len(data), 20)] = 0x21    # outside the window: rejects
     _check(eng, data, offs)
-    eng.set_variant("window")
-    _check(eng, data, offs)
+    for v in ("window6", "window"):
+        eng.set_variant(v)
+        _check(eng, data, offs)

@@ -34,7 +34,7 @@ MIX_GRID = {
     "nfa_split": (8 << 10, 32 << 10),
     "nfa_chunk": (8 << 10, 32 << 10),
     "ck": (64, 256),
-    "form": ("nibble", "slice5"),   # NFA tables and workgroup shape: 512 or 1024 threads
+    "form": ("nibble", "slice"),   # NFA tables and workgroup shape: 512 or 1024 threads
 }
 NEAR = 0.02   # relative timing difference on the sample treated as noise
 

@@ -288,4 +288,54 @@ __device__ __forceinline__ uint64_t nfa5_step(const char* __restrict__ tb, uint3
     return r0 | r1;
 }
 
+// ---------------------------------------------------------------------------
+// 6/5-bit slices: 12 lookups per byte. Each 32-bit half of the state is cut
+// into fields of 6, 6, 5, 5, 5 and 5 bits (no field crosses the halves):
+//
+//     G[c][r][v] = union of succ[c][b_r + i] over the bits i of v
+//
+// for the 12 fields r with first bit b_r. Rows of 6-bit fields hold 64
+// entries (512 bytes), rows of 5-bit fields 32 (256 bytes): 4,096 bytes per
+// class; class k's table starts at byte k * kClass6 with kClass6 = 4096 + 8,
+// so entry v of any row falls in bank pair (v + k + row offset / 8) mod 32.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kClass6 = 4096u + 8u;
+constexpr uint32_t kTab6Bytes = 32u * kClass6;
+
+template <int OFF, int W>
+__device__ __forceinline__ uint32_t field(uint32_t w) {
+    if constexpr (OFF == 0) {
+        return w & ((1u << W) - 1u);
+    } else if constexpr (OFF + W == 32) {
+        return w >> OFF;
+    } else {
+        uint32_t r;
+        asm("v_bfe_u32 %0, %1, %2, %3" : "=v"(r) : "v"(w), "n"(OFF), "n"(W));
+        return r;
+    }
+}
+
+__device__ __forceinline__ uint64_t nfa6_step(const char* __restrict__ tb, uint32_t k, uint64_t s) {
+    const uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32);
+    uint32_t base = k * kClass6;   // kept in a register: see nfa5_step
+    asm("" : "+v"(base));
+    const char* t = tb + base;
+    auto ld = [t](uint32_t off) { return *reinterpret_cast<const uint64_t*>(t + off); };
+    // row byte offsets: 0, 512, 1024, 1280, 1536, 1792 for a half, + 2048 for hi
+    uint64_t r0 = 0, r1 = 0;
+    r0 |= ld(0 + field<0, 6>(lo) * 8u);
+    r1 |= ld(512 + field<6, 6>(lo) * 8u);
+    r0 |= ld(1024 + field<12, 5>(lo) * 8u);
+    r1 |= ld(1280 + field<17, 5>(lo) * 8u);
+    r0 |= ld(1536 + field<22, 5>(lo) * 8u);
+    r1 |= ld(1792 + field<27, 5>(lo) * 8u);
+    r0 |= ld(2048 + field<0, 6>(hi) * 8u);
+    r1 |= ld(2560 + field<6, 6>(hi) * 8u);
+    r0 |= ld(3072 + field<12, 5>(hi) * 8u);
+    r1 |= ld(3328 + field<17, 5>(hi) * 8u);
+    r0 |= ld(3584 + field<22, 5>(hi) * 8u);
+    r1 |= ld(3840 + field<27, 5>(hi) * 8u);
+    return r0 | r1;
+}
+
 }  // namespace trexdev

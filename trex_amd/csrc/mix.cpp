@@ -27,9 +27,9 @@ using namespace trex;
 namespace {
 
 constexpr int kMixBlock = 512;      // workgroup of mix_persistent (nibble NFA tables)
-constexpr int kMixBlock5 = 1024;    // workgroup of mix_persistent5 (5-bit slice NFA tables)
+constexpr int kMixBlockSlice = 1024;   // workgroup of mix_persistent_slice (6/5-bit slice NFA tables)
 constexpr int kNfaClasses = 34;
-constexpr unsigned kTab5Bytes = 32u * (13u * 256u + 8u);   // 5-bit slice tables of the 32 window classes
+constexpr unsigned kTabSliceBytes = 32u * (4096u + 8u);   // 6/5-bit slice tables of the 32 window classes
 constexpr uint32_t kRoleDfaWhole = 0, kRoleDfaChunk = 1, kRoleNfaWhole = 2, kRoleNfaSpec = 3, kRoleNfaFixup = 4;
 constexpr int64_t kNfaWeight = 10;   // NFA-64 bytes cost about ten DFA-16 bytes (LDS lookups per byte)
 
@@ -112,8 +112,8 @@ struct trex_mix {
     std::vector<MixPattern> pats;
     bool has_device = false;
     MixPattern* d_pats = nullptr;
-    MixPattern* d_pats5 = nullptr;   // the same with the NFAs' 5-bit slice tables
-    bool can_slice5 = true;          // every NFA pattern has 5-bit slice tables
+    MixPattern* d_pats_slice = nullptr;   // the same with the NFAs' 6/5-bit slice tables
+    bool can_slice = true;                // every NFA pattern has slice tables
     int form = TREX_MIX_FORM_NIBBLE;
     Plan plan;
     // The plan's ten arrays, packed at 256-byte aligned offsets: built in
@@ -262,24 +262,24 @@ int trex_mix_create(const int32_t* kinds, const void* const* handles, int32_t n_
         }
         TREX_TRY(ensure_module());
     }
-    std::vector<MixPattern> pats5 = pats;
-    bool can5 = true;
+    std::vector<MixPattern> pats_slice = pats;
+    bool can_slice_all = true;
     for (int32_t i = 0; i < n_patterns && handles; ++i) {
         if (kinds[i] != TREX_MIX_NFA) continue;
         const trex_nfa* f = static_cast<const trex_nfa*>(handles[i]);
-        pats5[i].ntab = f->d_tab5;
-        pats5[i].ncls = nullptr;
-        can5 = can5 && f->d_tab5 != nullptr;
+        pats_slice[i].ntab = f->d_tab6;
+        pats_slice[i].ncls = nullptr;
+        can_slice_all = can_slice_all && f->d_tab6 != nullptr;
     }
     trex_mix* m = new trex_mix();
     m->n_patterns = n_patterns;
     m->kinds.assign(kinds, kinds + n_patterns);
     m->pats = pats;
     m->has_device = handles != nullptr;
-    m->can_slice5 = can5;
+    m->can_slice = can_slice_all;
     if (m->has_device) {
         int rc = upload((void**)&m->d_pats, pats.data(), pats.size() * sizeof(MixPattern));
-        if (!rc && can5) rc = upload((void**)&m->d_pats5, pats5.data(), pats5.size() * sizeof(MixPattern));
+        if (!rc && can_slice_all) rc = upload((void**)&m->d_pats_slice, pats_slice.data(), pats_slice.size() * sizeof(MixPattern));
         if (rc) { hipFree(m->d_pats); delete m; return rc; }
     }
     *out = m;
@@ -294,7 +294,7 @@ int trex_mix_destroy(trex_mix* m) {
     if (m->h_stage) hipHostFree(m->h_stage);
     for (uint8_t* p : m->h_retired) hipHostFree(p);
     hipFree(m->d_pats);
-    hipFree(m->d_pats5);
+    hipFree(m->d_pats_slice);
     for (DevBuf* b : {&m->d_plan, &m->d_words, &m->d_agg, &m->d_ckpt, &m->d_exit})
         release_async(*b);
     delete m;
@@ -317,7 +317,7 @@ int trex_mix_plan(trex_mix* m, const int64_t* begin, const int64_t* end, const i
     }
     drop_graph(m);
     Plan P;
-    P.block = m->form == TREX_MIX_FORM_SLICE5 ? kMixBlock5 : kMixBlock;
+    P.block = m->form == TREX_MIX_FORM_SLICE ? kMixBlockSlice : kMixBlock;
     P.ck = ck;
     P.n_ck = (int32_t)(nfa_chunk / ck);
     P.n_strings = n_strings;
@@ -463,7 +463,7 @@ int trex_mix_run(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, uint
     const Plan& Q = m->plan;
     if (Q.bfirst.empty()) return TREX_OK;
     if (!d_bytes || !d_out_mask || !d_out_accept) return fail(TREX_ERR_INVALID, "null buffer");
-    if (Q.block != (m->form == TREX_MIX_FORM_SLICE5 ? kMixBlock5 : kMixBlock))
+    if (Q.block != (m->form == TREX_MIX_FORM_SLICE ? kMixBlockSlice : kMixBlock))
         return fail(TREX_ERR_INVALID, "the plan's batches were cut for the other form: plan again after trex_mix_set_form");
     hipStream_t s = (hipStream_t)stream;
     TREX_TRY(upload_plan(m, s));
@@ -477,8 +477,8 @@ int trex_mix_run(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, uint
     }
     MixArgs a;
     a.bytes = d_bytes;
-    const bool s5 = m->form == TREX_MIX_FORM_SLICE5;
-    a.patterns = s5 ? m->d_pats5 : m->d_pats;
+    const bool sl = m->form == TREX_MIX_FORM_SLICE;
+    a.patterns = sl ? m->d_pats_slice : m->d_pats;
     uint8_t* pb = m->d_plan.as<uint8_t>();
     a.ibeg = reinterpret_cast<const int64_t*>(pb + m->off[0]);
     a.iend = reinterpret_cast<const int64_t*>(pb + m->off[1]);
@@ -505,20 +505,20 @@ int trex_mix_run(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, uint
     a.out_mask = d_out_mask;
     a.out_accept = d_out_accept;
     // persistent grid: by default two 512-thread workgroups per CU (70 KB of LDS each), or one
-    // 1024-thread workgroup (104 KB) with the 5-bit slice tables
-    const int64_t want = m->grid > 0 ? m->grid : (s5 ? 1 : 2) * (int64_t)device_cu_count();
+    // 1024-thread workgroup (128 KB) with the slice tables
+    const int64_t want = m->grid > 0 ? m->grid : (sl ? 1 : 2) * (int64_t)device_cu_count();
     const unsigned grid = (unsigned)std::min<int64_t>(want, (int64_t)Q.bfirst.size());
     void* args[] = {&a};
-    if (s5) return launch("mix_persistent5", grid, kMixBlock5, kTab5Bytes, s, args);
+    if (sl) return launch("mix_persistent_slice", grid, kMixBlockSlice, kTabSliceBytes, s, args);
     return launch("mix_persistent", grid, kMixBlock, kNfaClasses * 2048 + 256, s, args);
 }
 
 int trex_mix_set_form(trex_mix* m, int form) {
     if (!m) return fail(TREX_ERR_INVALID, "null handle");
-    if (form != TREX_MIX_FORM_NIBBLE && form != TREX_MIX_FORM_SLICE5)
-        return fail(TREX_ERR_INVALID, "form %d: use TREX_MIX_FORM_NIBBLE or TREX_MIX_FORM_SLICE5", form);
-    if (form == TREX_MIX_FORM_SLICE5 && m->has_device && !m->can_slice5)
-        return fail(TREX_ERR_INVALID, "an NFA pattern has no 5-bit slice tables");
+    if (form != TREX_MIX_FORM_NIBBLE && form != TREX_MIX_FORM_SLICE)
+        return fail(TREX_ERR_INVALID, "form %d: use TREX_MIX_FORM_NIBBLE or TREX_MIX_FORM_SLICE", form);
+    if (form == TREX_MIX_FORM_SLICE && m->has_device && !m->can_slice)
+        return fail(TREX_ERR_INVALID, "an NFA pattern has no slice tables");
     m->form = form;
     drop_graph(m);
     return TREX_OK;

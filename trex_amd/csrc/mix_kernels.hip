@@ -36,7 +36,7 @@
 #include "device_common.h"
 
 #define MIX_BLOCK 512
-#define MIX_BLOCK5 1024
+#define MIX_BLOCK_SLICE 1024
 
 namespace {
 
@@ -58,7 +58,7 @@ struct MixPattern {
     const uint8_t* t2;    // DFA pair rows [1024][16]
     const uint8_t* f2;    // DFA pair functions [1024] x uint64
     const uint8_t* acc;   // DFA accept [16]
-    const uint8_t* ntab;  // NFA window tables: [34][256] x uint64 (bank-swizzled) or 5-bit slices
+    const uint8_t* ntab;  // NFA window tables: [34][256] x uint64 (bank-swizzled) or 6/5-bit slices
     const uint8_t* ncls;  // NFA byte -> class [256]
 };
 
@@ -247,20 +247,20 @@ __device__ void dfa_chunk(const MixArgs& A, const MixPattern& P, const DfaLds& L
 
 // The NFA step over the tables of the launch's form: nibble tables (34
 // classes, reject and skip included, bank-swizzled) with a byte -> class map,
-// or 5-bit slice tables of the 32 window classes, where every byte enters the
+// or 6/5-bit slice tables of the 32 window classes, where every byte enters the
 // window check and steps with class byte & 31.
-template <bool S5>
+template <bool SLICE>
 struct NfaWalker {
     const void* tab;
     const uint8_t* cls;
     uint64_t s;
     WinCheck w;
     __device__ __forceinline__ uint64_t step(uint32_t k, uint64_t x) const {
-        if constexpr (S5) return nfa5_step(static_cast<const char*>(tab), k, x);
+        if constexpr (SLICE) return nfa6_step(static_cast<const char*>(tab), k, x);
         else return nfa_step(static_cast<const uint64_t*>(tab), k, x);
     }
     __device__ __forceinline__ void byte(uint32_t b) {
-        if constexpr (S5) {
+        if constexpr (SLICE) {
             w.add(b * 0x01010101u);
             s = step(b & 31u, s);
         } else {
@@ -284,10 +284,10 @@ struct NfaWalker {
     }
 };
 
-template <bool S5>
+template <bool SLICE>
 __device__ uint64_t nfa_walk(const uint8_t* __restrict__ bytes, int64_t p, int64_t end, uint64_t s,
                              const void* tab, const uint8_t* cls, uint32_t win4) {
-    NfaWalker<S5> nw{tab, cls, s, {}};
+    NfaWalker<SLICE> nw{tab, cls, s, {}};
     nw.w.init();
     const int64_t head_end = min(end, p + head_len(bytes, p));
     for (; p < head_end; ++p) nw.byte(bytes[p]);
@@ -313,12 +313,12 @@ __device__ uint64_t nfa_walk(const uint8_t* __restrict__ bytes, int64_t p, int64
 // position that is a multiple of ck (ck a multiple of 16) into ck_out[0..n_ck).
 // When `compare`, stops at the first such position whose set equals
 // ck_in[idx] and reports it through `met`.
-template <bool S5, bool RECORD, bool COMPARE>
+template <bool SLICE, bool RECORD, bool COMPARE>
 __device__ uint64_t nfa_walk_ck(const uint8_t* __restrict__ bytes, int64_t p, int64_t end, uint64_t s,
                                 const void* tab, const uint8_t* cls, uint32_t win4, int32_t ck,
                                 uint64_t* __restrict__ ck_out, const uint64_t* __restrict__ ck_in, int32_t n_ck,
                                 bool& met) {
-    NfaWalker<S5> nw{tab, cls, s, {}};
+    NfaWalker<SLICE> nw{tab, cls, s, {}};
     nw.w.init();
     met = false;
     const int64_t head_end = min(end, p + head_len(bytes, p));
@@ -349,9 +349,9 @@ __device__ uint64_t nfa_walk_ck(const uint8_t* __restrict__ bytes, int64_t p, in
 }
 
 // One persistent workgroup's loop. Dynamic LDS: the NFA tables of the form
-// (nibble: kNfaClasses * 2048 + 256 bytes; 5-bit slices: kTab5Bytes); the
+// (nibble: kNfaClasses * 2048 + 256 bytes; 6/5-bit slices: kTab6Bytes); the
 // DFA tables use the first 29 KB of the same space.
-template <bool S5>
+template <bool SLICE>
 __device__ __forceinline__ void mix_loop(const MixArgs& A)
 {
     extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
@@ -365,7 +365,7 @@ __device__ __forceinline__ void mix_loop(const MixArgs& A)
     dl.f2 = reinterpret_cast<const uint64_t*>(lds + 20480);   // 8 KB
     dl.acc = lds + 28672;                                     // 16 B
     const void* ntab = lds;
-    const uint8_t* ncls = S5 ? nullptr : lds + kNfaClasses * 2048;
+    const uint8_t* ncls = SLICE ? nullptr : lds + kNfaClasses * 2048;
 
     while (true) {
         __syncthreads();   // the previous batch is finished with LDS and s_batch
@@ -382,8 +382,8 @@ __device__ __forceinline__ void mix_loop(const MixArgs& A)
                 stage_table(lds + 4096, P.t2, 16384);
                 stage_table(lds + 20480, P.f2, 8192);
                 stage_table(lds + 28672, P.acc, 16);
-            } else if constexpr (S5) {
-                stage_table(lds, P.ntab, (int)kTab5Bytes);
+            } else if constexpr (SLICE) {
+                stage_table(lds, P.ntab, (int)kTab6Bytes);
             } else {
                 stage_table(lds, P.ntab, kNfaClasses * 2048);
                 stage_table(lds + kNfaClasses * 2048, P.ncls, 256);
@@ -406,14 +406,14 @@ __device__ __forceinline__ void mix_loop(const MixArgs& A)
             A.out_mask[A.ib[item]] = s;
             A.out_accept[A.ib[item]] = dl.acc[s];
         } else if (role == kRoleNfaWhole) {
-            const uint64_t s = nfa_walk<S5>(A.bytes, A.ibeg[item], A.iend[item], P.nstart, ntab, ncls, P.win4);
+            const uint64_t s = nfa_walk<SLICE>(A.bytes, A.ibeg[item], A.iend[item], P.nstart, ntab, ncls, P.win4);
             A.out_mask[A.ib[item]] = s;
             A.out_accept[A.ib[item]] = (s & P.naccept) != 0 ? 1 : 0;
         } else if (role == kRoleNfaSpec) {
             const bool first = (A.iflags[item] & 1u) != 0;
             const int32_t o = A.ic[item];
             bool met;
-            const uint64_t s = nfa_walk_ck<S5, true, false>(A.bytes, A.ibeg[item], A.iend[item],
+            const uint64_t s = nfa_walk_ck<SLICE, true, false>(A.bytes, A.ibeg[item], A.iend[item],
                                                             first ? P.nstart : ~0ull, ntab, ncls, P.win4, A.ck,
                                                             A.ckpt + (int64_t)o * A.n_ck, nullptr, A.n_ck, met);
             A.spec_exit[o] = s;
@@ -440,7 +440,7 @@ __device__ __forceinline__ void mix_loop(const MixArgs& A)
                 for (int64_t c = c0 + 1; c < c1 && s != 0; ++c) {
                     const int32_t o = A.ic[c];
                     bool met;
-                    const uint64_t t = nfa_walk_ck<S5, false, true>(A.bytes, A.ibeg[c], A.iend[c], s, ntab, ncls,
+                    const uint64_t t = nfa_walk_ck<SLICE, false, true>(A.bytes, A.ibeg[c], A.iend[c], s, ntab, ncls,
                                                                     P.win4, A.ck, nullptr,
                                                                     A.ckpt + (int64_t)o * A.n_ck, A.n_ck, met);
                     s = met ? A.spec_exit[o] : t;
@@ -462,10 +462,10 @@ mix_persistent(MixArgs A)
     mix_loop<false>(A);
 }
 
-// 5-bit slice NFA tables (13 reads per byte instead of 16): one 1024-thread
-// workgroup per CU (104 KB of LDS).
-extern "C" __global__ void __launch_bounds__(MIX_BLOCK5) __attribute__((amdgpu_waves_per_eu(4)))
-mix_persistent5(MixArgs A)
+// 6/5-bit slice NFA tables (12 reads per byte instead of 16): one
+// 1024-thread workgroup per CU (128 KB of LDS).
+extern "C" __global__ void __launch_bounds__(MIX_BLOCK_SLICE) __attribute__((amdgpu_waves_per_eu(4)))
+mix_persistent_slice(MixArgs A)
 {
     mix_loop<true>(A);
 }

@@ -68,6 +68,30 @@ std::vector<uint8_t> slice5_tables(const std::vector<const uint64_t*>& succ_of_c
     return out;
 }
 
+// 6/5-bit slice tables (layout of nfa64_window6): each 32-bit half of the
+// state cut into fields of 6, 6, 5, 5, 5, 5 bits; class k at byte k * kClass6,
+// field r's row at byte kRow6[r] of it, entry v = union of succ[b_r + i] over
+// the bits i of v.
+constexpr size_t kClass6 = 4096 + 8;
+constexpr int kBits6[12] = {6, 6, 5, 5, 5, 5, 6, 6, 5, 5, 5, 5};
+constexpr int kFirst6[12] = {0, 6, 12, 17, 22, 27, 32, 38, 44, 49, 54, 59};
+constexpr size_t kRow6[12] = {0, 512, 1024, 1280, 1536, 1792, 2048, 2560, 3072, 3328, 3584, 3840};
+
+std::vector<uint8_t> slice6_tables(const std::vector<const uint64_t*>& succ_of_class) {
+    std::vector<uint8_t> out(32 * kClass6, 0);
+    for (int k = 0; k < 32; ++k) {
+        const uint64_t* succ64 = succ_of_class[k];
+        for (int r = 0; r < 12; ++r)
+            for (int v = 0; v < (1 << kBits6[r]); ++v) {
+                uint64_t m = 0;
+                for (int i = 0; i < kBits6[r]; ++i)
+                    if (v & (1 << i)) m |= succ64[kFirst6[r] + i];
+                std::memcpy(out.data() + k * kClass6 + kRow6[r] + v * 8, &m, 8);
+            }
+    }
+    return out;
+}
+
 }  // namespace
 
 extern "C" {
@@ -135,10 +159,14 @@ int trex_nfa_create(const uint64_t* succ, int n_symbols, const int16_t* byte_to_
             }
             const std::vector<uint8_t> t5 = slice5_tables(src);
             rc = upload((void**)&h->d_tab5, t5.data(), t5.size());
+            if (!rc) {
+                const std::vector<uint8_t> t6 = slice6_tables(src);
+                rc = upload((void**)&h->d_tab6, t6.data(), t6.size());
+            }
         }
     }
     if (rc) { trex_nfa_destroy(h); return rc; }
-    h->variant = h->d_tab5 ? TREX_NFA_WINDOW5 : h->window_base >= 0 ? TREX_NFA_WINDOW : TREX_NFA_BYTES;
+    h->variant = h->d_tab6 ? TREX_NFA_WINDOW6 : h->window_base >= 0 ? TREX_NFA_WINDOW : TREX_NFA_BYTES;
     *out = h;
     return TREX_OK;
 }
@@ -151,6 +179,7 @@ int trex_nfa_destroy(trex_nfa* h) {
     hipFree(h->d_tab_win);
     hipFree(h->d_cls_win);
     hipFree(h->d_tab5);
+    hipFree(h->d_tab6);
     h->in_bytes.release();
     h->in_offsets.release();
     h->out_mask.release();
@@ -171,7 +200,7 @@ int trex_nfa_info(const trex_nfa* h, int* variant, int* window_base) {
 int trex_nfa_set_variant(trex_nfa* h, int variant) {
     if (!h) return fail(TREX_ERR_INVALID, "null handle");
     if (variant == TREX_NFA_BYTES || (variant == TREX_NFA_WINDOW && h->window_base >= 0) ||
-        (variant == TREX_NFA_WINDOW5 && h->d_tab5)) {
+        (variant == TREX_NFA_WINDOW5 && h->d_tab5) || (variant == TREX_NFA_WINDOW6 && h->d_tab6)) {
         h->variant = variant;
         return TREX_OK;
     }
@@ -184,14 +213,16 @@ int trex_nfa_match_ranges(trex_nfa* h, const uint8_t* d_bytes, const int64_t* d_
     if (n_strings < 0) return fail(TREX_ERR_INVALID, "n_strings < 0");
     if (n_strings == 0) return TREX_OK;
     if (!d_bytes || !d_begin || !d_end || !d_final_mask || !d_accept) return fail(TREX_ERR_INVALID, "null buffer");
-    if (h->variant == TREX_NFA_WINDOW5) {
+    if (h->variant == TREX_NFA_WINDOW5 || h->variant == TREX_NFA_WINDOW6) {
+        const bool six = h->variant == TREX_NFA_WINDOW6;
         const int64_t grid5 = (n_strings + kNfa5Block - 1) / kNfa5Block;
         if (grid5 > 0x7fffffff) return fail(TREX_ERR_CAPACITY, "too many strings");
-        uint8_t* tab5 = h->d_tab5;
+        uint8_t* tab5 = six ? h->d_tab6 : h->d_tab5;
         uint32_t win4 = (uint32_t)h->window_base * 0x01010101u;
         uint64_t start = h->start, accept = h->accept;
         void* args[] = {&d_bytes, &d_begin, &d_end, &n_strings, &tab5, &win4, &start, &accept, &d_final_mask, &d_accept};
-        return launch("nfa64_window5", (unsigned)grid5, kNfa5Block, 0, (hipStream_t)stream, args);
+        return launch(six ? "nfa64_window6" : "nfa64_window5", (unsigned)grid5, kNfa5Block, 0, (hipStream_t)stream,
+                      args);
     }
     const int64_t grid64 = (n_strings + kNfaBlock - 1) / kNfaBlock;
     if (grid64 > 0x7fffffff) return fail(TREX_ERR_CAPACITY, "too many strings");

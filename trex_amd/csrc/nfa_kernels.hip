@@ -104,14 +104,17 @@ __device__ __forceinline__ void nfa_kernel(const uint8_t* __restrict__ bytes, co
 
 #define TREX_NFA5_BLOCK 1024
 
-__device__ __forceinline__ void nfa5_kernel(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
+template <bool SIX>
+__device__ __forceinline__ void nfa_slice_kernel(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
                                             const int64_t* __restrict__ endp, int64_t n_strings,
                                             const uint8_t* __restrict__ tabg, uint32_t win4, uint64_t start,
                                             uint64_t accept, uint64_t* __restrict__ out_mask,
                                             uint8_t* __restrict__ out_accept)
 {
-    __shared__ __attribute__((aligned(16))) char tb[kTab5Bytes];
-    stage_table(reinterpret_cast<uint8_t*>(tb), tabg, (int)kTab5Bytes);
+    constexpr uint32_t kBytes = SIX ? kTab6Bytes : kTab5Bytes;
+    __shared__ __attribute__((aligned(16))) char tb[kBytes];
+    stage_table(reinterpret_cast<uint8_t*>(tb), tabg, (int)kBytes);
+    auto step = [&](uint32_t k, uint64_t x) { return SIX ? nfa6_step(tb, k, x) : nfa5_step(tb, k, x); };
     __syncthreads();
 
     const int64_t sid = (int64_t)blockIdx.x * TREX_NFA5_BLOCK + threadIdx.x;
@@ -123,17 +126,17 @@ __device__ __forceinline__ void nfa5_kernel(const uint8_t* __restrict__ bytes, c
     w.init();
     auto byte = [&](uint32_t b) {
         w.add(b * 0x01010101u);
-        s = nfa5_step(tb, b & 31u, s);
+        s = step(b & 31u, s);
     };
     const int64_t head_end = min(end, p + head_len(bytes, p));
     for (; p < head_end; ++p) byte(bytes[p]);
     const int64_t nvec = (end - p) >> 4;
     auto word = [&](uint32_t x) {
         w.add(x);
-        s = nfa5_step(tb, x & 31u, s);
-        s = nfa5_step(tb, (x >> 8) & 31u, s);
-        s = nfa5_step(tb, (x >> 16) & 31u, s);
-        s = nfa5_step(tb, (x >> 24) & 31u, s);
+        s = step(x & 31u, s);
+        s = step((x >> 8) & 31u, s);
+        s = step((x >> 16) & 31u, s);
+        s = step((x >> 24) & 31u, s);
     };
     stream_body<2>(reinterpret_cast<const uint4*>(bytes + p), nvec,
                    [&](uint4 v) { word(v.x); word(v.y); word(v.z); word(v.w); });
@@ -153,7 +156,18 @@ nfa64_window5(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ beg
               int64_t n_strings, const uint8_t* __restrict__ tabg, uint32_t win4, uint64_t start, uint64_t accept,
               uint64_t* __restrict__ out_mask, uint8_t* __restrict__ out_accept)
 {
-    nfa5_kernel(bytes, begin, end, n_strings, tabg, win4, start, accept, out_mask, out_accept);
+    nfa_slice_kernel<false>(bytes, begin, end, n_strings, tabg, win4, start, accept, out_mask, out_accept);
+}
+
+// tabg: 32 window classes of 6/5-bit slice tables, class k at byte k * 4104
+// (12 rows: 64, 64, 32, 32, 32, 32 uint64 entries per half; 8 bytes of
+// padding); kTab6Bytes in total.
+extern "C" __global__ void __launch_bounds__(TREX_NFA5_BLOCK)
+nfa64_window6(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin, const int64_t* __restrict__ end,
+              int64_t n_strings, const uint8_t* __restrict__ tabg, uint32_t win4, uint64_t start, uint64_t accept,
+              uint64_t* __restrict__ out_mask, uint8_t* __restrict__ out_accept)
+{
+    nfa_slice_kernel<true>(bytes, begin, end, n_strings, tabg, win4, start, accept, out_mask, out_accept);
 }
 
 // tabg: [n_classes][16][16] uint64 nibble tables; clsg: [256] class of every

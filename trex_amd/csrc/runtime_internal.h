@@ -80,6 +80,7 @@ struct trex_nfa {
     uint64_t* d_tab_win = nullptr;  // [34][16][16]: window bytes base+0..31, reject, skip
     uint8_t* d_cls_win = nullptr;   // [256]
     uint8_t* d_tab5 = nullptr;      // 32 window classes of 5-bit slice tables, class k at byte k * 3336
+    uint8_t* d_tab6 = nullptr;      // 32 window classes of 6/5-bit slice tables, class k at byte k * 4104
     trex::DevBuf in_bytes, in_offsets, out_mask, out_accept;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};

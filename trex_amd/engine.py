@@ -331,7 +331,7 @@ class DeviceBatch:
             b.free()
 
 
-NFA_VARIANTS = {"bytes": 1, "window": 2, "window5": 3}
+NFA_VARIANTS = {"bytes": 1, "window": 2, "window5": 3, "window6": 4}
 NFA_VARIANT_NAMES = {v: k for k, v in NFA_VARIANTS.items()}
 
 

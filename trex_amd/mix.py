@@ -30,7 +30,7 @@ BLOCK = 512   # lane items per work batch (one workgroup)
 @dataclass
 class MixParams:
     """Split lengths, chunk lengths and checkpoint spacing of a plan, and the
-    NFA table form ("nibble" / "slice5"; None keeps the engine's form)."""
+    NFA table form ("nibble" / "slice"; None keeps the engine's form)."""
     dfa_split: int = 256 << 10
     dfa_chunk: int = 64 << 10
     nfa_split: int = 32 << 10
@@ -42,7 +42,7 @@ class MixParams:
         return (int(self.dfa_split), int(self.dfa_chunk), int(self.nfa_split), int(self.nfa_chunk), int(self.ck))
 
 
-MIX_FORMS = {"nibble": 0, "slice5": 1}
+MIX_FORMS = {"nibble": 0, "slice": 1}
 
 
 class MixEngine:
@@ -109,12 +109,12 @@ class MixEngine:
 
     def set_grid(self, workgroups: int) -> None:
         """Persistent workgroups of later runs (0: two per compute unit, one
-        with the "slice5" form)."""
+        with the "slice" form)."""
         _lib.call("trex_mix_set_grid", self.handle, int(workgroups))
 
     def set_form(self, form: str) -> None:
         """NFA table form: "nibble" (default, 16 LDS reads per NFA byte) or
-        "slice5" (13 reads, 1024-thread workgroups). Call before :meth:`plan`."""
+        "slice" (12 reads, 1024-thread workgroups). Call before :meth:`plan`."""
         _lib.call("trex_mix_set_form", self.handle, MIX_FORMS[form])
 
     def status_to(self, dst: int, stream: Stream | None = None) -> None:
