@@ -52,7 +52,8 @@ def test_kernel_module_built_for_gfx950():
         assert b"gfx950" in data, o
         blob += data
     for kernel in [b"dfa16_k2w_pf16", b"dfa16_k2f_pf16", b"dfa16_k1", b"dfa_k1", b"copy_u4", b"pnfa64_f16x2",
-                   b"pnfa64_bf16x2"]:
+                   b"pnfa64_bf16x2", b"dfa16_multi_k2f_p4_il", b"dfa16_multi_k2w_p8", b"nfa64_window6",
+                   b"nfa64_window5", b"mix_persistent_slice", b"pnfa64_chunk_f16x2", b"pnfa64_combine"]:
         assert kernel in blob
 
 
