@@ -89,6 +89,7 @@ class PackedEngine:
             self._keep[ord(c)] = True
         self._engines = None
         self._multi = None
+        self._stream = None
         if use_gpu:
             self._init_gpu()
 
@@ -110,14 +111,15 @@ class PackedEngine:
     def _prepare(self, strings: Sequence):
         """Pack the strings and delete bytes outside every pattern's alphabet."""
         from trex_amd.engine import pack_strings
-        data, offsets = pack_strings(strings)
+        return self._delete_outside(*pack_strings(strings))
+
+    def _delete_outside(self, data: np.ndarray, offsets: np.ndarray):
         keep = self._keep[data]
-        if not keep.all():
-            kept = np.zeros(len(data) + 1, np.int64)
-            np.cumsum(keep, out=kept[1:])
-            data = np.ascontiguousarray(data[keep])
-            offsets = kept[offsets]
-        return data, offsets
+        if keep.all():
+            return data, offsets
+        kept = np.zeros(len(data) + 1, np.int64)
+        np.cumsum(keep, out=kept[1:])
+        return np.ascontiguousarray(data[keep]), kept[offsets]
 
     def _run(self, data: np.ndarray, offsets: np.ndarray) -> tuple:
         """(accept uint8 [P, n], kernel_ms) on the GPU."""
@@ -128,7 +130,9 @@ class PackedEngine:
         P = self._n_patterns
         batch = DeviceBatch(data, offsets, layout="csr")
         d_f, d_a = DeviceBuffer(P * n), DeviceBuffer(P * n)
-        stream = Stream()
+        if self._stream is None:
+            self._stream = Stream()
+        stream = self._stream
         e0, e1 = Event(), Event()
         try:
             e0.record(stream)
@@ -167,12 +171,7 @@ class PackedEngine:
     def match_arrays(self, data: np.ndarray, offsets: np.ndarray) -> np.ndarray:
         """CSR batch (uint8 bytes, int64 offsets) -> accept uint8 [P, n]."""
         from trex_amd.engine import _as_csr
-        data, offsets = _as_csr(data=data, offsets=offsets)
-        keep = self._keep[data]
-        if not keep.all():
-            kept = np.zeros(len(data) + 1, np.int64)
-            np.cumsum(keep, out=kept[1:])
-            data, offsets = np.ascontiguousarray(data[keep]), kept[offsets]
+        data, offsets = self._delete_outside(*_as_csr(data=data, offsets=offsets))
         if len(offsets) == 1:
             return np.zeros((self._n_patterns, 0), np.uint8)
         return self._run(data, offsets)[0]
