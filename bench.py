@@ -425,14 +425,14 @@ def multi_leg(args, engines, batch, stream, per_pattern_ms: float, nbytes: int):
         return [a.elapsed_ms(b) for a, b in evs]
 
     if args.multi_sweep:
-        for form in ("k2f", "k2w"):
+        for form in ("k2f", "k2fp", "k2w"):
             multi.set_form(form)
             for group in (1, 2, 3, 4, 5, 6, 7, 8):
                 multi.set_group(group)
                 ms = float(np.median(timed(max(3, args.steps))))
                 log(f"multi sweep: form {form} group {group}: {ms:.4f} ms per 8 patterns, "
                     f"{P * nbytes / (ms * 1e-3) / 1e9:.0f} Gchars/s, {per_pattern_ms / ms:.2f}x per-pattern launches")
-        multi.set_form("k2f")
+        multi.set_form("k2fp")
         multi.set_group(4)
     ms = timed(max(3, args.steps))
     avg = float(np.mean(ms))

@@ -167,8 +167,11 @@ typedef struct trex_dfa_multi trex_dfa_multi;
  * the pattern). */
 int trex_dfa_multi_create(trex_dfa* const* patterns, int n_patterns, trex_dfa_multi** out);
 int trex_dfa_multi_destroy(trex_dfa_multi* m);
-/* Table form: TREX_DFA_K2_WINDOW_F (default, 8 KB of LDS per pattern) or
+/* Table form: TREX_DFA_MULTI_K2F_PAIRED (default: the pair functions of two
+ * patterns side by side, one 16-byte LDS read for both; rebuilt when the
+ * group changes), TREX_DFA_K2_WINDOW_F (8 KB of LDS per pattern) or
  * TREX_DFA_K2_WINDOW (16 KB per pattern). */
+#define TREX_DFA_MULTI_K2F_PAIRED 7
 int trex_dfa_multi_set_form(trex_dfa_multi* m, int form);
 /* Patterns advanced per launch at most, 1..8 (default 4); the patterns are
  * split evenly over ceil(n_patterns / group) launches. */

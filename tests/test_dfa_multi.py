@@ -1,7 +1,7 @@
 """Multi-pattern DFA-16 pass (trex_dfa_multi_*): every (pattern, string)
 result equals the C oracle's table walk and the pattern's own single-pattern
 kernel, bit for bit, over ragged and empty strings, bytes outside the
-patterns' windows, patterns with different windows, every layout, both table
+patterns' windows, patterns with different windows, every layout, all three table
 forms and every group size."""
 
 import numpy as np
@@ -59,7 +59,7 @@ def test_multi_binding_declared():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("layout", ["csr", "staggered", "interleaved"])
-@pytest.mark.parametrize("form", ["k2f", "k2w"])
+@pytest.mark.parametrize("form", ["k2f", "k2w", "k2fp"])
 def test_gpu_multi_equals_oracle(gpu_lib, layout, form):
     from trex_amd.engine import DfaEngine, DfaMultiEngine
     rng = np.random.default_rng(101 + len(layout) + len(form))
@@ -97,7 +97,7 @@ def test_gpu_multi_every_launch_width(gpu_lib, n_patterns):
     offsets = offsets + 3                                   # no string starts 16-byte aligned
     data = np.concatenate([np.zeros(3, np.uint8), data])
     want_f, want_a = _oracle(tables, data, offsets)
-    for form in ("k2f", "k2w"):
+    for form in ("k2f", "k2w", "k2fp"):
         multi = DfaMultiEngine([DfaEngine(t) for t in tables])
         multi.set_form(form)
         assert multi.info() == {"n_patterns": n_patterns, "form": form, "group": 4}

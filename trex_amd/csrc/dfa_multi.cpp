@@ -6,14 +6,16 @@
 // byte table, so it does not depend on the member handles after creation.
 // A dispatch launches ceil(n_patterns / group) kernels, the patterns split
 // as evenly as possible among them; each reads the batch once and advances
-// its patterns. The default group of 4 is the measured optimum on gfx950:
-// the kernels are bound by LDS bank conflicts of the table reads, and four
-// patterns (33 KB of tables) still leave room for four workgroups per CU.
+// its patterns. The kernels are bound by LDS bank conflicts of the table
+// reads; the default paired form reads two patterns' pair functions with one
+// 16-byte read, and the default group of 4 (33 KB of tables) leaves room for
+// four workgroups per CU.
 
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
 #include <cstdio>
+#include <utility>
 #include <vector>
 
 #include "runtime_internal.h"
@@ -39,14 +41,60 @@ struct MultiParams {
 
 struct trex_dfa_multi {
     int n = 0;
-    int form = TREX_DFA_K2_WINDOW_F;
+    int form = TREX_DFA_MULTI_K2F_PAIRED;
     int group = kMultiGroup;
     uint8_t* d_f2 = nullptr;   // [n][1024] uint64 pair functions
     uint8_t* d_f1 = nullptr;   // [n][32] uint64 single-symbol functions
     uint8_t* d_t2 = nullptr;   // [n][1024][16] pair walk
     uint8_t* d_t1 = nullptr;   // [n][32][16] single-symbol walk
     std::vector<uint32_t> win4, start, dead, accept;
+    std::vector<uint64_t> h_f2;   // [n][1024] pair functions (host copy, for the paired tables)
+    uint8_t* d_fp = nullptr;      // paired tables of the current grouping, launch after launch
+    std::vector<size_t> fp_off;   // byte offset of each launch's paired tables in d_fp
+    int fp_group = -1;            // the group d_fp was built for
 };
+
+namespace {
+
+// Launch l of a dispatch covers patterns [first, first + size): the patterns
+// split evenly over ceil(n / group) launches (sizes differ by at most one).
+std::vector<std::pair<int, int>> launches_of(int n, int group) {
+    std::vector<std::pair<int, int>> out;
+    const int count = (n + group - 1) / group;
+    for (int l = 0, p0 = 0; l < count; ++l) {
+        const int P = (n - p0) / (count - l);
+        out.push_back({p0, P});
+        p0 += P;
+    }
+    return out;
+}
+
+// Paired tables for the current grouping: per launch, (P + 1) / 2 tables of
+// 1024 x 16 bytes, entry x = (f2[a][x], f2[b][x]) for the launch's patterns
+// a = first + 2j, b = a + 1 (the identity when b is past the launch).
+int build_paired(trex_dfa_multi* m) {
+    constexpr uint64_t kIdentity = 0xfedcba9876543210ull;   // nibble s = s
+    std::vector<uint64_t> host;
+    m->fp_off.clear();
+    for (const auto& [p0, P] : launches_of(m->n, m->group)) {
+        m->fp_off.push_back(host.size() * 8);
+        for (int j = 0; j < (P + 1) / 2; ++j) {
+            const int a = p0 + 2 * j, b = a + 1;
+            for (int x = 0; x < 1024; ++x) {
+                host.push_back(m->h_f2[(size_t)a * 1024 + x]);
+                host.push_back(b < p0 + P ? m->h_f2[(size_t)b * 1024 + x] : kIdentity);
+            }
+        }
+    }
+    hipFree(m->d_fp);
+    m->d_fp = nullptr;
+    m->fp_group = -1;
+    TREX_TRY(upload((void**)&m->d_fp, host.data(), host.size() * 8));
+    m->fp_group = m->group;
+    return TREX_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -56,6 +104,7 @@ int trex_dfa_multi_destroy(trex_dfa_multi* m) {
     hipFree(m->d_f1);
     hipFree(m->d_t2);
     hipFree(m->d_t1);
+    hipFree(m->d_fp);
     delete m;
     return TREX_OK;
 }
@@ -82,6 +131,7 @@ int trex_dfa_multi_create(trex_dfa* const* patterns, int n_patterns, trex_dfa_mu
     if (e == hipSuccess) e = hipMalloc((void**)&m->d_t2, n * 1024 * 16);
     if (e != hipSuccess) rc = hip_fail(e, "hipMalloc(multi-pattern tables)");
     std::vector<uint64_t> f1(n * 32, 0);
+    m->h_f2.resize(n * 1024);
     std::vector<uint8_t> t1w(n * 32 * 16, 0), t1(256 * 16), acc(16);
     for (int i = 0; i < n_patterns && !rc; ++i) {
         const trex_dfa* h = patterns[i];
@@ -89,6 +139,8 @@ int trex_dfa_multi_create(trex_dfa* const* patterns, int n_patterns, trex_dfa_mu
         if (e == hipSuccess) e = hipMemcpy(m->d_t2 + (size_t)i * 1024 * 16, h->d_t2, 1024 * 16, hipMemcpyDeviceToDevice);
         if (e == hipSuccess) e = hipMemcpy(t1.data(), h->d_t1, t1.size(), hipMemcpyDeviceToHost);
         if (e == hipSuccess) e = hipMemcpy(acc.data(), h->d_acc, acc.size(), hipMemcpyDeviceToHost);
+        if (e == hipSuccess)
+            e = hipMemcpy(m->h_f2.data() + (size_t)i * 1024, h->d_f2, 1024 * 8, hipMemcpyDeviceToHost);
         if (e != hipSuccess) { rc = hip_fail(e, "hipMemcpy(multi-pattern tables)"); break; }
         const int base = h->window_base;
         for (int c = 0; c < 32; ++c)
@@ -106,6 +158,7 @@ int trex_dfa_multi_create(trex_dfa* const* patterns, int n_patterns, trex_dfa_mu
     }
     if (!rc) rc = upload((void**)&m->d_f1, f1.data(), f1.size() * sizeof(uint64_t));
     if (!rc) rc = upload((void**)&m->d_t1, t1w.data(), t1w.size());
+    if (!rc) rc = build_paired(m);
     if (rc) { trex_dfa_multi_destroy(m); return rc; }
     *out = m;
     return TREX_OK;
@@ -113,9 +166,11 @@ int trex_dfa_multi_create(trex_dfa* const* patterns, int n_patterns, trex_dfa_mu
 
 int trex_dfa_multi_set_form(trex_dfa_multi* m, int form) {
     if (!m) return fail(TREX_ERR_INVALID, "null handle");
-    if (form == TREX_DFA_AUTO) form = TREX_DFA_K2_WINDOW_F;
-    if (form != TREX_DFA_K2_WINDOW_F && form != TREX_DFA_K2_WINDOW)
-        return fail(TREX_ERR_INVALID, "form %d: use TREX_DFA_K2_WINDOW_F or TREX_DFA_K2_WINDOW", form);
+    if (form == TREX_DFA_AUTO) form = TREX_DFA_MULTI_K2F_PAIRED;
+    if (form != TREX_DFA_K2_WINDOW_F && form != TREX_DFA_K2_WINDOW && form != TREX_DFA_MULTI_K2F_PAIRED)
+        return fail(TREX_ERR_INVALID, "form %d: use TREX_DFA_K2_WINDOW_F, TREX_DFA_K2_WINDOW or "
+                    "TREX_DFA_MULTI_K2F_PAIRED", form);
+    if (form == TREX_DFA_MULTI_K2F_PAIRED && m->fp_group != m->group) TREX_TRY(build_paired(m));
     m->form = form;
     return TREX_OK;
 }
@@ -124,6 +179,7 @@ int trex_dfa_multi_set_group(trex_dfa_multi* m, int group) {
     if (!m) return fail(TREX_ERR_INVALID, "null handle");
     if (group < 1 || group > kMultiMax) return fail(TREX_ERR_INVALID, "group=%d outside 1..%d", group, kMultiMax);
     m->group = group;
+    if (m->form == TREX_DFA_MULTI_K2F_PAIRED && m->fp_group != m->group) TREX_TRY(build_paired(m));
     return TREX_OK;
 }
 
@@ -148,11 +204,13 @@ int multi_dispatch(trex_dfa_multi* m, const uint8_t* d_bytes, const int64_t* d_b
     const int64_t grid64 = (n_strings + kBlock - 1) / kBlock;
     if (grid64 > 0x7fffffff) return fail(TREX_ERR_CAPACITY, "too many strings");
     const unsigned grid = (unsigned)grid64;
-    const bool f = m->form == TREX_DFA_K2_WINDOW_F;
+    const bool paired = m->form == TREX_DFA_MULTI_K2F_PAIRED;
+    const bool f = m->form == TREX_DFA_K2_WINDOW_F || paired;
     const size_t pair_bytes = f ? 1024 * 8 : 1024 * 16, sym_bytes = f ? 32 * 8 : 32 * 16;
-    const int launches = (m->n + m->group - 1) / m->group;
-    for (int l = 0, p0 = 0; l < launches; ++l) {
-        const int P = (m->n - p0) / (launches - l);   // even split: sizes differ by at most one
+    if (paired && m->fp_group != m->group) return fail(TREX_ERR_INVALID, "paired tables not built");
+    const auto launches = launches_of(m->n, m->group);
+    for (size_t l = 0; l < launches.size(); ++l) {
+        const auto [p0, P] = launches[l];
         MultiParams prm{};
         for (int k = 0; k < P; ++k) {
             prm.win4[k] = m->win4[p0 + k];
@@ -160,15 +218,15 @@ int multi_dispatch(trex_dfa_multi* m, const uint8_t* d_bytes, const int64_t* d_b
             prm.dead[k] = m->dead[p0 + k];
             prm.accept[k] = m->accept[p0 + k];
         }
-        const uint8_t* pairs = (f ? m->d_f2 : m->d_t2) + (size_t)p0 * pair_bytes;
+        const uint8_t* pairs = paired ? m->d_fp + m->fp_off[l] : (f ? m->d_f2 : m->d_t2) + (size_t)p0 * pair_bytes;
         const uint8_t* syms = (f ? m->d_f1 : m->d_t1) + (size_t)p0 * sym_bytes;
         uint8_t* out_s = d_final_state + (size_t)p0 * (size_t)n_strings;
         uint8_t* out_a = d_accept + (size_t)p0 * (size_t)n_strings;
         void* args[] = {&d_bytes, &d_begin, &d_end, &n_strings, &pairs, &syms, &prm, &out_s, &out_a};
         char name[48];
-        std::snprintf(name, sizeof(name), "dfa16_multi_%s_p%d%s", f ? "k2f" : "k2w", P, il ? "_il" : "");
+        std::snprintf(name, sizeof(name), "dfa16_multi_%s_p%d%s", paired ? "k2fp" : f ? "k2f" : "k2w", P,
+                      il ? "_il" : "");
         TREX_TRY(launch(name, grid, kBlock, 0, (hipStream_t)stream, args));
-        p0 += P;
     }
     return TREX_OK;
 }

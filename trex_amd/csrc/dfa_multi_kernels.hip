@@ -151,6 +151,85 @@ __device__ __forceinline__ void multi_kernel(const uint8_t* __restrict__ bytes, 
     }
 }
 
+// Paired form: patterns 2j and 2j + 1 of a launch share one table, entry x =
+// (f2_{2j}[x], f2_{2j+1}[x]) in 16 bytes, so one ds_read_b128 serves both
+// (an odd launch's last pattern is paired with the identity). A 16-byte read
+// occupies a whole quad of banks; random entries spread over the 16 quads.
+template <int P, int PF, bool IL>
+__device__ __forceinline__ void multi_kernel_paired(const uint8_t* __restrict__ bytes,
+                                                    const int64_t* __restrict__ begin,
+                                                    const int64_t* __restrict__ endp, int64_t n_strings,
+                                                    const uint8_t* __restrict__ pairg,
+                                                    const uint8_t* __restrict__ symg, const MultiParams& prm,
+                                                    uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
+{
+    constexpr int PP = (P + 1) / 2;
+    __shared__ __attribute__((aligned(16))) uint8_t tp[PP * 16384];
+    __shared__ __attribute__((aligned(16))) uint8_t ts[P * MultiF::kSymBytes];
+    stage_table(tp, pairg, PP * 16384);
+    stage_table(ts, symg, P * MultiF::kSymBytes);
+    __syncthreads();
+
+    const int64_t sid = (int64_t)blockIdx.x * TREX_MULTI_BLOCK + threadIdx.x;
+    if (sid >= n_strings) return;
+    int64_t p = begin[sid];
+    const int64_t end = endp[sid];
+    uint32_t s[2 * PP];
+#pragma unroll
+    for (int k = 0; k < 2 * PP; ++k) s[k] = k < P ? prm.start[k] : 0u;
+    WinCheck w; w.init();
+
+    auto pstep = [&](uint32_t x) {
+#pragma unroll
+        for (int j = 0; j < PP; ++j) {
+            const uint4 v = reinterpret_cast<const uint4*>(tp + j * 16384)[x];
+            const uint64_t f0 = (uint64_t)v.x | ((uint64_t)v.y << 32);
+            const uint64_t f1 = (uint64_t)v.z | ((uint64_t)v.w << 32);
+            s[2 * j] = (uint32_t)(f0 >> (s[2 * j] << 2)) & 15u;
+            s[2 * j + 1] = (uint32_t)(f1 >> (s[2 * j + 1] << 2)) & 15u;
+        }
+    };
+    auto word = [&](uint32_t x) {
+        w.add(x);
+        pstep(pair_lo(x));
+        pstep(pair_hi(x));
+    };
+    auto vec = [&](uint4 v) { word(v.x); word(v.y); word(v.z); word(v.w); };
+    auto byte = [&](uint32_t b) {
+        w.add(b * 0x01010101u);
+#pragma unroll
+        for (int k = 0; k < P; ++k) s[k] = MultiF::sym(ts + k * MultiF::kSymBytes, s[k], b & 31u);
+    };
+
+    if constexpr (IL) {
+        const int64_t len = end - p;
+        const int64_t nvec = len >> 4;
+        stream_body_nt<PF, kIlGroup>(reinterpret_cast<const uint4*>(bytes + p), nvec, vec);
+        const uint8_t* tail = bytes + p + nvec * (16 * kIlGroup);
+        for (int t = 0; t < (int)(len & 15); ++t) byte(tail[t]);
+    } else {
+        const int64_t head_end = min(end, p + head_len(bytes, p));
+        for (; p < head_end; ++p) byte(bytes[p]);
+        const int64_t nvec = (end - p) >> 4;
+        stream_body<PF>(reinterpret_cast<const uint4*>(bytes + p), nvec, vec);
+        p += nvec << 4;
+        for (; p < end; ++p) byte(bytes[p]);
+    }
+
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const uint32_t q = w.outside(prm.win4[k]) ? prm.dead[k] : s[k];
+        out_state[(int64_t)k * n_strings + sid] = (uint8_t)q;
+        out_accept[(int64_t)k * n_strings + sid] = (uint8_t)((prm.accept[k] >> q) & 1u);
+    }
+}
+
+template <int P>
+constexpr int paired_waves() {
+    const int blocks = (160 * 1024) / (((P + 1) / 2) * 16384 + P * MultiF::kSymBytes);
+    return blocks < 1 ? 1 : (blocks > 4 ? 4 : blocks);
+}
+
 // Waves per SIMD the LDS footprint allows (160 KB per CU, one wave per SIMD
 // per workgroup), at most 4: the register budget the compiler is held to.
 template <class Form, int P>
@@ -183,3 +262,20 @@ constexpr int multi_waves() {
 
 TREX_MULTI_KERNELS(MultiF, k2f)
 TREX_MULTI_KERNELS(MultiW, k2w)
+
+// Paired form. pairg: [(P + 1) / 2][1024] x 16 bytes, symg: [P][32] uint64.
+#define TREX_PAIRED_KERNEL(NAME, P, IL)                                                                        \
+    extern "C" __global__ void __launch_bounds__(TREX_MULTI_BLOCK)                                             \
+    __attribute__((amdgpu_waves_per_eu(paired_waves<P>())))                                                   \
+    NAME(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin, const int64_t* __restrict__ end, \
+         int64_t n_strings, const uint8_t* __restrict__ pairg, const uint8_t* __restrict__ symg, MultiParams prm, \
+         uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept) {                                 \
+        multi_kernel_paired<P, 8, IL>(bytes, begin, end, n_strings, pairg, symg, prm, out_state, out_accept);  \
+    }
+
+#define TREX_PAIRED_KERNELS_P(P)                                 \
+    TREX_PAIRED_KERNEL(dfa16_multi_k2fp_p##P, P, false)          \
+    TREX_PAIRED_KERNEL(dfa16_multi_k2fp_p##P##_il, P, true)
+
+TREX_PAIRED_KERNELS_P(1) TREX_PAIRED_KERNELS_P(2) TREX_PAIRED_KERNELS_P(3) TREX_PAIRED_KERNELS_P(4)
+TREX_PAIRED_KERNELS_P(5) TREX_PAIRED_KERNELS_P(6) TREX_PAIRED_KERNELS_P(7) TREX_PAIRED_KERNELS_P(8)
