@@ -12,7 +12,7 @@ import pytest
 from oracle import cpu as ocpu
 from oracle.nfa import set_walk_one, simulate_nfa
 from trex_amd.engine import NfaEngine, nfa_byte_map, pack_strings
-from trex_amd.nfa_matrices import NFAMatrices, compile_nfa_matrices
+from trex_amd.nfa_matrices import compile_nfa_matrices
 from trex_amd.workloads import random_nfa64, ragged_batch, uniform_batch
 
 pytestmark = pytest.mark.gpu

@@ -6,7 +6,6 @@ oracle is the DFA simulation of the string with those characters removed."""
 
 import random
 
-import numpy as np
 import pytest
 
 from oracle.dfa import dfa_simulate

@@ -31,7 +31,7 @@ window, else through one launch per pattern of the general byte walk.
 from __future__ import annotations
 
 import time
-from typing import Optional, Sequence
+from typing import Sequence
 
 import numpy as np
 
