@@ -123,5 +123,6 @@ def test_gpu_packed_empty_and_timed(gpu_lib):
     rng = random.Random(11)
     strings = _strings(rng, "ab", 2000, 256)
     assert pe.match_batch(strings) == _want(BINARY[:2], strings)
-    acc = pe.match_arrays(*__import__("trex_amd.engine", fromlist=["pack_strings"]).pack_strings(strings))
+    from trex_amd.engine import pack_strings
+    acc = pe.match_arrays(*pack_strings(strings))
     assert acc.shape == (2, 2000) and [[bool(x) for x in row] for row in acc] == _want(BINARY[:2], strings)
