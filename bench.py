@@ -138,6 +138,8 @@ def main() -> int:
     ap.add_argument("--serve-prob-cap", type=int, default=1 << 16,
                     help="longest probabilistic sequence in serving requests (each symbol is a sequential step)")
     ap.add_argument("--sweep", action="store_true", help="time every kernel variant and layout, then exit")
+    ap.add_argument("--serve-mix-form", default="nibble", choices=["nibble", "slice"],
+                    help="NFA table form of the server's mixed-queue lanes")
     ap.add_argument("--multi-sweep", action="store_true",
                     help="also time the multi-pattern pass for every table form and patterns-per-launch")
     ap.add_argument("--copy-probe", action="store_true",
@@ -895,7 +897,7 @@ def serve_leg(args, stream) -> dict:
     n_req = int(qps * duration)
     gaps = arr_rng.exponential(1.0 / qps, size=n_req)
     reqs = [make_request(arr_rng) for _ in range(n_req)]
-    server = Server(engines, hbm_budget=args.serve_hbm_gb << 30)
+    server = Server(engines, hbm_budget=args.serve_hbm_gb << 30, mix_form=args.serve_mix_form)
     t_start = time.perf_counter()
     with server:
         due = t_start
@@ -941,7 +943,7 @@ def serve_leg(args, stream) -> dict:
         if not isinstance(engines[r.pattern], ProbNfaEngine):
             burst.append(r)
     bb = sum(r.nbytes for r in burst)
-    srv = Server(engines, hbm_budget=args.serve_hbm_gb << 30)
+    srv = Server(engines, hbm_budget=args.serve_hbm_gb << 30, mix_form=args.serve_mix_form)
     t0 = time.perf_counter()
     with srv:
         for r in burst:

@@ -49,7 +49,8 @@ def _requests(rng, n_patterns, kinds, count, max_strings, median, sigma, cap):
 
 
 @pytest.mark.gpu
-def test_gpu_served_equals_alone(gpu_lib):
+@pytest.mark.parametrize("mix_form", ["nibble", "slice"])
+def test_gpu_served_equals_alone(gpu_lib, mix_form):
     from trex_amd.engine import DfaEngine, NfaEngine, ProbNfaEngine
     from trex_amd.serve import Server
     from trex_amd.tables import dfa_byte_table
@@ -61,7 +62,8 @@ def test_gpu_served_equals_alone(gpu_lib):
     engines += [ProbNfaEngine(*dirichlet_pnfa(rng, 32))]
     kinds = ["mix"] * 5 + ["prob"]
     reqs = _requests(rng, len(engines), kinds, 60, 64, 512, 2.0, 300_000)
-    server = Server(engines, hbm_budget=256 << 20, small_bytes=64 << 10, prob_chunk_min=1024, prob_chunk_len=128)
+    server = Server(engines, hbm_budget=256 << 20, small_bytes=64 << 10, prob_chunk_min=1024, prob_chunk_len=128,
+                    mix_form=mix_form)
     with server:
         for r in reqs:
             server.submit(r)

@@ -170,7 +170,8 @@ class Server:
                  small_params: MixParams | None = None, bulk_params: MixParams | None = None,
                  bulk_grid: int | None = None, small_budget: int | None = None, prob_slots: int = 2,
                  bulk_slots: int = 1, prob_grid: int = 32, reserve_strings: int = 1 << 18,
-                 prob_chunk_min: int = 1024, prob_chunk_len: int = 256, mix_max_spins: int = 0):
+                 prob_chunk_min: int = 1024, prob_chunk_len: int = 256, mix_max_spins: int = 0,
+                 mix_form: str = "nibble"):
         hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4)   # HIP's default is 4
         if hwq < 8:
             warnings.warn(f"GPU_MAX_HW_QUEUES is {hwq}: the server's seven streams share hardware queues, so "
@@ -210,6 +211,8 @@ class Server:
             lane["params"] = (small_params or MixParams(dfa_split=4 << 10, dfa_chunk=4 << 10, nfa_split=2 << 10,
                                                         nfa_chunk=2 << 10, ck=64)) if name == "small" else \
                 (bulk_params or MixParams())
+            if lane["params"].form is None:   # the lanes' NFA table form (and workgroup shape)
+                lane["params"] = MixParams(**{**lane["params"].__dict__, "form": mix_form})
         for slot in self.lanes["bulk"]["slots"]:
             slot.mix.set_grid(bulk_grid if bulk_grid is not None else max(1, cus // 2))
         for slot in self.lanes["small"]["slots"]:
