@@ -292,16 +292,7 @@ __device__ uint64_t nfa_walk(const uint8_t* __restrict__ bytes, int64_t p, int64
     const int64_t head_end = min(end, p + head_len(bytes, p));
     for (; p < head_end; ++p) nw.byte(bytes[p]);
     const int64_t nvec = (end - p) >> 4;
-    const uint4* v = reinterpret_cast<const uint4*>(bytes + p);
-    if (nvec > 0) {
-        uint4 a = v[0];
-        for (int64_t i = 1; i < nvec; ++i) {
-            const uint4 b = v[i];
-            nw.vec(a);
-            a = b;
-        }
-        nw.vec(a);
-    }
+    stream_body<2>(reinterpret_cast<const uint4*>(bytes + p), nvec, [&](uint4 x) { nw.vec(x); });
     nw.settle(win4);
     p += nvec << 4;
     for (; p < end; ++p) nw.byte(bytes[p]);
@@ -326,8 +317,16 @@ __device__ uint64_t nfa_walk_ck(const uint8_t* __restrict__ bytes, int64_t p, in
     int32_t idx = 0;
     const uint4* v = reinterpret_cast<const uint4*>(bytes + p);
     const int64_t nvec = (end - p) >> 4;
+    // two vectors in flight ahead of the one being walked (a walk that stops
+    // at a checkpoint drops them)
+    uint4 n0 = make_uint4(0u, 0u, 0u, 0u), n1 = n0;
+    if (nvec > 0) n0 = v[0];
+    if (nvec > 1) n1 = v[1];
     for (int64_t i = 0; i < nvec; ++i) {
-        nw.vec(v[i]);
+        const uint4 cur = n0;
+        n0 = n1;
+        if (i + 2 < nvec) n1 = v[i + 2];
+        nw.vec(cur);
         const int64_t pos = p + ((i + 1) << 4);
         if (pos % ck == 0 && idx < n_ck) {
             nw.settle(win4);
