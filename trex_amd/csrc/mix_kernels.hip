@@ -322,13 +322,19 @@ __device__ uint64_t nfa_walk_ck(const uint8_t* __restrict__ bytes, int64_t p, in
     uint4 n0 = make_uint4(0u, 0u, 0u, 0u), n1 = n0;
     if (nvec > 0) n0 = v[0];
     if (nvec > 1) n1 = v[1];
+    // the first multiple of ck after p; positions advance by 16 and ck is a
+    // multiple of 16, so pos meets every later one exactly (no division per
+    // vector)
+    int64_t next_ck = (p + 16 + ck - 1) / ck * ck;
     for (int64_t i = 0; i < nvec; ++i) {
         const uint4 cur = n0;
         n0 = n1;
         if (i + 2 < nvec) n1 = v[i + 2];
         nw.vec(cur);
         const int64_t pos = p + ((i + 1) << 4);
-        if (pos % ck == 0 && idx < n_ck) {
+        if (pos != next_ck) continue;
+        next_ck += ck;
+        if (idx < n_ck) {
             nw.settle(win4);
             if constexpr (RECORD) ck_out[idx] = nw.s;
             if constexpr (COMPARE) {
