@@ -27,6 +27,9 @@ Secondary legs (reported under "secondary"):
   * nfa64: one Boolean NFA-64 (64 states, each per-symbol edge present with
     p = 0.06, 32 symbols, state 0 start, 4 accept states) over the same 1 GiB
     batch, bit-exact against the C oracle's set walk;
+  * lookback_scan: the same 1 GiB as 16 strings of 64 MiB, the DFA leg's
+    first pattern, cut into 64 KiB chunks chained by the decoupled look-back
+    (one launch), bit-exact against the C oracle;
   * prob_nfa64: probabilistic NFA-64, 32 symbols, T[c] column-stochastic with
     Dirichlet(0.1) columns, uniform start, 16 accept states; 16,384 sequences
     x 2,048 uniform symbols; every mode's relative log-likelihood error
@@ -41,7 +44,7 @@ Secondary legs (reported under "secondary"):
     strings x 1,000 characters through OptimizedEngine.match_batch (packing,
     copies, kernel, results as list[bool]), split into its stages.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs dfa,nfa,prob,mix,serve,host]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs dfa,nfa,scan,prob,mix,serve,host]
 """
 
 from __future__ import annotations
@@ -125,8 +128,8 @@ def main() -> int:
     ap.add_argument("--prefetch", type=int, default=8, help="16-byte vectors in flight per lane (8 or 16)")
     ap.add_argument("--layout", default="interleaved", choices=["interleaved", "staggered", "csr"],
                     help="HBM layout of the DFA batch (the NFA leg reads a staggered copy when this is interleaved)")
-    ap.add_argument("--legs", default="dfa,nfa,prob,mix,serve,host",
-                    help="comma list of dfa, nfa, prob, mix, serve, host")
+    ap.add_argument("--legs", default="dfa,nfa,scan,prob,mix,serve,host",
+                    help="comma list of dfa, nfa, scan, prob, mix, serve, host")
     ap.add_argument("--prob-modes", default="f16x2,bf16x2,f16,bf16")
     ap.add_argument("--mix-strings", type=int, default=MIX_STRINGS, help="strings of the uneven-mix leg")
     ap.add_argument("--mix-tune", type=int, default=1, help="autotune the uneven-mix plan (0: defaults)")
@@ -140,6 +143,7 @@ def main() -> int:
     ap.add_argument("--sweep", action="store_true", help="time every kernel variant and layout, then exit")
     ap.add_argument("--serve-mix-form", default="nibble", choices=["nibble", "slice"],
                     help="NFA table form of the server's mixed-queue lanes")
+    ap.add_argument("--scan-sweep", action="store_true", help="also time the look-back scan for several chunk lengths")
     ap.add_argument("--multi-sweep", action="store_true",
                     help="also time the multi-pattern pass for every table form and patterns-per-launch")
     ap.add_argument("--copy-probe", action="store_true",
@@ -170,7 +174,7 @@ def main() -> int:
     ok = True
 
     batch = data = offsets = None
-    if "dfa" in legs or "nfa" in legs or args.sweep:
+    if "dfa" in legs or "nfa" in legs or "scan" in legs or args.sweep:
         rng = np.random.default_rng(SEED)
         t0 = time.perf_counter()
         data, offsets = uniform_batch(rng, N_STRINGS, LENGTH, 0x40, 0x60)
@@ -202,6 +206,9 @@ def main() -> int:
         ok &= secondary["nfa64"].get("verified_bit_exact") in (None, True)
     if batch is not None:
         batch.free()
+    if "scan" in legs:
+        secondary["lookback_scan"] = scan_leg(args, data, stream)
+        ok &= secondary["lookback_scan"].get("verified_bit_exact") in (None, True)
     if "prob" in legs:
         secondary["prob_nfa64"] = prob_leg(args, stream)
         ok &= secondary["prob_nfa64"].get("within_tolerance") in (None, True)
@@ -353,6 +360,61 @@ def dfa_leg(args, data, offsets, batch, stream, result: dict, copy_gbs) -> int:
         "multi_pattern": multi_res,
     })
     return 0 if verified in (None, True) else 1
+
+
+def scan_leg(args, data, stream) -> dict:
+    """Long strings through the decoupled look-back scan: the 1 GiB batch read
+    as 16 strings of 64 MiB, cut into chunks of lookback_chunk_len (8 KiB
+    here; one launch for all chunks), with the DFA leg's first pattern."""
+    from trex_amd.device import DeviceBuffer
+    from trex_amd.engine import ChunkPlan, DeviceBatch, DfaEngine
+    from trex_amd.tables import dfa_byte_table
+    from trex_amd.workloads import ALPHABET_32, random_dfa_pattern
+
+    n_long = 16
+    per = data.nbytes // n_long
+    offsets = np.arange(n_long + 1, dtype=np.int64) * per
+    table = dfa_byte_table(random_dfa_pattern(np.random.default_rng(SEED), ALPHABET_32)[1])
+    eng = DfaEngine(table)
+    batch = DeviceBatch(data, offsets, layout="csr")
+    d_f, d_a = DeviceBuffer(n_long), DeviceBuffer(n_long)
+    if args.scan_sweep:
+        for clen in (2048, 4096, 6144, 7936, 8192, 8448, 10240, 12288, 16384, 32768):
+            pl = ChunkPlan.for_batch(batch, clen)
+            t = time_launches(lambda: eng.match_plan(pl, batch.d_bytes.ptr, d_f.ptr, d_a.ptr, stream), stream,
+                              max(3, args.steps))
+            log(f"look-back scan sweep: {clen:6d} B chunks ({pl.n_chunks} chunks): {t:.4f} ms, "
+                f"{data.nbytes / (t * 1e-3) / 1e9:.0f} Gchars/s, error word {pl.error(stream)}")
+            pl.close()
+    plan = ChunkPlan.for_batch(batch)
+    ms = time_launches(lambda: eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr, stream), stream,
+                       max(3, args.steps))
+    err = plan.error(stream)
+    alg_bytes = data.nbytes + 2 * n_long
+    gchars = data.nbytes / (ms * 1e-3) / 1e9
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    log(f"look-back scan: {n_long} x {per >> 20} MiB, {plan.n_chunks} chunks of {plan.chunk_len >> 10} KiB, "
+        f"{ms:.4f} ms per launch, "
+        f"{gchars:.1f} Gchars/s, {achieved:.0f} GB/s, error word {err}")
+    out = {"workload": f"{n_long} strings x {per >> 20} MiB, {plan.chunk_len >> 10} KiB chunks chained by the look-back",
+           "kernel": "dfa16_chunks_window", "chunks": plan.n_chunks, "chunk_len": plan.chunk_len, "kernel_avg_ms": round(ms, 4),
+           "value": round(gchars, 1), "unit": "Gchars/s", "error_word": int(err),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "alg_bytes_per_launch": alg_bytes},
+           "verified_bit_exact": None}
+    if not args.no_verify:
+        from oracle import cpu as ocpu
+        rf, ra = ocpu.dfa_walk(table.delta, table.accept, table.start, data, offsets)
+        same = bool(np.array_equal(d_f.download(np.uint8, n_long), rf)
+                    and np.array_equal(d_a.download(np.uint8, n_long), ra) and err == 0)
+        out["verified_bit_exact"] = same
+        log(f"look-back scan: bit-exact={same}")
+        if not same:
+            log("PARITY FAILURE (look-back scan)")
+    plan.close()
+    batch.free()
+    eng.close()
+    return out
 
 
 def host_leg(args) -> dict:

@@ -118,3 +118,77 @@ def test_plan_even_a_long_strings_parity(gpu_lib):
     for i in range(len(lengths)):
         seg = data[offs[i]:offs[i + 1]]
         assert bool(a[i]) == (int(np.count_nonzero(seg == ord("a"))) % 2 == 0)
+
+
+def test_lookback_chunk_len():
+    from trex_amd.engine import LOOKBACK_TARGET_CHUNKS, lookback_chunk_len
+    assert lookback_chunk_len(1 << 30) == 8192
+    assert lookback_chunk_len(0) == 1024 and lookback_chunk_len(1 << 20) == 1024
+    assert lookback_chunk_len(1 << 40) == 1 << 16
+    for total in (3 << 27, 5 << 28, 1 << 31):
+        c = lookback_chunk_len(total)
+        assert c & (c - 1) == 0 and 1024 <= c <= 1 << 16
+        assert total / c / LOOKBACK_TARGET_CHUNKS < 2 ** 0.5 * 1.0001 or c == 1 << 16
+
+
+@pytest.mark.gpu
+def test_plan_constant_and_composed_chunks_side_by_side(gpu_lib):
+    """Chunks whose function is constant (a byte sending every state to the
+    dead state) publish their final state at once, next to chunks of the same
+    string that still look back (a permutation automaton): both kinds share
+    waves, and every result equals the sequential walk."""
+    tab = dfa_byte_table(compile_regex(EVEN_A))
+    rng = np.random.default_rng(5)
+    lengths = np.array([300_000, 123_457, 64, 0, 90_001], np.int64)
+    data, offs = ragged_batch(rng, lengths, ord("a"), ord("c"))
+    # a '~' (dead for this automaton) in about a third of the 256-byte chunks
+    starts = np.arange(0, len(data), 256)
+    starts = starts[rng.random(len(starts)) < 0.3]
+    data[np.minimum(starts + rng.integers(0, 256, size=len(starts)), len(data) - 1)] = 0x7e
+    f, a, n_chunks = _run_plan(tab, data, offs, 256)
+    rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offs)
+    assert np.array_equal(f, rf) and np.array_equal(a, ra)
+    assert n_chunks > 2000
+
+
+@pytest.mark.gpu
+def test_plan_adaptive_chunks_small_table(gpu_lib):
+    """A table of fewer than 16 states (the padding slots start at state 0 so
+    that a synchronised function is seen constant) over long strings, with the
+    chunk length picked from the batch size."""
+    from trex_amd.device import DeviceBuffer
+    from trex_amd.engine import ChunkPlan, DeviceBatch, DfaEngine, lookback_chunk_len
+    rng = np.random.default_rng(17)
+    _, dfa = random_dfa_pattern(rng, ALPHABET_32)
+    tab = dfa_byte_table(dfa)
+    assert tab.n_states < 16
+    lengths = np.array([40 << 20, (24 << 20) + 5, 3, 1 << 20], np.int64)
+    data, offs = ragged_batch(rng, lengths, 0x40, 0x60)
+    eng = DfaEngine(tab)
+    batch = DeviceBatch(data, offs, layout="csr")
+    plan = ChunkPlan.for_batch(batch)
+    assert plan.chunk_len == lookback_chunk_len(int(lengths.sum()))
+    d_f, d_a = DeviceBuffer(len(lengths)), DeviceBuffer(len(lengths))
+    eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr)
+    assert plan.error() == 0
+    rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offs)
+    assert np.array_equal(d_f.download(np.uint8, len(lengths)), rf)
+    assert np.array_equal(d_a.download(np.uint8, len(lengths)), ra)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("chunk_len", [64, 1024, 8192])
+def test_plan_byte_kernel_two_windows(gpu_lib, chunk_len):
+    """A pattern whose bytes lie in two 32-byte windows runs the byte kernel
+    (dfa16_chunks_bytes): same results as the sequential walk."""
+    from trex_amd.engine import DfaEngine
+    tab = dfa_byte_table(compile_regex("(a|B)*aBB"))
+    assert DfaEngine(tab).info()["window_base"] == -1
+    rng = np.random.default_rng(chunk_len + 1)
+    lengths = np.concatenate([rng.integers(0, 200, 300), [0, 1, 16, chunk_len, chunk_len + 1, 250_000, 77_777]])
+    offs = np.zeros(len(lengths) + 1, np.int64)
+    np.cumsum(lengths, out=offs[1:])
+    data = np.frombuffer(b"aBB~", np.uint8)[rng.choice(4, size=int(offs[-1]), p=[0.45, 0.45, 0.099, 0.001])]
+    f, a, n_chunks = _run_plan(tab, np.ascontiguousarray(data), offs, chunk_len)
+    rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offs)
+    assert np.array_equal(f, rf) and np.array_equal(a, ra)

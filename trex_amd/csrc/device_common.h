@@ -133,6 +133,20 @@ __device__ __forceinline__ Fn16 fn_then(const Fn16& f, const Fn16& g) {
     return Fn16{{fn_gather(f.d[0], g), fn_gather(f.d[1], g), fn_gather(f.d[2], g), fn_gather(f.d[3], g)}};
 }
 
+// The identity on states 0 .. n-1 with the slots of absent states (s >= n,
+// whose table rows are padded with the identity) started at state 0: every
+// slot then always holds the image of a real state, so the function is
+// constant on the real states exactly when all 16 slots are equal.
+__device__ __forceinline__ Fn16 fn_identity_on(uint32_t n) {
+    Fn16 f = fn_identity();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int k = (int)n - 4 * i;   // real slots in word i
+        f.d[i] &= k >= 4 ? 0xffffffffu : (k <= 0 ? 0u : (1u << (8 * k)) - 1u);
+    }
+    return f;
+}
+
 __device__ __forceinline__ bool fn_is_const(const Fn16& f) {
     const uint32_t c = (f.d[0] & 0xffu) * 0x01010101u;
     return (f.d[0] == c) & (f.d[1] == c) & (f.d[2] == c) & (f.d[3] == c);

@@ -39,6 +39,8 @@ struct MixPattern {
     uint32_t win4;
     uint32_t start;
     uint32_t dead;
+    uint32_t nstates;
+    uint32_t reserved;
     uint64_t nstart;
     uint64_t naccept;
     const uint8_t* t1;
@@ -242,6 +244,7 @@ int trex_mix_create(const int32_t* kinds, const void* const* handles, int32_t n_
                 P.win4 = (uint32_t)d->window_base * 0x01010101u;
                 P.start = (uint32_t)d->start;
                 P.dead = (uint32_t)d->dead;
+                P.nstates = (uint32_t)d->n_states;
                 P.t1 = d->d_t1;
                 P.t2 = d->d_t2;
                 P.f2 = d->d_f2;

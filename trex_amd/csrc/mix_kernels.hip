@@ -52,6 +52,8 @@ struct MixPattern {
     uint32_t win4;        // window base byte x 0x01010101
     uint32_t start;       // DFA start state
     uint32_t dead;        // DFA dead state
+    uint32_t nstates;     // DFA state count (<= 16)
+    uint32_t reserved;
     uint64_t nstart;      // NFA start set
     uint64_t naccept;     // NFA accept set
     const uint8_t* t1;    // DFA byte table [256][16]
@@ -171,7 +173,7 @@ __device__ void dfa_chunk(const MixArgs& A, const MixPattern& P, const DfaLds& L
     }
     sc.walking = pred < 0;
     sc.s = P.start;
-    sc.f = fn_identity();
+    sc.f = fn_identity_on(P.nstates);
     const uint8_t* bytes = A.bytes;
     const int64_t head_end = min(end, p + head_len(bytes, p));
     for (; p < head_end; ++p) sc.byte(bytes[p]);
@@ -192,20 +194,23 @@ __device__ void dfa_chunk(const MixArgs& A, const MixPattern& P, const DfaLds& L
     }
     p += nvec << 4;
     for (; p < end; ++p) sc.byte(bytes[p]);
+    sc.settle();
     if (sc.w.outside(P.win4)) { sc.walking = true; sc.s = P.dead; }
 
-    uint64_t my_fn = 0;
+    // A chunk with a constant function knows its final state without looking
+    // back and publishes PREFIX at once. Every lane's status is stored before
+    // any lane of the wave enters the look-back (a lane may look back to a
+    // neighbour in its own wave): one store for both kinds, then a convergent
+    // wave barrier, keeps the compiler from moving the publish behind the loop.
+    const bool known = pred < 0 || sc.walking;
+    const uint64_t my_fn = known ? 0ull : fn_pack(sc.f);
     if (active) {
-        if (pred < 0) {
-            store_release(&A.status[item], kStatusPrefix | (sc.s << 8));
-        } else {
-            my_fn = sc.walking ? fn_pack(fn_const(sc.s)) : fn_pack(sc.f);
-            A.agg[item] = my_fn;
-            store_release(&A.status[item], kStatusAggregate);
-        }
+        if (!known) A.agg[item] = my_fn;
+        store_release(&A.status[item], known ? (kStatusPrefix | (sc.s << 8)) : kStatusAggregate);
     }
+    __builtin_amdgcn_wave_barrier();
     uint32_t state = sc.s;
-    if (active && pred >= 0) {
+    if (active && !known) {
         uint64_t acc_fn = 0xfedcba9876543210ull;   // identity
         int32_t j = pred;
         uint32_t spins = 0;

@@ -140,12 +140,14 @@ int trex_dfa_match_plan(trex_dfa* h, trex_plan* p, const uint8_t* d_bytes, uint8
     uint32_t win4 = window ? (uint32_t)h->window_base * 0x01010101u : 0u;
     uint32_t start = (uint32_t)h->start;
     uint32_t dead = h->dead >= 0 ? (uint32_t)h->dead : 0u;
+    uint32_t n_states = (uint32_t)h->n_states;
     uint8_t* t2 = window ? h->d_t2 : h->d_t1;
+    uint8_t* f2 = window ? h->d_f2 : h->d_t1;   // the byte kernel builds its own functions
     int64_t n_chunks = p->n_chunks;
     uint32_t* ticket = p->control.as<uint32_t>();
     uint32_t* error = ticket + 1;
     void* args[] = {&d_bytes, &p->begin.ptr, &p->end.ptr, &p->sid.ptr, &p->pred.ptr, &p->flags.ptr, &n_chunks,
-                    &t2, &h->d_t1, &h->d_acc, &win4, &start, &dead, &p->status.ptr, &p->agg.ptr, &ticket,
+                    &t2, &h->d_t1, &f2, &h->d_acc, &win4, &start, &dead, &n_states, &p->status.ptr, &p->agg.ptr, &ticket,
                     &error, &max_spins, &d_final_state, &d_accept};
     // persistent grid: enough waves to fill every CU, never more than the work
     const int64_t waves_needed = (n_chunks + 63) / 64;

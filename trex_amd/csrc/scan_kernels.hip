@@ -13,9 +13,11 @@
 //    three byte permutes per register (v_perm_b32), G being the 16-byte row
 //    of the pair table (two bytes per step) or of the byte table. Once F maps
 //    every state to one state it stays constant-valued, so the lane switches
-//    to walking that single state.
+//    to walking that single state (the slots of absent states, s >= n_states,
+//    start at state 0 so that they never hold F from being seen constant).
 //    The chunk then publishes AGGREGATE (its function), or PREFIX (its final
-//    state) for a first chunk.
+//    state) for a first chunk and for any chunk whose function is constant
+//    (its final state then does not depend on the chunks before it).
 //  Phase 2 (look-back, cuda/monoid_scan.cu:128-222 restated over functions):
 //    walk back over predecessors, composing their aggregates, until one with
 //    PREFIX is found; the chunk's prefix state is F(acc(p)). Waiting is only
@@ -45,27 +47,36 @@ using namespace trexdev;
 
 constexpr uint32_t kStatusAggregate = 1u;
 constexpr uint32_t kStatusPrefix = 2u;
+constexpr int kScanPF = 8;   // 16-byte vectors in flight per lane
 
 // The per-lane scanner: either walking one state or composing a function.
+// The walk reads nibble-packed transition functions (nibble s of f1[b] /
+// f2[pair] = successor of s), so the LDS address of a step does not depend on
+// the state: the reads of a whole vector issue together and only a shift and
+// a mask per step wait on the previous state. Composition gathers the 16-byte
+// rows of t1 / t2.
 template <bool WINDOW>
 struct Scanner {
-    const uint8_t* t2;   // pair table [1024][16] (WINDOW)
-    const uint8_t* t1;   // byte table [256][16]
+    const uint8_t* t2;    // pair rows [1024][16] (WINDOW)
+    const uint8_t* t1;    // byte rows [256][16]
+    const uint64_t* f2;   // pair functions [1024] (WINDOW)
+    const uint64_t* f1;   // byte functions [256]
     bool walking;
     uint32_t s;
     Fn16 f;
     WinCheck w;
 
     __device__ __forceinline__ void byte(uint32_t b) {
-        if (walking) s = t1[(b << 4) | s];
+        if (walking) s = (uint32_t)(f1[b] >> (s << 2)) & 15u;
         else f = fn_then(f, load_row(t1, b));
     }
     __device__ __forceinline__ void word(uint32_t x) {
         if constexpr (WINDOW) {
             w.add(x);
             if (walking) {
-                s = t2[(pair_lo(x) << 4) | s];
-                s = t2[(pair_hi(x) << 4) | s];
+                const uint64_t a = f2[pair_lo(x)], b = f2[pair_hi(x)];
+                s = (uint32_t)(a >> (s << 2)) & 15u;
+                s = (uint32_t)(b >> (s << 2)) & 15u;
             } else {
                 f = fn_then(f, load_row(t2, pair_lo(x)));
                 f = fn_then(f, load_row(t2, pair_hi(x)));
@@ -74,7 +85,7 @@ struct Scanner {
             byte(x & 255u); byte((x >> 8) & 255u); byte((x >> 16) & 255u); byte(x >> 24);
         }
     }
-    // After a block: a constant function becomes a single walked state.
+    // After a vector: a constant function becomes a single walked state.
     __device__ __forceinline__ void settle() {
         if (!walking && fn_is_const(f)) {
             walking = true;
@@ -83,25 +94,44 @@ struct Scanner {
     }
 };
 
+// Always inlined into its two kernels. With 16 vectors in flight the byte
+// kernel's copy was compiled as a called function (280 VGPRs, 112-byte stack
+// for its arguments) and its first run did not finish; the cause was not
+// found, so the depth stays at 8, where both kernels inline (<= 106 VGPRs, no
+// stack) and pass every test.
 template <bool WINDOW>
-__device__ void chunk_scan(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ cbegin,
+__device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ cbegin,
                            const int64_t* __restrict__ cend, const int32_t* __restrict__ csid,
                            const int32_t* __restrict__ cpred, const uint32_t* __restrict__ cflags, int64_t n_chunks,
                            const uint8_t* __restrict__ t2g, const uint8_t* __restrict__ t1g,
-                           const uint8_t* __restrict__ accg, uint32_t win4, uint32_t start, uint32_t dead,
-                           uint32_t* __restrict__ status, uint64_t* __restrict__ agg, uint32_t* __restrict__ ticket,
-                           uint32_t* __restrict__ error, uint32_t max_spins, uint8_t* __restrict__ out_state,
-                           uint8_t* __restrict__ out_accept)
+                           const uint64_t* __restrict__ f2g, const uint8_t* __restrict__ accg, uint32_t win4, uint32_t start, uint32_t dead,
+                           uint32_t n_states, uint32_t* __restrict__ status, uint64_t* __restrict__ agg,
+                           uint32_t* __restrict__ ticket, uint32_t* __restrict__ error, uint32_t max_spins,
+                           uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
 {
     __shared__ __attribute__((aligned(16))) uint8_t t2[WINDOW ? 1024 * 16 : 16];
     __shared__ __attribute__((aligned(16))) uint8_t t1[256 * 16];
+    __shared__ uint64_t f2[WINDOW ? 1024 : 1];
+    __shared__ uint64_t f1[256];
     __shared__ __attribute__((aligned(16))) uint8_t acc[16];
     {
         const uint4* s4 = reinterpret_cast<const uint4*>(t1g);
-        for (int i = threadIdx.x; i < 256; i += blockDim.x) reinterpret_cast<uint4*>(t1)[i] = s4[i];
+        for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+            const uint4 row = s4[i];
+            reinterpret_cast<uint4*>(t1)[i] = row;
+            // nibble-pack the row (entries are states < 16)
+            const uint32_t w[4] = {row.x, row.y, row.z, row.w};
+            uint64_t fn = 0;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) fn |= (uint64_t)((w[k >> 2] >> ((k & 3) * 8)) & 15u) << (4 * k);
+            f1[i] = fn;
+        }
         if constexpr (WINDOW) {
             const uint4* p4 = reinterpret_cast<const uint4*>(t2g);
-            for (int i = threadIdx.x; i < 1024; i += blockDim.x) reinterpret_cast<uint4*>(t2)[i] = p4[i];
+            for (int i = threadIdx.x; i < 1024; i += blockDim.x) {
+                reinterpret_cast<uint4*>(t2)[i] = p4[i];
+                f2[i] = f2g[i];
+            }
         }
         if (threadIdx.x < 16) acc[threadIdx.x] = accg[threadIdx.x];
     }
@@ -120,6 +150,8 @@ __device__ void chunk_scan(const uint8_t* __restrict__ bytes, const int64_t* __r
         Scanner<WINDOW> sc;
         sc.t2 = t2;
         sc.t1 = t1;
+        sc.f2 = f2;
+        sc.f1 = f1;
         sc.w.init();
         int32_t pred = -1;
         uint32_t flags = 0;
@@ -132,47 +164,45 @@ __device__ void chunk_scan(const uint8_t* __restrict__ bytes, const int64_t* __r
         }
         sc.walking = pred < 0;
         sc.s = start;
-        sc.f = fn_identity();
+        sc.f = fn_identity_on(n_states);
 
         // ---- phase 1: walk or compose this chunk's bytes ----
         const int64_t head_end = min(end, p + (int64_t)((16u - ((uint32_t)(uintptr_t)(bytes + p) & 15u)) & 15u));
         for (; p < head_end; ++p) sc.byte(bytes[p]);
+        // body: 16-byte vectors, kScanPF of them in flight while the previous
+        // ones are walked (one lane's loads are its own chunk's next bytes)
         const int64_t nvec = (end - p) >> 4;
-        const uint4* v = reinterpret_cast<const uint4*>(bytes + p);
-        int64_t i = 0;
-        for (; i + 4 <= nvec; i += 4) {
-            const uint4 a0 = v[i], a1 = v[i + 1], a2 = v[i + 2], a3 = v[i + 3];
-            sc.word(a0.x); sc.word(a0.y); sc.word(a0.z); sc.word(a0.w);
-            sc.word(a1.x); sc.word(a1.y); sc.word(a1.z); sc.word(a1.w);
-            sc.word(a2.x); sc.word(a2.y); sc.word(a2.z); sc.word(a2.w);
-            sc.word(a3.x); sc.word(a3.y); sc.word(a3.z); sc.word(a3.w);
-            sc.settle();
-        }
-        for (; i < nvec; ++i) {
-            const uint4 a = v[i];
+        stream_body<kScanPF>(reinterpret_cast<const uint4*>(bytes + p), nvec, [&](const uint4& a) {
             sc.word(a.x); sc.word(a.y); sc.word(a.z); sc.word(a.w);
-        }
+            sc.settle();
+        });
         p += nvec << 4;
         for (; p < end; ++p) sc.byte(bytes[p]);
+        sc.settle();
         if constexpr (WINDOW) {
             if (sc.w.outside(win4)) { sc.walking = true; sc.s = dead; }
         }
 
         // ---- publish ----
-        uint64_t my_fn = 0;
+        // A constant function fixes the chunk's final state whatever came
+        // before it: such a chunk publishes PREFIX at once and skips phase 2,
+        // and look-backs of later chunks stop at it.
+        // Every lane's status is stored before any lane of the wave enters
+        // phase 2: a lane may look back to a neighbour in its own wave, so a
+        // store held back behind the look-back loop would never be seen. One
+        // store for both kinds, then a convergent wave barrier, keeps the
+        // compiler from merging the publish into the phase-2 branch.
+        const bool known = pred < 0 || sc.walking;
+        const uint64_t my_fn = known ? 0ull : fn_pack(sc.f);
         if (active) {
-            if (pred < 0) {
-                store_release(&status[c], kStatusPrefix | (sc.s << 8));
-            } else {
-                my_fn = sc.walking ? fn_pack(fn_const(sc.s)) : fn_pack(sc.f);
-                agg[c] = my_fn;
-                store_release(&status[c], kStatusAggregate);
-            }
+            if (!known) agg[c] = my_fn;
+            store_release(&status[c], known ? (kStatusPrefix | (sc.s << 8)) : kStatusAggregate);
         }
+        __builtin_amdgcn_wave_barrier();
 
         // ---- phase 2: look back to the nearest published prefix ----
         uint32_t state = sc.s;
-        if (active && pred >= 0) {
+        if (active && !known) {
             uint64_t acc_fn = 0xfedcba9876543210ull;   // identity
             int32_t j = pred;
             uint32_t spins = 0;
@@ -219,13 +249,13 @@ dfa16_chunks_window(const uint8_t* __restrict__ bytes, const int64_t* __restrict
                     const int64_t* __restrict__ cend, const int32_t* __restrict__ csid,
                     const int32_t* __restrict__ cpred, const uint32_t* __restrict__ cflags, int64_t n_chunks,
                     const uint8_t* __restrict__ t2g, const uint8_t* __restrict__ t1g,
-                    const uint8_t* __restrict__ accg, uint32_t win4, uint32_t start, uint32_t dead,
-                    uint32_t* __restrict__ status, uint64_t* __restrict__ agg, uint32_t* __restrict__ ticket,
-                    uint32_t* __restrict__ error, uint32_t max_spins, uint8_t* __restrict__ out_state,
-                    uint8_t* __restrict__ out_accept)
+                    const uint64_t* __restrict__ f2g, const uint8_t* __restrict__ accg, uint32_t win4, uint32_t start, uint32_t dead,
+                    uint32_t n_states, uint32_t* __restrict__ status, uint64_t* __restrict__ agg,
+                    uint32_t* __restrict__ ticket, uint32_t* __restrict__ error, uint32_t max_spins,
+                    uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
 {
-    chunk_scan<true>(bytes, cbegin, cend, csid, cpred, cflags, n_chunks, t2g, t1g, accg, win4, start, dead, status,
-                     agg, ticket, error, max_spins, out_state, out_accept);
+    chunk_scan<true>(bytes, cbegin, cend, csid, cpred, cflags, n_chunks, t2g, t1g, f2g, accg, win4, start, dead, n_states,
+                     status, agg, ticket, error, max_spins, out_state, out_accept);
 }
 
 extern "C" __global__ void __launch_bounds__(TREX_BLOCK)
@@ -233,13 +263,13 @@ dfa16_chunks_bytes(const uint8_t* __restrict__ bytes, const int64_t* __restrict_
                    const int64_t* __restrict__ cend, const int32_t* __restrict__ csid,
                    const int32_t* __restrict__ cpred, const uint32_t* __restrict__ cflags, int64_t n_chunks,
                    const uint8_t* __restrict__ t2g, const uint8_t* __restrict__ t1g,
-                   const uint8_t* __restrict__ accg, uint32_t win4, uint32_t start, uint32_t dead,
-                   uint32_t* __restrict__ status, uint64_t* __restrict__ agg, uint32_t* __restrict__ ticket,
-                   uint32_t* __restrict__ error, uint32_t max_spins, uint8_t* __restrict__ out_state,
-                   uint8_t* __restrict__ out_accept)
+                   const uint64_t* __restrict__ f2g, const uint8_t* __restrict__ accg, uint32_t win4, uint32_t start, uint32_t dead,
+                   uint32_t n_states, uint32_t* __restrict__ status, uint64_t* __restrict__ agg,
+                   uint32_t* __restrict__ ticket, uint32_t* __restrict__ error, uint32_t max_spins,
+                   uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
 {
-    chunk_scan<false>(bytes, cbegin, cend, csid, cpred, cflags, n_chunks, t2g, t1g, accg, win4, start, dead, status,
-                      agg, ticket, error, max_spins, out_state, out_accept);
+    chunk_scan<false>(bytes, cbegin, cend, csid, cpred, cflags, n_chunks, t2g, t1g, f2g, accg, win4, start, dead, n_states,
+                      status, agg, ticket, error, max_spins, out_state, out_accept);
 }
 
 // Sets dst[0 .. n) = value (status words, tickets, error flags).

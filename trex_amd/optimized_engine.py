@@ -48,9 +48,9 @@ from trex_amd.tables import MAX_TABLE_STATES, dfa_byte_table, tmap_byte_table
 CONFIGS = (None, "monoid", "monoid+kgram", "baseline", "nfa", "monoid+gpu", "batched+gpu", "kgram+gpu",
            "monoid_batch+gpu", "prefix+gpu", "fp16_tc+gpu")
 
-# strings longer than this are matched through the chunked look-back scan
+# strings longer than this are matched through the chunked look-back scan,
+# cut into chunks of lookback_chunk_len(total bytes)
 LONG_STRING = 1 << 20
-CHUNK_LEN = 1 << 16
 
 
 class OptimizedEngine:
@@ -248,7 +248,7 @@ class OptimizedEngine:
         from trex_amd.device import DeviceBuffer
         from trex_amd.engine import ChunkPlan, DeviceBatch
         batch = DeviceBatch(data, offsets, layout="csr")
-        plan = ChunkPlan.for_batch(batch, CHUNK_LEN)
+        plan = ChunkPlan.for_batch(batch)
         n = batch.n_strings
         d_f, d_a = DeviceBuffer(n), DeviceBuffer(n)
         eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr)
