@@ -411,6 +411,35 @@ def scan_leg(args, data, stream) -> dict:
         log(f"look-back scan: bit-exact={same}")
         if not same:
             log("PARITY FAILURE (look-back scan)")
+    # an automaton that never synchronises: every chunk keeps a permutation as
+    # its function and resolves through the look-back chain
+    from trex_amd.tables import DfaByteTable
+    delta = np.full((16, 256), 15, np.uint8)
+    for b in range(0x40, 0x60):
+        delta[:15, b] = (np.arange(15) + (b & 15)) % 15
+    accept = np.zeros(16, np.uint8)
+    accept[0] = 1
+    perm = DfaByteTable(delta, accept, 0, 15, 16)
+    peng = DfaEngine(perm)
+    pms = time_launches(lambda: peng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr, stream), stream,
+                        max(3, args.steps))
+    perr = plan.error(stream)
+    log(f"look-back scan, sum of symbols mod 15 (never synchronises): {pms:.4f} ms per launch, "
+        f"{data.nbytes / (pms * 1e-3) / 1e9:.1f} Gchars/s, error word {perr}")
+    out["permutation_automaton"] = {"automaton": "sum of (byte & 15) mod 15, 16 states with the dead state",
+                                    "kernel_avg_ms": round(pms, 4),
+                                    "value": round(data.nbytes / (pms * 1e-3) / 1e9, 1), "error_word": int(perr),
+                                    "verified_bit_exact": None}
+    if not args.no_verify:
+        from oracle import cpu as ocpu
+        rf, ra = ocpu.dfa_walk(perm.delta, perm.accept, perm.start, data, offsets)
+        same = bool(np.array_equal(d_f.download(np.uint8, n_long), rf)
+                    and np.array_equal(d_a.download(np.uint8, n_long), ra) and perr == 0)
+        out["permutation_automaton"]["verified_bit_exact"] = same
+        log(f"look-back scan (mod 15): bit-exact={same}")
+        if not same:
+            log("PARITY FAILURE (look-back scan, mod 15)")
+    peng.close()
     plan.close()
     batch.free()
     eng.close()

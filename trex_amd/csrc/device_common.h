@@ -152,8 +152,14 @@ __device__ __forceinline__ bool fn_is_const(const Fn16& f) {
     return (f.d[0] == c) & (f.d[1] == c) & (f.d[2] == c) & (f.d[3] == c);
 }
 
+// f(s): byte s & 7 of each half by a byte permute (selector byte 0x0c reads
+// as zero), then the half by a select on the results. Choosing the words
+// first, or indexing f.d with s, lets the compiler put f in scratch memory.
 __device__ __forceinline__ uint32_t fn_apply(const Fn16& f, uint32_t s) {
-    return (f.d[s >> 2] >> ((s & 3u) * 8u)) & 0xffu;
+    const uint32_t sel = 0x0c0c0c00u | (s & 7u);
+    const uint32_t lo = __builtin_amdgcn_perm(f.d[1], f.d[0], sel);
+    const uint32_t hi = __builtin_amdgcn_perm(f.d[3], f.d[2], sel);
+    return (s & 8u) ? hi : lo;
 }
 
 __device__ __forceinline__ uint64_t fn_pack(const Fn16& f) {

@@ -17,7 +17,7 @@ struct trex_plan {
     int64_t n_chunks = 0;
     int64_t chunk_len = 0;
     DevBuf begin, end, sid, pred, flags;   // per chunk, in processing order
-    DevBuf status, agg, control;           // look-back state; control = {ticket, error}
+    DevBuf status, agg, jump, control;     // look-back state; control = {ticket, error}
     std::mutex mu;                         // one launch at a time per plan
 };
 
@@ -86,6 +86,7 @@ int trex_plan_create(const int64_t* begin, const int64_t* end, int64_t n_strings
     if (!rc) rc = p->flags.reserve(m * 4);
     if (!rc) rc = p->status.reserve(m * 4);
     if (!rc) rc = p->agg.reserve(m * 8);
+    if (!rc) rc = p->jump.reserve(m * 4);
     if (!rc) rc = p->control.reserve(16);
     if (!rc && n > 0) {
         hipError_t e = hipMemcpy(p->begin.ptr, ob.data(), (size_t)n * 8, hipMemcpyHostToDevice);
@@ -102,7 +103,8 @@ int trex_plan_create(const int64_t* begin, const int64_t* end, int64_t n_strings
 
 int trex_plan_destroy(trex_plan* p) {
     if (!p) return TREX_OK;
-    for (DevBuf* b : {&p->begin, &p->end, &p->sid, &p->pred, &p->flags, &p->status, &p->agg, &p->control}) b->release();
+    for (DevBuf* b : {&p->begin, &p->end, &p->sid, &p->pred, &p->flags, &p->status, &p->agg, &p->jump, &p->control})
+        b->release();
     delete p;
     return TREX_OK;
 }
@@ -147,7 +149,8 @@ int trex_dfa_match_plan(trex_dfa* h, trex_plan* p, const uint8_t* d_bytes, uint8
     uint32_t* ticket = p->control.as<uint32_t>();
     uint32_t* error = ticket + 1;
     void* args[] = {&d_bytes, &p->begin.ptr, &p->end.ptr, &p->sid.ptr, &p->pred.ptr, &p->flags.ptr, &n_chunks,
-                    &t2, &h->d_t1, &f2, &h->d_acc, &win4, &start, &dead, &n_states, &p->status.ptr, &p->agg.ptr, &ticket,
+                    &t2, &h->d_t1, &f2, &h->d_acc, &win4, &start, &dead, &n_states, &p->status.ptr, &p->agg.ptr,
+                    &p->jump.ptr, &ticket,
                     &error, &max_spins, &d_final_state, &d_accept};
     // persistent grid: enough waves to fill every CU, never more than the work
     const int64_t waves_needed = (n_chunks + 63) / 64;

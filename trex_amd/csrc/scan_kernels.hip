@@ -101,13 +101,16 @@ struct Scanner {
 // stack) and pass every test.
 template <bool WINDOW>
 __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ cbegin,
-                           const int64_t* __restrict__ cend, const int32_t* __restrict__ csid,
-                           const int32_t* __restrict__ cpred, const uint32_t* __restrict__ cflags, int64_t n_chunks,
-                           const uint8_t* __restrict__ t2g, const uint8_t* __restrict__ t1g,
-                           const uint64_t* __restrict__ f2g, const uint8_t* __restrict__ accg, uint32_t win4, uint32_t start, uint32_t dead,
-                           uint32_t n_states, uint32_t* __restrict__ status, uint64_t* __restrict__ agg,
-                           uint32_t* __restrict__ ticket, uint32_t* __restrict__ error, uint32_t max_spins,
-                           uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
+                                           const int64_t* __restrict__ cend, const int32_t* __restrict__ csid,
+                                           const int32_t* __restrict__ cpred, const uint32_t* __restrict__ cflags,
+                                           int64_t n_chunks, const uint8_t* __restrict__ t2g,
+                                           const uint8_t* __restrict__ t1g, const uint64_t* __restrict__ f2g,
+                                           const uint8_t* __restrict__ accg, uint32_t win4, uint32_t start,
+                                           uint32_t dead, uint32_t n_states, uint32_t* __restrict__ status,
+                                           uint64_t* __restrict__ agg, int32_t* __restrict__ jump,
+                                           uint32_t* __restrict__ ticket, uint32_t* __restrict__ error,
+                                           uint32_t max_spins, uint8_t* __restrict__ out_state,
+                                           uint8_t* __restrict__ out_accept)
 {
     __shared__ __attribute__((aligned(16))) uint8_t t2[WINDOW ? 1024 * 16 : 16];
     __shared__ __attribute__((aligned(16))) uint8_t t1[256 * 16];
@@ -183,26 +186,60 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
             if (sc.w.outside(win4)) { sc.walking = true; sc.s = dead; }
         }
 
+        // ---- segmented scan of the wave's 64 chunk functions ----
+        // Lane k continues lane k-1's segment when its chunk directly follows
+        // lane k-1's chunk in the same string (pred == c - 1); every other
+        // lane starts a segment. Six shuffle steps (Hillis-Steele) leave in g
+        // the composition of the chunks from the segment's first lane up to
+        // this one. A first chunk, or any chunk whose function turned
+        // constant, enters as a constant function, so everything after it in
+        // the segment comes out constant: its final state is known here.
+        Fn16 g = (pred < 0 || sc.walking) ? fn_const(sc.s) : sc.f;
+        const bool link = active && lane > 0u && pred == (int32_t)(c - 1);
+        bool closed = !link;   // the lanes folded into g reach back to a segment head
+#pragma unroll
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            Fn16 up;
+            up.d[0] = __shfl_up(g.d[0], d, 64);
+            up.d[1] = __shfl_up(g.d[1], d, 64);
+            up.d[2] = __shfl_up(g.d[2], d, 64);
+            up.d[3] = __shfl_up(g.d[3], d, 64);
+            const bool up_closed = __shfl_up((int)closed, d, 64) != 0;
+            if (lane >= d && !closed) {
+                g = fn_then(up, g);   // the earlier lanes' chunks apply first
+                closed = up_closed;
+            }
+        }
+        const uint64_t heads = __ballot(!link);
+        const uint32_t head_lane = 63u - (uint32_t)__clzll(heads & ((2ull << lane) - 1ull));
+        const int32_t head_pred = __shfl(pred, (int)head_lane, 64);
+
         // ---- publish ----
-        // A constant function fixes the chunk's final state whatever came
-        // before it: such a chunk publishes PREFIX at once and skips phase 2,
-        // and look-backs of later chunks stop at it.
+        // A constant g publishes PREFIX (its final state). Otherwise agg[c] =
+        // g, the function of the segment's chunks up to c, and jump[c] = the
+        // chunk before the segment: a look-back that meets c composes g and
+        // continues at jump[c], one step per segment instead of per chunk.
         // Every lane's status is stored before any lane of the wave enters
-        // phase 2: a lane may look back to a neighbour in its own wave, so a
-        // store held back behind the look-back loop would never be seen. One
-        // store for both kinds, then a convergent wave barrier, keeps the
-        // compiler from merging the publish into the phase-2 branch.
-        const bool known = pred < 0 || sc.walking;
-        const uint64_t my_fn = known ? 0ull : fn_pack(sc.f);
+        // phase 2 (a segment head may look back into its own wave): one store
+        // for both kinds, then a convergent wave barrier, keeps the compiler
+        // from moving a lane's store behind the look-back loop.
+        const bool known = fn_is_const(g);
+        const uint32_t kstate = g.d[0] & 0xffu;
+        const uint64_t my_fn = known ? 0ull : fn_pack(g);
         if (active) {
-            if (!known) agg[c] = my_fn;
-            store_release(&status[c], known ? (kStatusPrefix | (sc.s << 8)) : kStatusAggregate);
+            if (!known) {
+                agg[c] = my_fn;
+                jump[c] = head_pred;
+            }
+            store_release(&status[c], known ? (kStatusPrefix | (kstate << 8)) : kStatusAggregate);
         }
         __builtin_amdgcn_wave_barrier();
 
-        // ---- phase 2: look back to the nearest published prefix ----
-        uint32_t state = sc.s;
-        if (active && !known) {
+        // ---- phase 2: the head of each unresolved segment looks back ----
+        // (a segment is unresolved only if its head is: a constant head makes
+        // every later lane of the segment constant)
+        uint32_t incoming = 0;
+        if (active && !known && head_lane == lane) {
             uint64_t acc_fn = 0xfedcba9876543210ull;   // identity
             int32_t j = pred;
             uint32_t spins = 0;
@@ -212,28 +249,31 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
                 if (st == 0u) {
                     if (++spins > max_spins) {
                         atomicOr(error, 1u);
-                        state = dead;
+                        incoming = dead;
                         break;
                     }
                     __builtin_amdgcn_s_sleep(2);
                     continue;
                 }
-                fence_acquire();   // agg[j] was written before status[j] was released
+                fence_acquire();   // agg[j] and jump[j] were written before status[j] was released
                 if ((st & 0xffu) == kStatusPrefix) {
-                    state = nib(my_fn, nib(acc_fn, (st >> 8) & 15u));
+                    incoming = nib(acc_fn, (st >> 8) & 15u);
                     done = true;
                 } else {
-                    acc_fn = nib_then(agg[j], acc_fn);   // earlier chunk applies first
-                    j = cpred[j];
+                    acc_fn = nib_then(agg[j], acc_fn);   // earlier chunks apply first
+                    j = jump[j];
                     if (j < 0) {   // cannot happen: a string's first chunk publishes a prefix
                         atomicOr(error, 2u);
-                        state = dead;
+                        incoming = dead;
                         break;
                     }
                 }
             }
-            store_release(&status[c], kStatusPrefix | (state << 8));
         }
+        // the state before each segment, from its head, then each lane's own
+        incoming = __shfl(incoming, (int)head_lane, 64);
+        const uint32_t state = known ? kstate : fn_apply(g, incoming);
+        if (active && !known) store_release(&status[c], kStatusPrefix | (state << 8));
         if (active && (flags & 2u)) {
             const int32_t sid = csid[c];
             out_state[sid] = (uint8_t)state;
@@ -249,27 +289,27 @@ dfa16_chunks_window(const uint8_t* __restrict__ bytes, const int64_t* __restrict
                     const int64_t* __restrict__ cend, const int32_t* __restrict__ csid,
                     const int32_t* __restrict__ cpred, const uint32_t* __restrict__ cflags, int64_t n_chunks,
                     const uint8_t* __restrict__ t2g, const uint8_t* __restrict__ t1g,
-                    const uint64_t* __restrict__ f2g, const uint8_t* __restrict__ accg, uint32_t win4, uint32_t start, uint32_t dead,
-                    uint32_t n_states, uint32_t* __restrict__ status, uint64_t* __restrict__ agg,
-                    uint32_t* __restrict__ ticket, uint32_t* __restrict__ error, uint32_t max_spins,
-                    uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
+                    const uint64_t* __restrict__ f2g, const uint8_t* __restrict__ accg, uint32_t win4, uint32_t start,
+                    uint32_t dead, uint32_t n_states, uint32_t* __restrict__ status, uint64_t* __restrict__ agg,
+                    int32_t* __restrict__ jump, uint32_t* __restrict__ ticket, uint32_t* __restrict__ error,
+                    uint32_t max_spins, uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
 {
-    chunk_scan<true>(bytes, cbegin, cend, csid, cpred, cflags, n_chunks, t2g, t1g, f2g, accg, win4, start, dead, n_states,
-                     status, agg, ticket, error, max_spins, out_state, out_accept);
+    chunk_scan<true>(bytes, cbegin, cend, csid, cpred, cflags, n_chunks, t2g, t1g, f2g, accg, win4, start, dead,
+                     n_states, status, agg, jump, ticket, error, max_spins, out_state, out_accept);
 }
 
 extern "C" __global__ void __launch_bounds__(TREX_BLOCK)
 dfa16_chunks_bytes(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ cbegin,
                    const int64_t* __restrict__ cend, const int32_t* __restrict__ csid,
                    const int32_t* __restrict__ cpred, const uint32_t* __restrict__ cflags, int64_t n_chunks,
-                   const uint8_t* __restrict__ t2g, const uint8_t* __restrict__ t1g,
-                   const uint64_t* __restrict__ f2g, const uint8_t* __restrict__ accg, uint32_t win4, uint32_t start, uint32_t dead,
-                   uint32_t n_states, uint32_t* __restrict__ status, uint64_t* __restrict__ agg,
+                   const uint8_t* __restrict__ t2g, const uint8_t* __restrict__ t1g, const uint64_t* __restrict__ f2g,
+                   const uint8_t* __restrict__ accg, uint32_t win4, uint32_t start, uint32_t dead, uint32_t n_states,
+                   uint32_t* __restrict__ status, uint64_t* __restrict__ agg, int32_t* __restrict__ jump,
                    uint32_t* __restrict__ ticket, uint32_t* __restrict__ error, uint32_t max_spins,
                    uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
 {
-    chunk_scan<false>(bytes, cbegin, cend, csid, cpred, cflags, n_chunks, t2g, t1g, f2g, accg, win4, start, dead, n_states,
-                      status, agg, ticket, error, max_spins, out_state, out_accept);
+    chunk_scan<false>(bytes, cbegin, cend, csid, cpred, cflags, n_chunks, t2g, t1g, f2g, accg, win4, start, dead,
+                      n_states, status, agg, jump, ticket, error, max_spins, out_state, out_accept);
 }
 
 // Sets dst[0 .. n) = value (status words, tickets, error flags).
