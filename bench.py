@@ -402,6 +402,7 @@ def scan_leg(args, data, stream) -> dict:
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "alg_bytes_per_launch": alg_bytes},
            "verified_bit_exact": None}
+    rf = ra = None
     if not args.no_verify:
         from oracle import cpu as ocpu
         rf, ra = ocpu.dfa_walk(table.delta, table.accept, table.start, data, offsets)
@@ -411,6 +412,35 @@ def scan_leg(args, data, stream) -> dict:
         log(f"look-back scan: bit-exact={same}")
         if not same:
             log("PARITY FAILURE (look-back scan)")
+
+    # the same plan over its chunk-interleaved image (one wave load = 1 KiB)
+    t0 = time.perf_counter()
+    image = plan.build_image(data)
+    t_build = time.perf_counter() - t0
+    d_img = DeviceBuffer(image.nbytes)
+    d_img.upload(image, stream=stream)
+    stream.synchronize()
+    del image
+    ims = time_launches(lambda: eng.match_plan_image(plan, d_img.ptr, d_f.ptr, d_a.ptr, stream), stream,
+                        max(3, args.steps))
+    ierr = plan.error(stream)
+    iach = alg_bytes / (ims * 1e-3) / 1e9
+    log(f"look-back scan over the chunk image ({plan.image_size() >> 20} MiB, built on the host in "
+        f"{t_build:.2f} s): {ims:.4f} ms per launch, {data.nbytes / (ims * 1e-3) / 1e9:.1f} Gchars/s, "
+        f"{iach:.0f} GB/s, error word {ierr}")
+    out["csr"] = {"kernel": "dfa16_chunks_window", "kernel_avg_ms": out["kernel_avg_ms"], "value": out["value"],
+                  "roofline_frac": out["roofline"]["frac"]}
+    out.update({"kernel": "dfa16_chunks_window_img", "kernel_avg_ms": round(ims, 4),
+                "value": round(data.nbytes / (ims * 1e-3) / 1e9, 1), "error_word": int(ierr),
+                "layout": "chunk-interleaved image of the plan (tile of 64 chunks, 16-byte vectors side by side)"})
+    out["roofline"].update({"achieved": round(iach, 1), "frac": round(iach / HBM_PEAK_GBS, 4)})
+    if not args.no_verify:
+        same = bool(np.array_equal(d_f.download(np.uint8, n_long), rf)
+                    and np.array_equal(d_a.download(np.uint8, n_long), ra) and ierr == 0)
+        out["verified_bit_exact"] = bool(out["verified_bit_exact"]) and same
+        log(f"look-back scan (chunk image): bit-exact={same}")
+        if not same:
+            log("PARITY FAILURE (look-back scan, chunk image)")
     # an automaton that never synchronises: every chunk keeps a permutation as
     # its function and resolves through the look-back chain
     from trex_amd.tables import DfaByteTable
@@ -421,10 +451,10 @@ def scan_leg(args, data, stream) -> dict:
     accept[0] = 1
     perm = DfaByteTable(delta, accept, 0, 15, 16)
     peng = DfaEngine(perm)
-    pms = time_launches(lambda: peng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr, stream), stream,
+    pms = time_launches(lambda: peng.match_plan_image(plan, d_img.ptr, d_f.ptr, d_a.ptr, stream), stream,
                         max(3, args.steps))
     perr = plan.error(stream)
-    log(f"look-back scan, sum of symbols mod 15 (never synchronises): {pms:.4f} ms per launch, "
+    log(f"look-back scan over the chunk image, sum of symbols mod 15 (never synchronises): {pms:.4f} ms per launch, "
         f"{data.nbytes / (pms * 1e-3) / 1e9:.1f} Gchars/s, error word {perr}")
     out["permutation_automaton"] = {"automaton": "sum of (byte & 15) mod 15, 16 states with the dead state",
                                     "kernel_avg_ms": round(pms, 4),
@@ -440,6 +470,7 @@ def scan_leg(args, data, stream) -> dict:
         if not same:
             log("PARITY FAILURE (look-back scan, mod 15)")
     peng.close()
+    d_img.free()
     plan.close()
     batch.free()
     eng.close()

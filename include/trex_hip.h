@@ -402,6 +402,25 @@ int trex_plan_info(const trex_plan* plan, int64_t* n_chunks, int64_t* n_strings)
  * trex_plan_error reports it. */
 int trex_dfa_match_plan(trex_dfa* h, trex_plan* plan, const uint8_t* d_bytes, uint8_t* d_final_state,
                         uint8_t* d_accept, uint32_t max_spins, void* stream);
+
+/* The plan's chunk-interleaved image: the same bytes laid out so that one
+ * wave load reads 1 KiB contiguously. Tile t holds the plan's chunks 64t ..
+ * 64t+63 (processing order); vector i (16 bytes) of the tile's k-th chunk
+ * lies at tile_off[t] + (64 i + k) * 16, a chunk's last partial vector
+ * zero-padded, each tile as long as its longest chunk. The image form of
+ * the HBM layout for long strings, as interleaved rows are for batches of
+ * similar lengths (same matching as monoid_r3_kernel, cuda/monoid_scan.cu:
+ * 128-222; no counterpart of the layout in the reference).
+ * trex_plan_image_size: bytes of the image. trex_plan_build_image: fill it on
+ * the host from `src`, the host copy of what the plan's ranges index
+ * (src_len bytes); dst holds at least the image size. */
+int trex_plan_image_size(const trex_plan* plan, int64_t* bytes);
+int trex_plan_build_image(const trex_plan* plan, const uint8_t* src, int64_t src_len, uint8_t* dst,
+                          int64_t dst_len);
+/* trex_dfa_match_plan over the plan's image in device memory (d_image, built
+ * by trex_plan_build_image for this plan and uploaded). */
+int trex_dfa_match_plan_image(trex_dfa* h, trex_plan* plan, const uint8_t* d_image, uint8_t* d_final_state,
+                              uint8_t* d_accept, uint32_t max_spins, void* stream);
 /* Error word of the plan's last launch (0 = none; bit 0 = a wait expired),
  * read synchronously after the work queued on `stream`. */
 int trex_plan_error(trex_plan* plan, uint32_t* error, void* stream);

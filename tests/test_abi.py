@@ -53,7 +53,9 @@ def test_kernel_module_built_for_gfx950():
         blob += data
     for kernel in [b"dfa16_k2w_pf16", b"dfa16_k2f_pf16", b"dfa16_k1", b"dfa_k1", b"copy_u4", b"pnfa64_f16x2",
                    b"pnfa64_bf16x2", b"dfa16_multi_k2f_p4_il", b"dfa16_multi_k2w_p8", b"nfa64_window6",
-                   b"nfa64_window5", b"mix_persistent_slice", b"pnfa64_chunk_f16x2", b"pnfa64_combine"]:
+                   b"nfa64_window5", b"mix_persistent_slice", b"pnfa64_chunk_f16x2", b"pnfa64_combine",
+                   b"dfa16_chunks_window", b"dfa16_chunks_bytes", b"dfa16_chunks_window_img",
+                   b"dfa16_chunks_bytes_img"]:
         assert kernel in blob
 
 

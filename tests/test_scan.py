@@ -192,3 +192,64 @@ def test_plan_byte_kernel_two_windows(gpu_lib, chunk_len):
     f, a, n_chunks = _run_plan(tab, np.ascontiguousarray(data), offs, chunk_len)
     rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offs)
     assert np.array_equal(f, rf) and np.array_equal(a, ra)
+
+
+def _run_plan_image(tab, data, offsets, chunk_len):
+    """Match through the plan's chunk-interleaved image; also returns the
+    plain (CSR) results of the same plan."""
+    from trex_amd.device import DeviceBuffer
+    from trex_amd.engine import ChunkPlan, DeviceBatch, DfaEngine
+    eng = DfaEngine(tab)
+    batch = DeviceBatch(data, offsets, layout="csr")
+    plan = ChunkPlan.for_batch(batch, chunk_len)
+    image = plan.build_image(data)
+    assert plan.image_size() >= len(data) and plan.image_size() % 1024 == 0
+    d_img = DeviceBuffer(image.nbytes)
+    d_img.upload(image)
+    n = len(offsets) - 1
+    d_f, d_a = DeviceBuffer(max(n, 1)), DeviceBuffer(max(n, 1))
+    eng.match_plan_image(plan, d_img.ptr, d_f.ptr, d_a.ptr)
+    assert plan.error() == 0
+    fi, ai = d_f.download(np.uint8, n), d_a.download(np.uint8, n)
+    eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr)
+    assert plan.error() == 0
+    return fi, ai, d_f.download(np.uint8, n), d_a.download(np.uint8, n)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pattern,chunk_len", [("random32", 1024), ("random32", 8192), (EVEN_A, 64),
+                                               (EVEN_A, 4096), ("(a|B)*aBB", 512)])
+def test_plan_image_matches_oracle(gpu_lib, pattern, chunk_len):
+    """The chunk-interleaved image gives the sequential walk's results (and
+    the CSR launch's) over ragged lengths, partial vectors, empty strings,
+    synchronising and permutation automata and the byte kernel."""
+    rng = np.random.default_rng(chunk_len + len(pattern))
+    if pattern == "random32":
+        _, dfa = random_dfa_pattern(rng, ALPHABET_32)
+        alphabet = bytes(range(0x40, 0x60))
+    else:
+        dfa = compile_regex(pattern)
+        alphabet = b"ab" if pattern == EVEN_A else b"aBB"
+    tab = dfa_byte_table(dfa)
+    lengths = np.concatenate([rng.integers(0, 300, 300), rng.integers(1000, 40000, 40),
+                              [0, 1, 15, 16, 17, chunk_len - 1, chunk_len, chunk_len + 1, 3 * chunk_len + 5,
+                               300_001]])
+    rng.shuffle(lengths)
+    offs = np.zeros(len(lengths) + 1, np.int64)
+    np.cumsum(lengths, out=offs[1:])
+    data = np.frombuffer(alphabet, np.uint8)[rng.integers(0, len(alphabet), int(offs[-1]))].copy()
+    data[rng.integers(0, len(data), 8)] = 0x7e
+    fi, ai, fc, ac = _run_plan_image(tab, data, offs, chunk_len)
+    rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offs)
+    assert np.array_equal(fi, rf) and np.array_equal(ai, ra)
+    assert np.array_equal(fc, rf) and np.array_equal(ac, ra)
+
+
+@pytest.mark.gpu
+def test_plan_image_size_equal_lengths(gpu_lib):
+    """Strings of one length that is a multiple of the chunk length: the image
+    is exactly the bytes (no padding)."""
+    from trex_amd.engine import ChunkPlan
+    begin = np.arange(4, dtype=np.int64) * (1 << 20)
+    plan = ChunkPlan(begin, begin + (1 << 20), 8192)
+    assert plan.n_chunks == 512 and plan.image_size() == 4 << 20

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cstring>
 #include <numeric>
 #include <vector>
 
@@ -18,6 +19,11 @@ struct trex_plan {
     int64_t chunk_len = 0;
     DevBuf begin, end, sid, pred, flags;   // per chunk, in processing order
     DevBuf status, agg, jump, control;     // look-back state; control = {ticket, error}
+    // chunk-interleaved image (see trex_plan_build_image): tile t = chunks
+    // 64t .. 64t+63 in processing order, tile_vecs[t] 16-byte vectors per lane
+    std::vector<int64_t> h_begin, h_end;   // chunk ranges in processing order (host copy)
+    std::vector<int64_t> h_tile_off;       // byte offset of each tile in the image, n_tiles + 1 entries
+    DevBuf tile_off;                       // the same offsets on the device
     std::mutex mu;                         // one launch at a time per plan
 };
 
@@ -88,6 +94,24 @@ int trex_plan_create(const int64_t* begin, const int64_t* end, int64_t n_strings
     if (!rc) rc = p->agg.reserve(m * 8);
     if (!rc) rc = p->jump.reserve(m * 4);
     if (!rc) rc = p->control.reserve(16);
+    // image tiles: 64 chunks each, as many 16-byte vectors per lane as the
+    // tile's longest chunk needs (chunks are sorted longest first, so a tile
+    // pads little)
+    const int64_t n_tiles = (n + 63) / 64;
+    p->h_tile_off.assign((size_t)n_tiles + 1, 0);
+    for (int64_t t = 0; t < n_tiles; ++t) {
+        int64_t longest = 0;
+        for (int64_t k = t * 64; k < std::min(n, t * 64 + 64); ++k) longest = std::max(longest, oe[k] - ob[k]);
+        p->h_tile_off[t + 1] = p->h_tile_off[t] + (longest + 15) / 16 * 16 * 64;
+    }
+    if (!rc) rc = p->tile_off.reserve(p->h_tile_off.size() * 8);
+    if (!rc) {
+        hipError_t e = hipMemcpy(p->tile_off.ptr, p->h_tile_off.data(), p->h_tile_off.size() * 8,
+                                 hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = hip_fail(e, "plan upload");
+    }
+    p->h_begin = ob;
+    p->h_end = oe;
     if (!rc && n > 0) {
         hipError_t e = hipMemcpy(p->begin.ptr, ob.data(), (size_t)n * 8, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(p->end.ptr, oe.data(), (size_t)n * 8, hipMemcpyHostToDevice);
@@ -103,7 +127,8 @@ int trex_plan_create(const int64_t* begin, const int64_t* end, int64_t n_strings
 
 int trex_plan_destroy(trex_plan* p) {
     if (!p) return TREX_OK;
-    for (DevBuf* b : {&p->begin, &p->end, &p->sid, &p->pred, &p->flags, &p->status, &p->agg, &p->jump, &p->control})
+    for (DevBuf* b : {&p->begin, &p->end, &p->sid, &p->pred, &p->flags, &p->status, &p->agg, &p->jump, &p->control,
+                      &p->tile_off})
         b->release();
     delete p;
     return TREX_OK;
@@ -116,8 +141,14 @@ int trex_plan_info(const trex_plan* p, int64_t* n_chunks, int64_t* n_strings) {
     return TREX_OK;
 }
 
-int trex_dfa_match_plan(trex_dfa* h, trex_plan* p, const uint8_t* d_bytes, uint8_t* d_final_state,
-                        uint8_t* d_accept, uint32_t max_spins, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// One launch over all chunks of the plan: `d_bytes` is what the plan's ranges
+// index, or (image) the plan's chunk-interleaved image.
+int match_plan(trex_dfa* h, trex_plan* p, const uint8_t* d_bytes, bool image, uint8_t* d_final_state,
+               uint8_t* d_accept, uint32_t max_spins, void* stream) {
     if (!h || !p) return fail(TREX_ERR_INVALID, "null handle or plan");
     if (h->n_states > 16) return fail(TREX_ERR_INVALID, "chunked matching needs a table of at most 16 states");
     if (p->n_chunks == 0) return TREX_OK;
@@ -148,15 +179,58 @@ int trex_dfa_match_plan(trex_dfa* h, trex_plan* p, const uint8_t* d_bytes, uint8
     int64_t n_chunks = p->n_chunks;
     uint32_t* ticket = p->control.as<uint32_t>();
     uint32_t* error = ticket + 1;
-    void* args[] = {&d_bytes, &p->begin.ptr, &p->end.ptr, &p->sid.ptr, &p->pred.ptr, &p->flags.ptr, &n_chunks,
-                    &t2, &h->d_t1, &f2, &h->d_acc, &win4, &start, &dead, &n_states, &p->status.ptr, &p->agg.ptr,
-                    &p->jump.ptr, &ticket,
-                    &error, &max_spins, &d_final_state, &d_accept};
+    void* args[] = {&d_bytes, &p->tile_off.ptr, &p->begin.ptr, &p->end.ptr, &p->sid.ptr, &p->pred.ptr,
+                    &p->flags.ptr, &n_chunks, &t2, &h->d_t1, &f2, &h->d_acc, &win4, &start, &dead, &n_states,
+                    &p->status.ptr, &p->agg.ptr, &p->jump.ptr, &ticket, &error, &max_spins, &d_final_state,
+                    &d_accept};
     // persistent grid: enough waves to fill every CU, never more than the work
     const int64_t waves_needed = (n_chunks + 63) / 64;
     const int64_t blocks = std::min<int64_t>((waves_needed + 3) / 4, (int64_t)device_cu_count() * 8);
-    return launch(window ? "dfa16_chunks_window" : "dfa16_chunks_bytes", (unsigned)std::max<int64_t>(blocks, 1),
-                  kBlock, 0, s, args);
+    const char* kernel = window ? (image ? "dfa16_chunks_window_img" : "dfa16_chunks_window")
+                                : (image ? "dfa16_chunks_bytes_img" : "dfa16_chunks_bytes");
+    return launch(kernel, (unsigned)std::max<int64_t>(blocks, 1), kBlock, 0, s, args);
+}
+
+}  // namespace
+
+extern "C" {
+
+int trex_dfa_match_plan(trex_dfa* h, trex_plan* p, const uint8_t* d_bytes, uint8_t* d_final_state,
+                        uint8_t* d_accept, uint32_t max_spins, void* stream) {
+    return match_plan(h, p, d_bytes, false, d_final_state, d_accept, max_spins, stream);
+}
+
+int trex_plan_image_size(const trex_plan* p, int64_t* bytes) {
+    if (!p || !bytes) return fail(TREX_ERR_INVALID, "null argument");
+    *bytes = p->h_tile_off.empty() ? 0 : p->h_tile_off.back();
+    return TREX_OK;
+}
+
+int trex_plan_build_image(const trex_plan* p, const uint8_t* src, int64_t src_len, uint8_t* dst, int64_t dst_len) {
+    if (!p || (p->n_chunks > 0 && (!src || !dst))) return fail(TREX_ERR_INVALID, "null argument");
+    const int64_t size = p->h_tile_off.empty() ? 0 : p->h_tile_off.back();
+    if (dst_len < size) return fail(TREX_ERR_INVALID, "image buffer holds %lld bytes, the plan needs %lld",
+                                    (long long)dst_len, (long long)size);
+    const int64_t n = p->n_chunks;
+    for (int64_t c = 0; c < n; ++c)
+        if (p->h_begin[c] < 0 || p->h_end[c] > src_len)
+            return fail(TREX_ERR_INVALID, "chunk %lld lies outside the %lld source bytes", (long long)c,
+                        (long long)src_len);
+    if (size > 0) std::memset(dst, 0, (size_t)size);
+    for (int64_t c = 0; c < n; ++c) {
+        uint8_t* out = dst + p->h_tile_off[(size_t)(c >> 6)] + 16 * (c & 63);
+        const uint8_t* in = src + p->h_begin[c];
+        const int64_t len = p->h_end[c] - p->h_begin[c];
+        int64_t i = 0;
+        for (; i + 16 <= len; i += 16) std::memcpy(out + i * 64, in + i, 16);
+        if (i < len) std::memcpy(out + i * 64, in + i, (size_t)(len - i));
+    }
+    return TREX_OK;
+}
+
+int trex_dfa_match_plan_image(trex_dfa* h, trex_plan* p, const uint8_t* d_image, uint8_t* d_final_state,
+                              uint8_t* d_accept, uint32_t max_spins, void* stream) {
+    return match_plan(h, p, d_image, true, d_final_state, d_accept, max_spins, stream);
 }
 
 int trex_plan_error(trex_plan* p, uint32_t* error, void* stream) {

@@ -99,8 +99,13 @@ struct Scanner {
 // for its arguments) and its first run did not finish; the cause was not
 // found, so the depth stays at 8, where both kernels inline (<= 106 VGPRs, no
 // stack) and pass every test.
-template <bool WINDOW>
-__device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ cbegin,
+// IMAGE: `bytes` is the chunk-interleaved image of the plan (tile t = the
+// ticket's 64 chunks at byte tile_off[t], vector i of lane k's chunk at
+// tile_off[t] + (64 i + k) * 16, so one wave load reads 1 KiB contiguously);
+// otherwise the bytes the plan's ranges index.
+template <bool WINDOW, bool IMAGE>
+__device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ tile_off,
+                                           const int64_t* __restrict__ cbegin,
                                            const int64_t* __restrict__ cend, const int32_t* __restrict__ csid,
                                            const int32_t* __restrict__ cpred, const uint32_t* __restrict__ cflags,
                                            int64_t n_chunks, const uint8_t* __restrict__ t2g,
@@ -170,17 +175,31 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
         sc.f = fn_identity_on(n_states);
 
         // ---- phase 1: walk or compose this chunk's bytes ----
-        const int64_t head_end = min(end, p + (int64_t)((16u - ((uint32_t)(uintptr_t)(bytes + p) & 15u)) & 15u));
-        for (; p < head_end; ++p) sc.byte(bytes[p]);
-        // body: 16-byte vectors, kScanPF of them in flight while the previous
-        // ones are walked (one lane's loads are its own chunk's next bytes)
-        const int64_t nvec = (end - p) >> 4;
-        stream_body<kScanPF>(reinterpret_cast<const uint4*>(bytes + p), nvec, [&](const uint4& a) {
+        auto vector = [&](const uint4& a) {
             sc.word(a.x); sc.word(a.y); sc.word(a.z); sc.word(a.w);
             sc.settle();
-        });
-        p += nvec << 4;
-        for (; p < end; ++p) sc.byte(bytes[p]);
+        };
+        if constexpr (IMAGE) {
+            // vectors of this lane's chunk are 1 KiB apart; the tail bytes sit
+            // at the start of the next vector slot
+            const int64_t len = end - p;
+            const uint8_t* mine = bytes + tile_off[base >> 6] + 16u * lane;
+            const int64_t nvec = len >> 4;
+            stream_body<kScanPF, 64>(reinterpret_cast<const uint4*>(mine), nvec, vector);
+            const uint8_t* tail = mine + nvec * 1024;
+            for (int64_t k = 0; k < (len & 15); ++k) sc.byte(tail[k]);
+        } else {
+            const int64_t head_end =
+                min(end, p + (int64_t)((16u - ((uint32_t)(uintptr_t)(bytes + p) & 15u)) & 15u));
+            for (; p < head_end; ++p) sc.byte(bytes[p]);
+            // body: 16-byte vectors, kScanPF of them in flight while the
+            // previous ones are walked (one lane's loads are its own chunk's
+            // next bytes)
+            const int64_t nvec = (end - p) >> 4;
+            stream_body<kScanPF>(reinterpret_cast<const uint4*>(bytes + p), nvec, vector);
+            p += nvec << 4;
+            for (; p < end; ++p) sc.byte(bytes[p]);
+        }
         sc.settle();
         if constexpr (WINDOW) {
             if (sc.w.outside(win4)) { sc.walking = true; sc.s = dead; }
@@ -284,32 +303,32 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
 
 }  // namespace
 
-extern "C" __global__ void __launch_bounds__(TREX_BLOCK)
-dfa16_chunks_window(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ cbegin,
-                    const int64_t* __restrict__ cend, const int32_t* __restrict__ csid,
-                    const int32_t* __restrict__ cpred, const uint32_t* __restrict__ cflags, int64_t n_chunks,
-                    const uint8_t* __restrict__ t2g, const uint8_t* __restrict__ t1g,
-                    const uint64_t* __restrict__ f2g, const uint8_t* __restrict__ accg, uint32_t win4, uint32_t start,
-                    uint32_t dead, uint32_t n_states, uint32_t* __restrict__ status, uint64_t* __restrict__ agg,
-                    int32_t* __restrict__ jump, uint32_t* __restrict__ ticket, uint32_t* __restrict__ error,
-                    uint32_t max_spins, uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
-{
-    chunk_scan<true>(bytes, cbegin, cend, csid, cpred, cflags, n_chunks, t2g, t1g, f2g, accg, win4, start, dead,
-                     n_states, status, agg, jump, ticket, error, max_spins, out_state, out_accept);
-}
+#define TREX_CHUNK_PARAMS                                                                                 \
+    const uint8_t *__restrict__ bytes, const int64_t *__restrict__ tile_off, const int64_t *__restrict__ cbegin, \
+        const int64_t *__restrict__ cend, const int32_t *__restrict__ csid, const int32_t *__restrict__ cpred,  \
+        const uint32_t *__restrict__ cflags, int64_t n_chunks, const uint8_t *__restrict__ t2g,               \
+        const uint8_t *__restrict__ t1g, const uint64_t *__restrict__ f2g, const uint8_t *__restrict__ accg,   \
+        uint32_t win4, uint32_t start, uint32_t dead, uint32_t n_states, uint32_t *__restrict__ status,        \
+        uint64_t *__restrict__ agg, int32_t *__restrict__ jump, uint32_t *__restrict__ ticket,                 \
+        uint32_t *__restrict__ error, uint32_t max_spins, uint8_t *__restrict__ out_state,                     \
+        uint8_t *__restrict__ out_accept
+#define TREX_CHUNK_ARGS                                                                                     \
+    bytes, tile_off, cbegin, cend, csid, cpred, cflags, n_chunks, t2g, t1g, f2g, accg, win4, start, dead,   \
+        n_states, status, agg, jump, ticket, error, max_spins, out_state, out_accept
 
-extern "C" __global__ void __launch_bounds__(TREX_BLOCK)
-dfa16_chunks_bytes(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ cbegin,
-                   const int64_t* __restrict__ cend, const int32_t* __restrict__ csid,
-                   const int32_t* __restrict__ cpred, const uint32_t* __restrict__ cflags, int64_t n_chunks,
-                   const uint8_t* __restrict__ t2g, const uint8_t* __restrict__ t1g, const uint64_t* __restrict__ f2g,
-                   const uint8_t* __restrict__ accg, uint32_t win4, uint32_t start, uint32_t dead, uint32_t n_states,
-                   uint32_t* __restrict__ status, uint64_t* __restrict__ agg, int32_t* __restrict__ jump,
-                   uint32_t* __restrict__ ticket, uint32_t* __restrict__ error, uint32_t max_spins,
-                   uint8_t* __restrict__ out_state, uint8_t* __restrict__ out_accept)
-{
-    chunk_scan<false>(bytes, cbegin, cend, csid, cpred, cflags, n_chunks, t2g, t1g, f2g, accg, win4, start, dead,
-                      n_states, status, agg, jump, ticket, error, max_spins, out_state, out_accept);
+// Chunks of any string, bytes as the plan's ranges index them (tile_off unused).
+extern "C" __global__ void __launch_bounds__(TREX_BLOCK) dfa16_chunks_window(TREX_CHUNK_PARAMS) {
+    chunk_scan<true, false>(TREX_CHUNK_ARGS);
+}
+extern "C" __global__ void __launch_bounds__(TREX_BLOCK) dfa16_chunks_bytes(TREX_CHUNK_PARAMS) {
+    chunk_scan<false, false>(TREX_CHUNK_ARGS);
+}
+// The same over the plan's chunk-interleaved image.
+extern "C" __global__ void __launch_bounds__(TREX_BLOCK) dfa16_chunks_window_img(TREX_CHUNK_PARAMS) {
+    chunk_scan<true, true>(TREX_CHUNK_ARGS);
+}
+extern "C" __global__ void __launch_bounds__(TREX_BLOCK) dfa16_chunks_bytes_img(TREX_CHUNK_PARAMS) {
+    chunk_scan<false, true>(TREX_CHUNK_ARGS);
 }
 
 // Sets dst[0 .. n) = value (status words, tickets, error flags).

@@ -119,6 +119,14 @@ class DfaEngine:
         _lib.call("trex_dfa_match_plan", self.handle, plan.handle, d_bytes, d_final, d_accept, int(max_spins),
                   stream.handle if stream else None)
 
+    def match_plan_image(self, plan: "ChunkPlan", d_image: int, d_final: int, d_accept: int,
+                         stream: Stream | None = None, max_spins: int = 0) -> None:
+        """``match_plan`` over the plan's chunk-interleaved image in device
+        memory (``ChunkPlan.build_image``, uploaded): one wave load reads
+        1 KiB contiguously."""
+        _lib.call("trex_dfa_match_plan_image", self.handle, plan.handle, d_image, d_final, d_accept,
+                  int(max_spins), stream.handle if stream else None)
+
     def match_arrays(self, data: np.ndarray, offsets: np.ndarray):
         """Host CSR batch -> (final_state uint8[n], accept uint8[n], kernel_ms, total_ms)."""
         data, offsets = _as_csr(data=data, offsets=offsets)
@@ -594,6 +602,21 @@ class ChunkPlan:
     @classmethod
     def for_batch(cls, batch: "DeviceBatch", chunk_len: int | None = None) -> "ChunkPlan":
         return cls(batch.begin, batch.end, chunk_len)
+
+    def image_size(self) -> int:
+        """Bytes of the plan's chunk-interleaved image."""
+        size = ctypes.c_int64()
+        _lib.call("trex_plan_image_size", self.handle, ctypes.byref(size))
+        return size.value
+
+    def build_image(self, data: np.ndarray) -> np.ndarray:
+        """The plan's chunk-interleaved image of ``data`` (the host bytes its
+        ranges index): tile t = chunks 64t .. 64t+63 in processing order,
+        vector i of the tile's k-th chunk at tile_off[t] + (64 i + k) * 16."""
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        out = np.empty(max(self.image_size(), 16), np.uint8)
+        _lib.call("trex_plan_build_image", self.handle, data.ctypes.data, data.nbytes, out.ctypes.data, out.nbytes)
+        return out
 
     def error(self, stream: Stream | None = None) -> int:
         err = ctypes.c_uint32()
