@@ -364,8 +364,9 @@ def dfa_leg(args, data, offsets, batch, stream, result: dict, copy_gbs) -> int:
 
 def scan_leg(args, data, stream) -> dict:
     """Long strings through the decoupled look-back scan: the 1 GiB batch read
-    as 16 strings of 64 MiB, cut into chunks of lookback_chunk_len (8 KiB
-    here; one launch for all chunks), with the DFA leg's first pattern."""
+    as 16 strings of 64 MiB, cut into chunks of lookback_chunk_len (16 KiB
+    here; one launch for all chunks), with the DFA leg's first pattern,
+    directly and over the plan's chunk-interleaved image."""
     from trex_amd.device import DeviceBuffer
     from trex_amd.engine import ChunkPlan, DeviceBatch, DfaEngine
     from trex_amd.tables import dfa_byte_table
@@ -379,12 +380,21 @@ def scan_leg(args, data, stream) -> dict:
     batch = DeviceBatch(data, offsets, layout="csr")
     d_f, d_a = DeviceBuffer(n_long), DeviceBuffer(n_long)
     if args.scan_sweep:
-        for clen in (2048, 4096, 6144, 7936, 8192, 8448, 10240, 12288, 16384, 32768):
+        for clen in (4096, 8192, 12288, 16384, 20480, 24576, 32768, 65536):
             pl = ChunkPlan.for_batch(batch, clen)
             t = time_launches(lambda: eng.match_plan(pl, batch.d_bytes.ptr, d_f.ptr, d_a.ptr, stream), stream,
                               max(3, args.steps))
-            log(f"look-back scan sweep: {clen:6d} B chunks ({pl.n_chunks} chunks): {t:.4f} ms, "
-                f"{data.nbytes / (t * 1e-3) / 1e9:.0f} Gchars/s, error word {pl.error(stream)}")
+            img = pl.build_image(data)
+            d_im = DeviceBuffer(img.nbytes)
+            d_im.upload(img, stream=stream)
+            stream.synchronize()
+            del img
+            ti = time_launches(lambda: eng.match_plan_image(pl, d_im.ptr, d_f.ptr, d_a.ptr, stream), stream,
+                               max(3, args.steps))
+            log(f"look-back scan sweep: {clen:6d} B chunks ({pl.n_chunks} chunks): {t:.4f} ms "
+                f"({data.nbytes / (t * 1e-3) / 1e9:.0f} Gchars/s), chunk image {ti:.4f} ms "
+                f"({data.nbytes / (ti * 1e-3) / 1e9:.0f} Gchars/s), error word {pl.error(stream)}")
+            d_im.free()
             pl.close()
     plan = ChunkPlan.for_batch(batch)
     ms = time_launches(lambda: eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr, stream), stream,

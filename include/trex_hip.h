@@ -389,8 +389,8 @@ typedef struct trex_plan trex_plan;
 
 /* begin/end: host int64 byte ranges of n_strings strings (in the device byte
  * buffer the plan will be used with). chunk_len: multiple of 16, >= 64; the
- * launch is fastest with about 131,072 chunks in all (two waves of 64 chunk
- * lanes per SIMD), e.g. 8 KiB for 1 GiB of long strings. */
+ * launch is fastest with about 65,536 chunks in all (one wave of 64 chunk
+ * lanes per SIMD), e.g. 16 KiB for 1 GiB of long strings. */
 int trex_plan_create(const int64_t* begin, const int64_t* end, int64_t n_strings, int64_t chunk_len,
                      trex_plan** out);
 int trex_plan_destroy(trex_plan* plan);

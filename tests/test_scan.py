@@ -122,7 +122,7 @@ def test_plan_even_a_long_strings_parity(gpu_lib):
 
 def test_lookback_chunk_len():
     from trex_amd.engine import LOOKBACK_TARGET_CHUNKS, lookback_chunk_len
-    assert lookback_chunk_len(1 << 30) == 8192
+    assert lookback_chunk_len(1 << 30) == 16384
     assert lookback_chunk_len(0) == 1024 and lookback_chunk_len(1 << 20) == 1024
     assert lookback_chunk_len(1 << 40) == 1 << 16
     for total in (3 << 27, 5 << 28, 1 << 31):

@@ -92,6 +92,33 @@ struct Scanner {
             s = f.d[0] & 0xffu;
         }
     }
+    // One walked word, no mode test: the lookups depend only on the bytes.
+    __device__ __forceinline__ void walk_word(uint32_t x) {
+        if constexpr (WINDOW) {
+            w.add(x);
+            const uint64_t a = f2[pair_lo(x)], b = f2[pair_hi(x)];
+            s = (uint32_t)(a >> (s << 2)) & 15u;
+            s = (uint32_t)(b >> (s << 2)) & 15u;
+        } else {
+            const uint64_t a = f1[x & 255u], b = f1[(x >> 8) & 255u], c = f1[(x >> 16) & 255u], d = f1[x >> 24];
+            s = (uint32_t)(a >> (s << 2)) & 15u;
+            s = (uint32_t)(b >> (s << 2)) & 15u;
+            s = (uint32_t)(c >> (s << 2)) & 15u;
+            s = (uint32_t)(d >> (s << 2)) & 15u;
+        }
+    }
+    // One 16-byte vector. The mode test is made once per vector and for the
+    // whole wave: when every active lane walks, the branch-free path lets the
+    // vector's table reads issue together (a per-lane test around each word
+    // made every word wait for its own reads).
+    __device__ __forceinline__ void vector(const uint4& a) {
+        if (__ballot(!walking) == 0ull) {
+            walk_word(a.x); walk_word(a.y); walk_word(a.z); walk_word(a.w);
+        } else {
+            word(a.x); word(a.y); word(a.z); word(a.w);
+            settle();
+        }
+    }
 };
 
 // Always inlined into its two kernels. With 16 vectors in flight the byte
@@ -175,10 +202,7 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
         sc.f = fn_identity_on(n_states);
 
         // ---- phase 1: walk or compose this chunk's bytes ----
-        auto vector = [&](const uint4& a) {
-            sc.word(a.x); sc.word(a.y); sc.word(a.z); sc.word(a.w);
-            sc.settle();
-        };
+        auto vector = [&](const uint4& a) { sc.vector(a); };
         if constexpr (IMAGE) {
             // vectors of this lane's chunk are 1 KiB apart; the tail bytes sit
             // at the start of the next vector slot

@@ -558,19 +558,19 @@ class ProbNfaEngine:
             pass
 
 
-# Chunks the look-back scan aims for: two waves of 64 chunk lanes per SIMD
-# (1,024 SIMDs on 256 compute units). One lane walks one chunk with a
-# dependent LDS lookup per byte pair, so it is the count of chunks in flight
-# that hides that latency; smaller chunks add per-chunk work (metadata, the
-# composed prefix until the function turns constant, the status word).
-LOOKBACK_TARGET_CHUNKS = 2 * 1024 * 64
+# Chunks the look-back scan aims for: one wave of 64 chunk lanes per SIMD
+# (1,024 SIMDs on 256 compute units). A walking wave issues a whole vector's
+# table reads at once, so one wave per SIMD keeps it busy; smaller chunks add
+# per-chunk work (metadata, the composed prefix until the function turns
+# constant, the status word) and more, shorter HBM streams.
+LOOKBACK_TARGET_CHUNKS = 1024 * 64
 
 
 def lookback_chunk_len(total_bytes: int, target_chunks: int = LOOKBACK_TARGET_CHUNKS) -> int:
     """Chunk length for ``total_bytes`` of long strings: the power of two
     nearest ``total_bytes / target_chunks``, within 1 KiB .. 64 KiB. 1 GiB
-    gives 8 KiB (0.354 ms per GiB against 1.69 ms at 64 KiB and 1.04 ms at
-    1 KiB on one MI355X, 16 strings of 64 MiB)."""
+    gives 16 KiB (16 strings of 64 MiB on one MI355X: 0.25-0.27 ms per GiB,
+    against 0.28-0.32 ms at 8 KiB and 0.80 ms at 64 KiB)."""
     want = max(int(total_bytes), 1) / target_chunks
     k = int(round(np.log2(max(want, 1.0))))
     return int(min(max(1 << k, 1 << 10), 1 << 16))
