@@ -158,6 +158,22 @@ struct ChunkScanner {
             s = f.d[0] & 0xffu;
         }
     }
+    // One 16-byte vector, walked or composed as decided once for the whole
+    // wave: when every active lane walks, the branch-free path issues the
+    // vector's pair-function reads together (a per-lane test around each
+    // word made every word wait for its own reads).
+    __device__ __forceinline__ void vector(const uint4& a) {
+        if (__ballot(!walking) == 0ull) {
+            w.add(a.x); w.add(a.y); w.add(a.z); w.add(a.w);
+            s = k2f_word(L->f2, s, a.x);
+            s = k2f_word(L->f2, s, a.y);
+            s = k2f_word(L->f2, s, a.z);
+            s = k2f_word(L->f2, s, a.w);
+        } else {
+            word(a.x); word(a.y); word(a.z); word(a.w);
+            settle();
+        }
+    }
 };
 
 __device__ void dfa_chunk(const MixArgs& A, const MixPattern& P, const DfaLds& L, int64_t item, bool active) {
@@ -178,20 +194,7 @@ __device__ void dfa_chunk(const MixArgs& A, const MixPattern& P, const DfaLds& L
     const int64_t head_end = min(end, p + head_len(bytes, p));
     for (; p < head_end; ++p) sc.byte(bytes[p]);
     const int64_t nvec = (end - p) >> 4;
-    const uint4* v = reinterpret_cast<const uint4*>(bytes + p);
-    int64_t i = 0;
-    for (; i + 4 <= nvec; i += 4) {
-        const uint4 a0 = v[i], a1 = v[i + 1], a2 = v[i + 2], a3 = v[i + 3];
-        sc.word(a0.x); sc.word(a0.y); sc.word(a0.z); sc.word(a0.w);
-        sc.word(a1.x); sc.word(a1.y); sc.word(a1.z); sc.word(a1.w);
-        sc.word(a2.x); sc.word(a2.y); sc.word(a2.z); sc.word(a2.w);
-        sc.word(a3.x); sc.word(a3.y); sc.word(a3.z); sc.word(a3.w);
-        sc.settle();
-    }
-    for (; i < nvec; ++i) {
-        const uint4 a = v[i];
-        sc.word(a.x); sc.word(a.y); sc.word(a.z); sc.word(a.w);
-    }
+    stream_body<4>(reinterpret_cast<const uint4*>(bytes + p), nvec, [&](const uint4& a) { sc.vector(a); });
     p += nvec << 4;
     for (; p < end; ++p) sc.byte(bytes[p]);
     sc.settle();
