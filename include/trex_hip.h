@@ -424,6 +424,14 @@ int trex_dfa_match_plan_image(trex_dfa* h, trex_plan* plan, const uint8_t* d_ima
 /* Error word of the plan's last launch (0 = none; bit 0 = a wait expired),
  * read synchronously after the work queued on `stream`. */
 int trex_plan_error(trex_plan* plan, uint32_t* error, void* stream);
+/* Test hook, no counterpart in the reference (its tests ask for the
+ * look-back under an adversarial tile order, which in-order tickets rule
+ * out): later launches of the plan hold back each tile's publication by
+ * hold x (7 - t mod 8) sleeps of 512 clocks, so tiles publish in reverse
+ * order within every group of 8 and successors wait for and walk back over
+ * unpublished predecessors (at most about 3 ms per tile). 0 (the default) =
+ * off; at most 2000. */
+int trex_plan_set_hold(trex_plan* plan, uint32_t hold);
 
 /* Plain device copy with the engine's streaming-load kernel (16-byte aligned
  * buffers, size a multiple of 16): the reference point for HBM traffic

@@ -253,3 +253,67 @@ def test_plan_image_size_equal_lengths(gpu_lib):
     begin = np.arange(4, dtype=np.int64) * (1 << 20)
     plan = ChunkPlan(begin, begin + (1 << 20), 8192)
     assert plan.n_chunks == 512 and plan.image_size() == 4 << 20
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pattern", [EVEN_A, "random32"])
+def test_plan_delayed_publication(gpu_lib, pattern):
+    """Tiles publish in reverse order within groups of 8 (the plan's hold
+    hook), so successors wait for and walk back over unpublished
+    predecessors: results still equal the sequential walk, both layouts."""
+    from trex_amd.device import DeviceBuffer
+    from trex_amd.engine import ChunkPlan, DeviceBatch, DfaEngine
+    rng = np.random.default_rng(77)
+    if pattern == "random32":
+        _, dfa = random_dfa_pattern(rng, ALPHABET_32)
+        lo, hi = 0x40, 0x60
+    else:
+        dfa = compile_regex(pattern)
+        lo, hi = ord("a"), ord("c")
+    tab = dfa_byte_table(dfa)
+    lengths = np.array([400_000, 250_003, 64, 0, 99_999, 1, 150_000], np.int64)
+    data, offs = ragged_batch(rng, lengths, lo, hi)
+    eng = DfaEngine(tab)
+    batch = DeviceBatch(data, offs, layout="csr")
+    plan = ChunkPlan.for_batch(batch, 256)
+    assert plan.n_chunks > 64 * 16
+    plan.set_hold(40)
+    n = len(lengths)
+    d_f, d_a = DeviceBuffer(n), DeviceBuffer(n)
+    rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offs)
+    eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr)
+    assert plan.error() == 0
+    assert np.array_equal(d_f.download(np.uint8, n), rf) and np.array_equal(d_a.download(np.uint8, n), ra)
+    image = plan.build_image(data)
+    d_img = DeviceBuffer(image.nbytes)
+    d_img.upload(image)
+    eng.match_plan_image(plan, d_img.ptr, d_f.ptr, d_a.ptr)
+    assert plan.error() == 0
+    assert np.array_equal(d_f.download(np.uint8, n), rf) and np.array_equal(d_a.download(np.uint8, n), ra)
+
+
+@pytest.mark.gpu
+def test_plan_wait_expiry_reported(gpu_lib):
+    """A spin limit far below the hold makes waits expire: the launch still
+    ends, and the error word says so (failure forced from the host)."""
+    from trex_amd import _lib
+    from trex_amd.device import DeviceBuffer
+    from trex_amd.engine import ChunkPlan, DeviceBatch, DfaEngine
+    tab = dfa_byte_table(compile_regex(EVEN_A))
+    rng = np.random.default_rng(3)
+    lengths = np.array([200_000, 100_000], np.int64)
+    data, offs = ragged_batch(rng, lengths, ord("a"), ord("c"))
+    eng = DfaEngine(tab)
+    batch = DeviceBatch(data, offs, layout="csr")
+    plan = ChunkPlan.for_batch(batch, 256)
+    plan.set_hold(200)
+    d_f, d_a = DeviceBuffer(2), DeviceBuffer(2)
+    eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr, max_spins=4)
+    assert plan.error() & 1
+    plan.set_hold(0)
+    eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr)
+    assert plan.error() == 0
+    rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offs)
+    assert np.array_equal(d_a.download(np.uint8, 2), ra)
+    with pytest.raises(_lib.TrexError):
+        plan.set_hold(2001)

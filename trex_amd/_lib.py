@@ -120,6 +120,7 @@ _SIGNATURES = {
     "trex_dfa_match_plan": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_uint32, c_void_p]),
     "trex_plan_error": (c_int, [c_void_p, POINTER(ctypes.c_uint32), c_void_p]),
     "trex_plan_image_size": (c_int, [c_void_p, i64p]),
+    "trex_plan_set_hold": (c_int, [c_void_p, ctypes.c_uint32]),
     "trex_plan_build_image": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int64]),
     "trex_dfa_match_plan_image": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_uint32,
                                           c_void_p]),

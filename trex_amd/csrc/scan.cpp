@@ -24,6 +24,7 @@ struct trex_plan {
     std::vector<int64_t> h_begin, h_end;   // chunk ranges in processing order (host copy)
     std::vector<int64_t> h_tile_off;       // byte offset of each tile in the image, n_tiles + 1 entries
     DevBuf tile_off;                       // the same offsets on the device
+    uint32_t hold = 0;                     // test hook: delayed publication (trex_plan_set_hold)
     std::mutex mu;                         // one launch at a time per plan
 };
 
@@ -181,8 +182,8 @@ int match_plan(trex_dfa* h, trex_plan* p, const uint8_t* d_bytes, bool image, ui
     uint32_t* error = ticket + 1;
     void* args[] = {&d_bytes, &p->tile_off.ptr, &p->begin.ptr, &p->end.ptr, &p->sid.ptr, &p->pred.ptr,
                     &p->flags.ptr, &n_chunks, &t2, &h->d_t1, &f2, &h->d_acc, &win4, &start, &dead, &n_states,
-                    &p->status.ptr, &p->agg.ptr, &p->jump.ptr, &ticket, &error, &max_spins, &d_final_state,
-                    &d_accept};
+                    &p->status.ptr, &p->agg.ptr, &p->jump.ptr, &ticket, &error, &max_spins, &p->hold,
+                    &d_final_state, &d_accept};
     // persistent grid: enough waves to fill every CU, never more than the work
     const int64_t waves_needed = (n_chunks + 63) / 64;
     const int64_t blocks = std::min<int64_t>((waves_needed + 3) / 4, (int64_t)device_cu_count() * 8);
@@ -198,6 +199,14 @@ extern "C" {
 int trex_dfa_match_plan(trex_dfa* h, trex_plan* p, const uint8_t* d_bytes, uint8_t* d_final_state,
                         uint8_t* d_accept, uint32_t max_spins, void* stream) {
     return match_plan(h, p, d_bytes, false, d_final_state, d_accept, max_spins, stream);
+}
+
+int trex_plan_set_hold(trex_plan* p, uint32_t hold) {
+    if (!p) return fail(TREX_ERR_INVALID, "null plan");
+    if (hold > 2000u) return fail(TREX_ERR_INVALID, "hold %u above 2000", hold);   // <= ~3 ms per tile
+    std::lock_guard<std::mutex> lock(p->mu);
+    p->hold = hold;
+    return TREX_OK;
 }
 
 int trex_plan_image_size(const trex_plan* p, int64_t* bytes) {

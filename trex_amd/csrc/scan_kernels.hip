@@ -141,7 +141,7 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
                                            uint32_t dead, uint32_t n_states, uint32_t* __restrict__ status,
                                            uint64_t* __restrict__ agg, int32_t* __restrict__ jump,
                                            uint32_t* __restrict__ ticket, uint32_t* __restrict__ error,
-                                           uint32_t max_spins, uint8_t* __restrict__ out_state,
+                                           uint32_t max_spins, uint32_t hold, uint8_t* __restrict__ out_state,
                                            uint8_t* __restrict__ out_accept)
 {
     __shared__ __attribute__((aligned(16))) uint8_t t2[WINDOW ? 1024 * 16 : 16];
@@ -266,6 +266,12 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
         // phase 2 (a segment head may look back into its own wave): one store
         // for both kinds, then a convergent wave barrier, keeps the compiler
         // from moving a lane's store behind the look-back loop.
+        // test hook: hold back this tile's publication, tiles later in each
+        // group of 8 less, so successors find their predecessors unpublished
+        if (hold != 0u) {
+            const uint32_t n = hold * (7u - ((base >> 6) & 7u));
+            for (uint32_t k = 0; k < n; ++k) __builtin_amdgcn_s_sleep(8);
+        }
         const bool known = fn_is_const(g);
         const uint32_t kstate = g.d[0] & 0xffu;
         const uint64_t my_fn = known ? 0ull : fn_pack(g);
@@ -334,11 +340,11 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
         const uint8_t *__restrict__ t1g, const uint64_t *__restrict__ f2g, const uint8_t *__restrict__ accg,   \
         uint32_t win4, uint32_t start, uint32_t dead, uint32_t n_states, uint32_t *__restrict__ status,        \
         uint64_t *__restrict__ agg, int32_t *__restrict__ jump, uint32_t *__restrict__ ticket,                 \
-        uint32_t *__restrict__ error, uint32_t max_spins, uint8_t *__restrict__ out_state,                     \
+        uint32_t *__restrict__ error, uint32_t max_spins, uint32_t hold, uint8_t *__restrict__ out_state,      \
         uint8_t *__restrict__ out_accept
 #define TREX_CHUNK_ARGS                                                                                     \
     bytes, tile_off, cbegin, cend, csid, cpred, cflags, n_chunks, t2g, t1g, f2g, accg, win4, start, dead,   \
-        n_states, status, agg, jump, ticket, error, max_spins, out_state, out_accept
+        n_states, status, agg, jump, ticket, error, max_spins, hold, out_state, out_accept
 
 // Chunks of any string, bytes as the plan's ranges index them (tile_off unused).
 extern "C" __global__ void __launch_bounds__(TREX_BLOCK) dfa16_chunks_window(TREX_CHUNK_PARAMS) {

@@ -603,6 +603,11 @@ class ChunkPlan:
     def for_batch(cls, batch: "DeviceBatch", chunk_len: int | None = None) -> "ChunkPlan":
         return cls(batch.begin, batch.end, chunk_len)
 
+    def set_hold(self, hold: int) -> None:
+        """Test hook: later launches hold back each tile's publication,
+        in reverse order within groups of 8 tiles (0 = off)."""
+        _lib.call("trex_plan_set_hold", self.handle, int(hold))
+
     def image_size(self) -> int:
         """Bytes of the plan's chunk-interleaved image."""
         size = ctypes.c_int64()
