@@ -250,9 +250,22 @@ def test_plan_image_size_equal_lengths(gpu_lib):
     """Strings of one length that is a multiple of the chunk length: the image
     is exactly the bytes (no padding)."""
     from trex_amd.engine import ChunkPlan
-    begin = np.arange(4, dtype=np.int64) * (1 << 20)
-    plan = ChunkPlan(begin, begin + (1 << 20), 8192)
+    L, C = 1 << 20, 8192
+    begin = np.arange(4, dtype=np.int64) * L
+    plan = ChunkPlan(begin, begin + L, C)
     assert plan.n_chunks == 512 and plan.image_size() == 4 << 20
+    # the layout itself: chunks of equal length run first chunks first, then
+    # the others in string order; vector i of tile t's k-th chunk sits at
+    # t * C * 64 + (64 i + k) * 16
+    data = np.random.default_rng(0).integers(0, 256, 4 * L, dtype=np.uint8)
+    image = plan.build_image(data)
+    starts = [s * L for s in range(4)] + [s * L + j * C for s in range(4) for j in range(1, L // C)]
+    rows = image.reshape(512 // 64, C // 16, 64, 16).transpose(0, 2, 1, 3).reshape(512, C)
+    want = np.stack([data[a:a + C] for a in starts])
+    assert np.array_equal(rows, want)
+    from trex_amd import _lib
+    with pytest.raises(_lib.TrexError):
+        plan.build_image(data[:-1])          # a chunk would lie outside the source
 
 
 @pytest.mark.gpu
