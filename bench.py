@@ -461,13 +461,22 @@ def scan_leg(args, data, stream) -> dict:
     accept[0] = 1
     perm = DfaByteTable(delta, accept, 0, 15, 16)
     peng = DfaEngine(perm)
-    pms = time_launches(lambda: peng.match_plan_image(plan, d_img.ptr, d_f.ptr, d_a.ptr, stream), stream,
+    from trex_amd.engine import lookback_chunk_len, lookback_target_chunks
+    pplan = ChunkPlan.for_batch(batch, lookback_chunk_len(data.nbytes, lookback_target_chunks(perm)))
+    d_img.free()
+    image = pplan.build_image(data)
+    d_img = DeviceBuffer(image.nbytes)
+    d_img.upload(image, stream=stream)
+    stream.synchronize()
+    del image
+    pms = time_launches(lambda: peng.match_plan_image(pplan, d_img.ptr, d_f.ptr, d_a.ptr, stream), stream,
                         max(3, args.steps))
-    perr = plan.error(stream)
-    log(f"look-back scan over the chunk image, sum of symbols mod 15 (never synchronises): {pms:.4f} ms per launch, "
+    perr = pplan.error(stream)
+    log(f"look-back scan over the chunk image, sum of symbols mod 15 (never synchronises; "
+        f"{pplan.chunk_len >> 10} KiB chunks): {pms:.4f} ms per launch, "
         f"{data.nbytes / (pms * 1e-3) / 1e9:.1f} Gchars/s, error word {perr}")
     out["permutation_automaton"] = {"automaton": "sum of (byte & 15) mod 15, 16 states with the dead state",
-                                    "kernel_avg_ms": round(pms, 4),
+                                    "chunk_len": pplan.chunk_len, "kernel_avg_ms": round(pms, 4),
                                     "value": round(data.nbytes / (pms * 1e-3) / 1e9, 1), "error_word": int(perr),
                                     "verified_bit_exact": None}
     if not args.no_verify:
@@ -481,6 +490,7 @@ def scan_leg(args, data, stream) -> dict:
             log("PARITY FAILURE (look-back scan, mod 15)")
     peng.close()
     d_img.free()
+    pplan.close()
     plan.close()
     batch.free()
     eng.close()

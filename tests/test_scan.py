@@ -330,3 +330,43 @@ def test_plan_wait_expiry_reported(gpu_lib):
     assert np.array_equal(d_a.download(np.uint8, 2), ra)
     with pytest.raises(_lib.TrexError):
         plan.set_hold(2001)
+
+
+def test_synchronizes_and_chunk_target():
+    """Synchronisation over the live bytes: permutation automata (even number
+    of a's, a sum mod 15 over a window) do not synchronise although a byte
+    outside their alphabet kills every state; suffix and counting automata
+    do (a pair with the dead state merges once the other state dies)."""
+    from trex_amd.engine import (LOOKBACK_TARGET_CHUNKS, LOOKBACK_TARGET_CHUNKS_COMPOSING, lookback_chunk_len,
+                                 lookback_target_chunks, synchronizes)
+    from trex_amd.tables import DfaByteTable
+    delta = np.full((16, 256), 15, np.uint8)
+    for b in range(0x40, 0x60):
+        delta[:15, b] = (np.arange(15) + (b & 15)) % 15
+    mod15 = DfaByteTable(delta, np.eye(16, dtype=np.uint8)[0], 0, 15, 16)
+    assert not synchronizes(mod15.delta, 16, 15)
+    want = {EVEN_A: False, "(a|b)*abb": True, "(aa|b)*": True, "(ab)*": True, "(a|b)*a(a|b)(a|b)": True}
+    for rx, sync in want.items():
+        t = dfa_byte_table(compile_regex(rx))
+        assert synchronizes(t.delta, t.n_states, t.dead) == sync, rx
+    assert lookback_target_chunks(mod15) == LOOKBACK_TARGET_CHUNKS_COMPOSING
+    assert lookback_target_chunks(dfa_byte_table(compile_regex("(a|b)*abb"))) == LOOKBACK_TARGET_CHUNKS
+    assert lookback_chunk_len(1 << 30, LOOKBACK_TARGET_CHUNKS_COMPOSING) == 8192
+    # brute force on a small case: a synchronising word over the live bytes
+    # exists exactly when the check says so (the dead state counts only when
+    # a live byte reaches it)
+    import itertools
+    for rx, sync in want.items():
+        t = dfa_byte_table(compile_regex(rx))
+        alive = [s for s in range(t.n_states) if s != t.dead]
+        states = alive + ([t.dead] if any(int(t.delta[s, c]) == t.dead for s in alive for c in b"ab") else [])
+        found = False
+        for L in range(1, 7):
+            for w in itertools.product(b"ab", repeat=L):
+                img = set()
+                for s in states:
+                    for c in w:
+                        s = int(t.delta[s, c])
+                    img.add(s)
+                found |= len(img) == 1
+        assert found == sync, rx

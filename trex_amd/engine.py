@@ -564,6 +564,52 @@ class ProbNfaEngine:
 # per-chunk work (metadata, the composed prefix until the function turns
 # constant, the status word) and more, shorter HBM streams.
 LOOKBACK_TARGET_CHUNKS = 1024 * 64
+# Automata that never synchronise compose every byte pair (about 5x the
+# instructions of a walk), so they take twice the chunks: two waves per SIMD.
+LOOKBACK_TARGET_CHUNKS_COMPOSING = 2 * 1024 * 64
+
+
+def synchronizes(delta: np.ndarray, n_states: int, dead: int = -1) -> bool:
+    """True when some word over the live bytes sends every pair of states
+    to one state (then almost every chunk's function turns constant within a
+    few bytes and the chunk skips the look-back). Live bytes: those that send
+    some non-dead state elsewhere than the dead state; a byte that kills
+    everything would synchronise any automaton but never occurs in matched
+    text. Backward search from the diagonal over pairs of states; the dead
+    state is one of them when a live byte can reach it (a pair with it
+    merges once the other state dies)."""
+    n = int(n_states)
+    d = np.asarray(delta, dtype=np.int64)[:n]
+    alive = [s for s in range(n) if s != dead]
+    live = [b for b in range(d.shape[1]) if any(int(d[s, b]) != dead for s in alive)]
+    if not live:
+        return True
+    reaches_dead = 0 <= dead < n and any(int(d[s, b]) == dead for s in alive for b in live)
+    live_states = alive + ([dead] if reaches_dead else [])
+    if len(live_states) <= 1:
+        return True
+    # pairs (p, q), p < q, that reach the diagonal
+    good = np.zeros((n, n), bool)
+    np.fill_diagonal(good, True)
+    changed = True
+    while changed:
+        changed = False
+        for p in live_states:
+            for q in live_states:
+                if p < q and not good[p, q]:
+                    if any(good[d[p, b], d[q, b]] for b in live):
+                        good[p, q] = good[q, p] = True
+                        changed = True
+    return all(good[p, q] for p in live_states for q in live_states)
+
+
+def lookback_target_chunks(table) -> int:
+    """Chunk count to aim for with this byte table (``DfaByteTable``): one
+    wave per SIMD when the automaton synchronises, two when its chunks keep
+    composing."""
+    if synchronizes(table.delta, table.n_states, table.dead):
+        return LOOKBACK_TARGET_CHUNKS
+    return LOOKBACK_TARGET_CHUNKS_COMPOSING
 
 
 def lookback_chunk_len(total_bytes: int, target_chunks: int = LOOKBACK_TARGET_CHUNKS) -> int:

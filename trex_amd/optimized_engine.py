@@ -246,9 +246,10 @@ class OptimizedEngine:
 
     def _match_long(self, eng, data, offsets):
         from trex_amd.device import DeviceBuffer
-        from trex_amd.engine import ChunkPlan, DeviceBatch
+        from trex_amd.engine import ChunkPlan, DeviceBatch, lookback_chunk_len, lookback_target_chunks
         batch = DeviceBatch(data, offsets, layout="csr")
-        plan = ChunkPlan.for_batch(batch)
+        total = int(offsets[-1] - offsets[0]) if len(offsets) else 0
+        plan = ChunkPlan.for_batch(batch, lookback_chunk_len(total, lookback_target_chunks(self._table)))
         n = batch.n_strings
         d_f, d_a = DeviceBuffer(n), DeviceBuffer(n)
         eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr)
