@@ -389,8 +389,8 @@ def scan_leg(args, data, stream) -> dict:
             d_im.upload(img, stream=stream)
             stream.synchronize()
             del img
-            ti = time_launches(lambda: eng.match_plan_image(pl, d_im.ptr, d_f.ptr, d_a.ptr, stream), stream,
-                               max(3, args.steps))
+            ti = time_launches(lambda: eng.match_plan_image(pl, d_im.ptr, d_im.nbytes, d_f.ptr, d_a.ptr, stream),
+                               stream, max(3, args.steps))
             log(f"look-back scan sweep: {clen:6d} B chunks ({pl.n_chunks} chunks): {t:.4f} ms "
                 f"({data.nbytes / (t * 1e-3) / 1e9:.0f} Gchars/s), chunk image {ti:.4f} ms "
                 f"({data.nbytes / (ti * 1e-3) / 1e9:.0f} Gchars/s), error word {pl.error(stream)}")
@@ -431,8 +431,8 @@ def scan_leg(args, data, stream) -> dict:
     d_img.upload(image, stream=stream)
     stream.synchronize()
     del image
-    ims = time_launches(lambda: eng.match_plan_image(plan, d_img.ptr, d_f.ptr, d_a.ptr, stream), stream,
-                        max(3, args.steps))
+    ims = time_launches(lambda: eng.match_plan_image(plan, d_img.ptr, d_img.nbytes, d_f.ptr, d_a.ptr, stream),
+                        stream, max(3, args.steps))
     ierr = plan.error(stream)
     iach = alg_bytes / (ims * 1e-3) / 1e9
     log(f"look-back scan over the chunk image ({plan.image_size() >> 20} MiB, built on the host in "
@@ -469,8 +469,8 @@ def scan_leg(args, data, stream) -> dict:
     d_img.upload(image, stream=stream)
     stream.synchronize()
     del image
-    pms = time_launches(lambda: peng.match_plan_image(pplan, d_img.ptr, d_f.ptr, d_a.ptr, stream), stream,
-                        max(3, args.steps))
+    pms = time_launches(lambda: peng.match_plan_image(pplan, d_img.ptr, d_img.nbytes, d_f.ptr, d_a.ptr, stream),
+                        stream, max(3, args.steps))
     perr = pplan.error(stream)
     log(f"look-back scan over the chunk image, sum of symbols mod 15 (never synchronises; "
         f"{pplan.chunk_len >> 10} KiB chunks): {pms:.4f} ms per launch, "

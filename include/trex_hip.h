@@ -418,9 +418,10 @@ int trex_plan_image_size(const trex_plan* plan, int64_t* bytes);
 int trex_plan_build_image(const trex_plan* plan, const uint8_t* src, int64_t src_len, uint8_t* dst,
                           int64_t dst_len);
 /* trex_dfa_match_plan over the plan's image in device memory (d_image, built
- * by trex_plan_build_image for this plan and uploaded). */
-int trex_dfa_match_plan_image(trex_dfa* h, trex_plan* plan, const uint8_t* d_image, uint8_t* d_final_state,
-                              uint8_t* d_accept, uint32_t max_spins, void* stream);
+ * by trex_plan_build_image for this plan and uploaded; image_bytes = the
+ * buffer's size, refused when smaller than trex_plan_image_size). */
+int trex_dfa_match_plan_image(trex_dfa* h, trex_plan* plan, const uint8_t* d_image, int64_t image_bytes,
+                              uint8_t* d_final_state, uint8_t* d_accept, uint32_t max_spins, void* stream);
 /* Error word of the plan's last launch (0 = none; bit 0 = a wait expired),
  * read synchronously after the work queued on `stream`. */
 int trex_plan_error(trex_plan* plan, uint32_t* error, void* stream);

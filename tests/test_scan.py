@@ -208,7 +208,10 @@ def _run_plan_image(tab, data, offsets, chunk_len):
     d_img.upload(image)
     n = len(offsets) - 1
     d_f, d_a = DeviceBuffer(max(n, 1)), DeviceBuffer(max(n, 1))
-    eng.match_plan_image(plan, d_img.ptr, d_f.ptr, d_a.ptr)
+    from trex_amd import _lib
+    with pytest.raises(_lib.TrexError, match="image buffer"):   # refused on the host, nothing launched
+        eng.match_plan_image(plan, d_img.ptr, plan.image_size() - 16, d_f.ptr, d_a.ptr)
+    eng.match_plan_image(plan, d_img.ptr, d_img.nbytes, d_f.ptr, d_a.ptr)
     assert plan.error() == 0
     fi, ai = d_f.download(np.uint8, n), d_a.download(np.uint8, n)
     eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr)
@@ -300,7 +303,7 @@ def test_plan_delayed_publication(gpu_lib, pattern):
     image = plan.build_image(data)
     d_img = DeviceBuffer(image.nbytes)
     d_img.upload(image)
-    eng.match_plan_image(plan, d_img.ptr, d_f.ptr, d_a.ptr)
+    eng.match_plan_image(plan, d_img.ptr, d_img.nbytes, d_f.ptr, d_a.ptr)
     assert plan.error() == 0
     assert np.array_equal(d_f.download(np.uint8, n), rf) and np.array_equal(d_a.download(np.uint8, n), ra)
 

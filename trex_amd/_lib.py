@@ -122,8 +122,8 @@ _SIGNATURES = {
     "trex_plan_image_size": (c_int, [c_void_p, i64p]),
     "trex_plan_set_hold": (c_int, [c_void_p, ctypes.c_uint32]),
     "trex_plan_build_image": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int64]),
-    "trex_dfa_match_plan_image": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_uint32,
-                                          c_void_p]),
+    "trex_dfa_match_plan_image": (c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, c_void_p,
+                                          ctypes.c_uint32, c_void_p]),
     "trex_copy": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "trex_read_probe": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p]),
     "trex_dfa_match": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),

@@ -237,8 +237,13 @@ int trex_plan_build_image(const trex_plan* p, const uint8_t* src, int64_t src_le
     return TREX_OK;
 }
 
-int trex_dfa_match_plan_image(trex_dfa* h, trex_plan* p, const uint8_t* d_image, uint8_t* d_final_state,
-                              uint8_t* d_accept, uint32_t max_spins, void* stream) {
+int trex_dfa_match_plan_image(trex_dfa* h, trex_plan* p, const uint8_t* d_image, int64_t image_bytes,
+                              uint8_t* d_final_state, uint8_t* d_accept, uint32_t max_spins, void* stream) {
+    if (!p) return fail(TREX_ERR_INVALID, "null plan");
+    const int64_t need = p->h_tile_off.empty() ? 0 : p->h_tile_off.back();
+    if (image_bytes < need)
+        return fail(TREX_ERR_INVALID, "image buffer holds %lld bytes, the plan's image is %lld",
+                    (long long)image_bytes, (long long)need);
     return match_plan(h, p, d_image, true, d_final_state, d_accept, max_spins, stream);
 }
 

@@ -119,13 +119,14 @@ class DfaEngine:
         _lib.call("trex_dfa_match_plan", self.handle, plan.handle, d_bytes, d_final, d_accept, int(max_spins),
                   stream.handle if stream else None)
 
-    def match_plan_image(self, plan: "ChunkPlan", d_image: int, d_final: int, d_accept: int,
+    def match_plan_image(self, plan: "ChunkPlan", d_image: int, image_bytes: int, d_final: int, d_accept: int,
                          stream: Stream | None = None, max_spins: int = 0) -> None:
         """``match_plan`` over the plan's chunk-interleaved image in device
-        memory (``ChunkPlan.build_image``, uploaded): one wave load reads
-        1 KiB contiguously."""
-        _lib.call("trex_dfa_match_plan_image", self.handle, plan.handle, d_image, d_final, d_accept,
-                  int(max_spins), stream.handle if stream else None)
+        memory (``ChunkPlan.build_image``, uploaded; ``image_bytes`` = the
+        buffer's size, checked against the plan's image size): one wave load
+        reads 1 KiB contiguously."""
+        _lib.call("trex_dfa_match_plan_image", self.handle, plan.handle, d_image, int(image_bytes), d_final,
+                  d_accept, int(max_spins), stream.handle if stream else None)
 
     def match_arrays(self, data: np.ndarray, offsets: np.ndarray):
         """Host CSR batch -> (final_state uint8[n], accept uint8[n], kernel_ms, total_ms)."""
