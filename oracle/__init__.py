@@ -16,7 +16,9 @@ Modules and what pins each of them (the tests named here hold the checks):
   (src/kgram.py:195-232). Pinned by the reference's golden vectors
   (tests/golden/reference_vectors.json), by Python ``re.fullmatch`` and by
   closed forms (parity of a's, value mod 3, suffix tests):
-  tests/test_oracle.py.
+  tests/test_oracle.py; the k-gram walk by the same closed forms for every
+  k in 2..4 and tail length 0..k-1, and a k-gram table composed in the wrong
+  order is shown to fail: tests/test_construction.py.
 * ``oracle.nfa`` -- ``simulate_nfa`` (src/nfa_matrices.py:133-158) and a
   set-of-states walk over the packed successor masks. Pinned by the reference's
   ">= 3 a's" golden (cuda/nfa_tc_evolution.cu:1060-1120), by ``re.fullmatch``

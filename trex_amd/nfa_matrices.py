@@ -71,6 +71,13 @@ class NFAMatrices:
         obj.accept_mask = (np.asarray(accept_mask) != 0).astype(np.int8)
         return obj
 
+    def check_accept(self, state_vec: np.ndarray) -> bool:
+        """Whether any real (unpadded) state set in ``state_vec`` accepts
+        (``src/nfa_matrices.py:117-122``)."""
+        raw = self.n_states_raw
+        vec = np.asarray(state_vec)[:raw].astype(np.int8)
+        return bool(np.any(vec & self.accept_mask[:raw]))
+
     # -- packed forms for the GPU --------------------------------------------
     def successor_masks(self) -> np.ndarray:
         """``succ[c][src]`` = 64-bit mask of destinations (bit d = state d),

@@ -26,8 +26,9 @@ unchanged:
 * ``DFA.dead_state`` names the absorbing non-accepting state whenever the
   minimal DFA has one.
 
-Matching itself (walking the DFA over input bytes) is done on the GPU by
-:mod:`trex_amd.engine`; this module only builds tables.
+Matching batches of input bytes is done on the GPU by :mod:`trex_amd.engine`;
+this module builds tables, and keeps the reference's single-string
+``DFA.simulate`` as the construction type's own membership test.
 """
 
 from __future__ import annotations
@@ -236,6 +237,25 @@ class DFA:
     transitions: dict[int, dict[str, int]]
     alphabet: set[str]
     dead_state: Optional[int] = None
+
+    def simulate(self, input_str: str) -> bool:
+        """Whether the DFA accepts ``input_str``, walked one character at a
+        time on the host (the reference's ``DFA.simulate``,
+        ``src/regex_to_dfa.py:224-233``). A character with no transition moves
+        to ``dead_state``; without a dead state it rejects at once. This is the
+        construction type's own membership test for single strings; batches go
+        to the GPU engines."""
+        trans = self.transitions
+        dead = self.dead_state
+        state = self.start
+        for ch in input_str:
+            nxt = trans.get(state, {}).get(ch)
+            if nxt is None:
+                if dead is None:
+                    return False
+                nxt = dead
+            state = nxt
+        return state in self.accept_states
 
 
 def epsilon_closure(nfa: NFA, states) -> frozenset[int]:

@@ -49,6 +49,28 @@ class DFAMatrices:
     def get_matrix_for_char(self, ch: str):
         return self.matrices.get(ch)
 
+    def get_matrix_sequence(self, input_str: str) -> np.ndarray:
+        """``[L, N, N]`` stack of the string's transition matrices; a character
+        outside the alphabet contributes the identity (``src/simulation.py:80-93``)."""
+        n = self.n_states
+        idx = np.fromiter((self.char_to_idx.get(ch, -1) for ch in input_str), dtype=np.int64,
+                          count=len(input_str))
+        seq = np.zeros((len(input_str), n, n), dtype=np.int8)
+        known = idx >= 0
+        seq[known] = self.matrix_stack[idx[known]]
+        unknown = np.flatnonzero(~known)
+        if unknown.size:
+            diag = np.arange(n)
+            seq[unknown[:, None], diag[None, :], diag[None, :]] = 1
+        return seq
+
+    def check_accept(self, state_vec: np.ndarray) -> bool:
+        """Whether any real (unpadded) state set in ``state_vec`` accepts
+        (``src/simulation.py:95-97``)."""
+        raw = self.n_states_raw
+        vec = np.asarray(state_vec)[:raw].astype(np.int8)
+        return bool(np.any(vec & self.accept_mask[:raw]))
+
     def identity_matrix(self) -> np.ndarray:
         return np.eye(self.n_states, dtype=np.int8)
 
