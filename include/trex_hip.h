@@ -51,6 +51,13 @@ int trex_device_check(void);
 /* Copies the calling thread's last error message into buf[0..n); returns its length. */
 int trex_last_error(char* buf, int n);
 
+/* Time bound, in microseconds of device wall-clock time, of every cross-wave
+ * wait in the launches that follow (look-back plans and mixed queues); 0
+ * restores the default of 100 ms. Process-wide. A wait that expires sets its
+ * launch's error word and the launch still ends. No reference counterpart: the
+ * reference spins without a bound (cuda/monoid_scan.cu:188-192). */
+int trex_set_wait_bound_us(uint32_t us);
+
 /* Architecture name (e.g. gfx950:sramecc+:xnack-), compute units and HBM bytes of the current device. */
 int trex_device_info(char* name, int n, int* cu_count, int64_t* hbm_bytes);
 
@@ -344,8 +351,11 @@ int trex_mix_layout(const trex_mix* m, int64_t* ibeg, int64_t* iend, int32_t* ia
  * kernel for the queue counters, then the persistent kernel). The outputs are
  * only written, so they may be pinned host memory (trex_host_alloc): the
  * results then need no copy back. d_out_mask[i] = DFA final state or NFA final set,
- * d_out_accept[i] = accept bit, for every planned string. max_spins bounds
- * every wait (0: default). */
+ * d_out_accept[i] = accept bit, for every planned string. Every wait of the
+ * launch (a DFA chunk on an earlier chunk of another wave, a fix-up on its
+ * string's speculative chunks) ends after max_spins polls (0: no count limit)
+ * or 100 ms of device wall-clock time, whichever comes first; on expiry the
+ * launch still ends and trex_mix_error reports it. */
 int trex_mix_run(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, uint8_t* d_out_accept, uint32_t max_spins,
                  void* stream);
 /* Persistent workgroups of later runs (0: two per compute unit). Fewer leave
@@ -397,8 +407,10 @@ int trex_plan_destroy(trex_plan* plan);
 int trex_plan_info(const trex_plan* plan, int64_t* n_chunks, int64_t* n_strings);
 /* Final state and accept bit of every string of the plan (DFA of <= 16
  * states). A plan holds the look-back state of one launch: launches that use
- * the same plan must be ordered on one stream. max_spins bounds every wait on
- * a predecessor chunk (0 = default); on expiry the launch still ends and
+ * the same plan must be ordered on one stream. A chunk whose predecessor is
+ * in its own wave takes it by shuffle; any other waits for its predecessor's
+ * publication, at most max_spins polls (0: no count limit) and at most 100 ms
+ * of device wall-clock time; on expiry the launch still ends and
  * trex_plan_error reports it. */
 int trex_dfa_match_plan(trex_dfa* h, trex_plan* plan, const uint8_t* d_bytes, uint8_t* d_final_state,
                         uint8_t* d_accept, uint32_t max_spins, void* stream);

@@ -6,7 +6,7 @@ import pytest
 
 from oracle import cpu as ocpu
 from oracle.dfa import table_walk_one
-from oracle.scan import lookback_final_state
+from oracle.scan import lookback_final_state, plan_chunks, wave_protocol
 from trex_amd.regex_to_dfa import compile_regex
 from trex_amd.tables import dfa_byte_table
 from trex_amd.workloads import ALPHABET_32, random_dfa_pattern, ragged_batch
@@ -65,6 +65,62 @@ def test_lookback_oracle_mutation_caught():
         assert lookback_final_state(tab.delta, tab.start, data, 0, len(data), 4) == want
         bad += wrong(tab.delta, tab.start, data, 4) != want
     assert bad > 0
+
+
+# ---------------------------------------------------------------------------
+# the kernel's wave-level protocol, modelled on the CPU
+# ---------------------------------------------------------------------------
+def _model_case(seed, alpha=b"ab", pattern=EVEN_A):
+    rng = np.random.default_rng(seed)
+    tab = dfa_byte_table(compile_regex(pattern))
+    lengths = rng.integers(0, 700, int(rng.integers(3, 12)))
+    strings = [bytes(rng.choice(np.frombuffer(alpha, np.uint8), int(n))) for n in lengths]
+    want = [table_walk_one(tab.delta, tab.accept, tab.start, s)[0] for s in strings]
+    return rng, tab, strings, want
+
+
+@pytest.mark.parametrize("seed", range(12))
+@pytest.mark.parametrize("late", [False, True], ids=["publish-first", "publish-last"])
+def test_wave_protocol_reads_only_earlier_waves(seed, late):
+    """With in-wave chains resolved in registers, every look-back read is of
+    a chunk in an earlier wave, every wave finishes whatever the interleaving
+    and whichever order a wave gives its stores and its look-back, and every
+    string's final state equals the sequential walk."""
+    rng, tab, strings, want = _model_case(seed)
+    for wave, chunk in ((4, 16), (8, 32), (64, 16)):
+        out, reads, stalled = wave_protocol(tab.delta, tab.start, strings, chunk, rng, True, late, wave)
+        assert not stalled
+        assert out == want
+        assert all(j // wave < c // wave for c, j, _ in reads)
+
+
+def test_previous_protocol_can_wait_on_its_own_wave():
+    """The protocol before in-wave chains linked a lane only to the lane just
+    before it, so a segment head could look back to a chunk of its own wave
+    (a string's last, shorter chunk sorted next to its predecessor). That read
+    depends on the wave's own store coming first: with the store placed after
+    the look-back loop, the wave never finishes. This is the failure class of
+    a look-back that did not end; the kernel no longer has such reads."""
+    own_reads = stalls = 0
+    for seed in range(12):
+        rng, tab, strings, want = _model_case(seed)
+        out, reads, stalled = wave_protocol(tab.delta, tab.start, strings, 16, rng, False, False, 8)
+        assert not stalled and out == want            # publishing first, it completes
+        own_reads += sum(j // 8 == c // 8 for c, j, _ in reads)
+        _, _, stalled = wave_protocol(tab.delta, tab.start, strings, 16, rng, False, True, 8)
+        stalls += stalled
+    assert own_reads > 0 and stalls > 0
+
+
+def test_plan_chunks_order():
+    chunks = plan_chunks([40, 0, 33, 16], 16)
+    # full chunks first (first chunks ahead), then the shorter tails
+    assert [(s, lo, hi) for s, lo, hi, _ in chunks[:5]] == [(0, 0, 16), (2, 0, 16), (3, 0, 16), (0, 16, 32),
+                                                             (2, 16, 32)]
+    for k, (sid, lo, hi, pred) in enumerate(chunks):
+        assert pred < k
+        if lo > 0:
+            assert chunks[pred][0] == sid and chunks[pred][2] == lo
 
 
 # ---------------------------------------------------------------------------
@@ -310,8 +366,9 @@ def test_plan_delayed_publication(gpu_lib, pattern):
 
 @pytest.mark.gpu
 def test_plan_wait_expiry_reported(gpu_lib):
-    """A spin limit far below the hold makes waits expire: the launch still
-    ends, and the error word says so (failure forced from the host)."""
+    """A poll limit, or a wall-clock bound, far below the hold makes waits
+    expire: the launch still ends, and the error word says so (failure forced
+    from the host)."""
     from trex_amd import _lib
     from trex_amd.device import DeviceBuffer
     from trex_amd.engine import ChunkPlan, DeviceBatch, DfaEngine
@@ -326,6 +383,17 @@ def test_plan_wait_expiry_reported(gpu_lib):
     d_f, d_a = DeviceBuffer(2), DeviceBuffer(2)
     eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr, max_spins=4)
     assert plan.error() & 1
+    # the wall-clock bound alone (no poll limit): 2 us against a hold of
+    # hundreds of microseconds per tile
+    _lib.call("trex_set_wait_bound_us", 2)
+    try:
+        eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr, max_spins=0)
+        assert plan.error() & 1
+    finally:
+        _lib.call("trex_set_wait_bound_us", 0)
+    # the default bound (100 ms) lets the same held launch finish cleanly
+    eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr)
+    assert plan.error() == 0
     plan.set_hold(0)
     eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr)
     assert plan.error() == 0

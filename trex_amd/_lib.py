@@ -8,7 +8,7 @@ fallback behind any matching entry point.
 from __future__ import annotations
 
 import ctypes
-from ctypes import POINTER, c_char_p, c_float, c_int, c_int64, c_size_t, c_uint8, c_uint64, c_void_p, c_double
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int64, c_size_t, c_uint8, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -32,6 +32,7 @@ vpp = POINTER(c_void_p)
 _SIGNATURES = {
     "trex_device_check": (c_int, []),
     "trex_last_error": (c_int, [c_char_p, c_int]),
+    "trex_set_wait_bound_us": (c_int, [c_uint32]),
     "trex_device_info": (c_int, [c_char_p, c_int, intp, i64p]),
     "trex_malloc": (c_int, [vpp, c_size_t]),
     "trex_free": (c_int, [c_void_p]),

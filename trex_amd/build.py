@@ -38,16 +38,24 @@ def _run(cmd: list[str]) -> None:
     subprocess.run([str(c) for c in cmd], check=True)
 
 
-def build_kernels() -> list[Path]:
+def build_kernels(jobs: int | None = None) -> list[Path]:
+    """Compiles every kernel file, up to ``jobs`` hipcc processes at once
+    (default: the CPU count, at most 8)."""
+    from concurrent.futures import ThreadPoolExecutor
     KERNEL_DIR.mkdir(exist_ok=True)
-    built = []
-    for src in sorted(CSRC.glob("*.hip")):
+    sources = sorted(CSRC.glob("*.hip"))
+
+    def one(src: Path) -> Path:
         out = KERNEL_DIR / (src.stem + ".hsaco")
         tmp = out.with_suffix(".hsaco.tmp")
         _run([ROCM / "bin" / "hipcc", "--genco", "--no-gpu-bundle-output", *HIP_DEVICE_FLAGS,
               "-I", CSRC, "-o", tmp, src])
         os.replace(tmp, out)
-        built.append(out)
+        return out
+
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        built = list(pool.map(one, sources))
     for stale in KERNEL_DIR.glob("*.hsaco"):
         if stale not in built:
             stale.unlink()
@@ -64,8 +72,11 @@ def build_runtime() -> Path:
 
 
 def build_all() -> None:
-    build_kernels()
-    build_runtime()
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=2) as pool:
+        lib = pool.submit(build_runtime)
+        build_kernels()
+        lib.result()
 
 
 if __name__ == "__main__":

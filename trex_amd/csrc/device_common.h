@@ -202,6 +202,27 @@ __device__ __forceinline__ void store_release(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A wait bounded twice: by a count of unsuccessful polls (max_spins, 0 = no
+// count limit) and by elapsed time on the device's constant-rate wall clock
+// (max_ticks ticks of wall_clock64, whose rate the host reads as
+// hipDeviceAttributeWallClockRate). The clock starts at the first
+// unsuccessful poll, so a wait that never has to poll costs no clock read.
+struct BoundedWait {
+    uint32_t spins;
+    uint32_t max_spins;
+    uint64_t t0;
+    uint64_t max_ticks;
+    __device__ __forceinline__ BoundedWait(uint32_t ms, uint64_t mt) : spins(0), max_spins(ms), t0(0), max_ticks(mt) {}
+    // Call after each unsuccessful poll: true once either bound is passed.
+    __device__ __forceinline__ bool expired() {
+        const uint64_t now = (uint64_t)wall_clock64();
+        if (spins == 0u) t0 = now;
+        ++spins;
+        if (max_spins != 0u && spins > max_spins) return true;
+        return now - t0 > max_ticks;
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Boolean NFA-64 step over nibble-sliced successor tables
 // ---------------------------------------------------------------------------

@@ -76,6 +76,8 @@ struct MixArgs {
     uint32_t* error;
     uint32_t* converged;
     uint32_t max_spins;
+    uint32_t reserved;
+    uint64_t max_ticks;
     uint64_t* out_mask;
     uint8_t* out_accept;
 };
@@ -504,7 +506,11 @@ int trex_mix_run(trex_mix* m, const uint8_t* d_bytes, uint64_t* d_out_mask, uint
     a.agg = m->d_agg.as<uint64_t>();
     a.ckpt = m->d_ckpt.as<uint64_t>();
     a.spec_exit = m->d_exit.as<uint64_t>();
-    a.max_spins = max_spins ? max_spins : (1u << 24);
+    // waits end after max_spins polls (0: no count limit) or after
+    // the wall-clock bound (trex_set_wait_bound_us, 100 ms by default), whichever comes first
+    a.max_spins = max_spins;
+    a.reserved = 0;
+    a.max_ticks = device_wait_ticks();
     a.out_mask = d_out_mask;
     a.out_accept = d_out_accept;
     // persistent grid: by default two 512-thread workgroups per CU (70 KB of LDS each), or one

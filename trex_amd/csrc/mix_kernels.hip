@@ -27,8 +27,10 @@
 // Forward progress: the queue puts every batch that is waited on before the
 // batches that wait on it (DFA chunks after their predecessors, fix-ups after
 // all NFA_SPEC batches), so every waited-on item has been claimed by a running
-// workgroup, and no waited-on work waits itself. Spins are bounded by
-// max_spins; on expiry the error word is set and the wave moves on.
+// workgroup, and no waited-on work waits itself. A DFA chunk never waits on
+// a chunk of its own wave (those come by shuffle). Waits are bounded by a
+// poll count (max_spins) and by wall-clock ticks (max_ticks); on expiry the
+// error word is set and the wave moves on.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -88,7 +90,9 @@ struct MixArgs {
     uint64_t* spec_exit;      // [n_spec]
     uint32_t* error;
     uint32_t* converged;      // fix-up chunks that met a checkpoint
-    uint32_t max_spins;
+    uint32_t max_spins;       // wait bound in polls (0: none)
+    uint32_t reserved;
+    uint64_t max_ticks;       // wait bound in wall-clock ticks
     uint64_t* out_mask;       // per string: DFA final state / NFA final set
     uint8_t* out_accept;      // per string
 };
@@ -176,7 +180,10 @@ struct ChunkScanner {
     }
 };
 
-__device__ void dfa_chunk(const MixArgs& A, const MixPattern& P, const DfaLds& L, int64_t item, bool active) {
+// One chunk per lane; every lane of the workgroup calls it (inactive lanes
+// take part in the wave's shuffles). Always inlined into the persistent loop.
+__device__ __forceinline__ void dfa_chunk(const MixArgs& A, const MixPattern& P, const DfaLds& L, int64_t item,
+                                          bool active) {
     ChunkScanner sc;
     sc.L = &L;
     sc.w.init();
@@ -200,29 +207,56 @@ __device__ void dfa_chunk(const MixArgs& A, const MixPattern& P, const DfaLds& L
     sc.settle();
     if (sc.w.outside(P.win4)) { sc.walking = true; sc.s = P.dead; }
 
-    // A chunk with a constant function knows its final state without looking
-    // back and publishes PREFIX at once. Every lane's status is stored before
-    // any lane of the wave enters the look-back (a lane may look back to a
-    // neighbour in its own wave): one store for both kinds, then a convergent
-    // wave barrier, keeps the compiler from moving the publish behind the loop.
-    const bool known = pred < 0 || sc.walking;
-    const uint64_t my_fn = known ? 0ull : fn_pack(sc.f);
-    if (active) {
-        if (!known) A.agg[item] = my_fn;
-        store_release(&A.status[item], known ? (kStatusPrefix | (sc.s << 8)) : kStatusAggregate);
+    // In-wave chains in registers (as in scan_kernels.hip): a lane whose
+    // predecessor item lies in its own wave takes that lane's function by
+    // shuffle; pointer jumping leaves in g the composition from the chain's
+    // root lane up to this one. Only roots look back through memory, and a
+    // root's predecessor belongs to another wave (of this workgroup or of an
+    // earlier batch), which publishes without waiting on this one.
+    const uint32_t lane = threadIdx.x & 63u;
+    const int64_t wave_first = item - (int64_t)lane;
+    const bool own_known = pred < 0 || sc.walking;
+    const Fn16 own = own_known ? fn_const(sc.s) : sc.f;
+    Fn16 g = own;
+    const bool inwave = active && (int64_t)pred >= wave_first;
+    uint32_t root = inwave ? (uint32_t)((int64_t)pred - wave_first) : lane;
+    bool closed = !inwave;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+        Fn16 up;
+        up.d[0] = __shfl(g.d[0], (int)root, 64);
+        up.d[1] = __shfl(g.d[1], (int)root, 64);
+        up.d[2] = __shfl(g.d[2], (int)root, 64);
+        up.d[3] = __shfl(g.d[3], (int)root, 64);
+        const bool up_closed = __shfl((int)closed, (int)root, 64) != 0;
+        const uint32_t up_root = (uint32_t)__shfl((int)root, (int)root, 64);
+        if (!closed) {
+            g = fn_then(up, g);   // the earlier lanes' chunks apply first
+            closed = up_closed;
+            root = up_root;
+        }
     }
-    __builtin_amdgcn_wave_barrier();
-    uint32_t state = sc.s;
-    if (active && !known) {
+
+    // A chunk whose chain function is constant knows its final state and
+    // publishes PREFIX at once; any other publishes its own chunk's function
+    // (a look-back from another wave steps chunk by chunk over A.ia).
+    const bool known = fn_is_const(g);
+    const uint32_t kstate = g.d[0] & 0xffu;
+    if (active) {
+        if (!known) A.agg[item] = fn_pack(own);
+        store_release(&A.status[item], known ? (kStatusPrefix | (kstate << 8)) : kStatusAggregate);
+    }
+    uint32_t incoming = 0;
+    if (active && !known && root == lane) {
         uint64_t acc_fn = 0xfedcba9876543210ull;   // identity
         int32_t j = pred;
-        uint32_t spins = 0;
+        BoundedWait wait(A.max_spins, A.max_ticks);
         while (true) {
             const uint32_t st = load_relaxed(&A.status[j]);
             if (st == 0u) {
-                if (++spins > A.max_spins) {
+                if (wait.expired()) {
                     atomicOr(A.error, 1u);
-                    state = P.dead;
+                    incoming = P.dead;
                     break;
                 }
                 __builtin_amdgcn_s_sleep(2);
@@ -230,19 +264,21 @@ __device__ void dfa_chunk(const MixArgs& A, const MixPattern& P, const DfaLds& L
             }
             fence_acquire();   // agg[j] was written before status[j] was released
             if ((st & 0xffu) == kStatusPrefix) {
-                state = nib(my_fn, nib(acc_fn, (st >> 8) & 15u));
+                incoming = nib(acc_fn, (st >> 8) & 15u);
                 break;
             }
             acc_fn = nib_then(A.agg[j], acc_fn);   // the earlier chunk applies first
             j = (A.iflags[j] & 1u) ? -1 : A.ia[j];
             if (j < 0) {   // cannot happen: a string's first chunk publishes a prefix
                 atomicOr(A.error, 2u);
-                state = P.dead;
+                incoming = P.dead;
                 break;
             }
         }
-        store_release(&A.status[item], kStatusPrefix | (state << 8));
     }
+    incoming = __shfl(incoming, (int)root, 64);
+    const uint32_t state = known ? kstate : fn_apply(g, incoming);
+    if (active && !known) store_release(&A.status[item], kStatusPrefix | (state << 8));
     if (active && A.ib[item] >= 0) {
         A.out_mask[A.ib[item]] = state;
         A.out_accept[A.ib[item]] = L.acc[state];
@@ -435,10 +471,10 @@ __device__ __forceinline__ void mix_loop(const MixArgs& A)
         } else if (role == kRoleNfaFixup) {
             const int64_t c0 = A.ibeg[item], c1 = A.iend[item];
             const uint32_t need = (uint32_t)(c1 - c0);
-            uint32_t spins = 0;
+            BoundedWait wait(A.max_spins, A.max_ticks);
             bool ok = true;
             while (load_relaxed(&A.done[A.ia[item]]) < need) {
-                if (++spins > A.max_spins) {
+                if (wait.expired()) {
                     atomicOr(A.error, 4u);
                     ok = false;
                     break;

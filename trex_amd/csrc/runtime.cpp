@@ -18,6 +18,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -159,6 +160,21 @@ int device_cu_count() {
     return cached;
 }
 
+static std::atomic<uint32_t> g_wait_bound_us{kWaitBoundUs};
+
+uint64_t device_wait_ticks() {
+    static std::atomic<int> cached_khz{0};
+    int khz = cached_khz.load();
+    if (khz <= 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+            khz = 100000;   // the 100 MHz counter of gfx9 parts
+        cached_khz.store(khz);
+    }
+    return (uint64_t)khz * g_wait_bound_us.load() / 1000u;
+}
+
 int ensure_host_dispatch(hipStream_t& stream, hipEvent_t (&ev)[4]) {
     if (stream) return TREX_OK;
     hipStream_t s = nullptr;
@@ -202,6 +218,11 @@ int trex_device_check(void) {
     if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return fail(TREX_ERR_NO_DEVICE, "hipGetDeviceProperties failed");
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(TREX_ERR_ARCH, "device is %s; the kernels are built for gfx950", prop.gcnArchName);
+    return TREX_OK;
+}
+
+int trex_set_wait_bound_us(uint32_t us) {
+    g_wait_bound_us.store(us ? us : kWaitBoundUs);
     return TREX_OK;
 }
 

@@ -45,6 +45,15 @@ constexpr unsigned kBlock = 256;   // threads per workgroup of the string kernel
 // Number of compute units of the current device (cached).
 int device_cu_count();
 
+// Time bound of the kernels' cross-wave waits in ticks of the device counter
+// wall_clock64 reads, on the current device: trex_set_wait_bound_us, by
+// default kWaitBoundUs.
+uint64_t device_wait_ticks();
+// Default time bound of a cross-wave wait: far above any legitimate wait (a
+// predecessor's phase 1 over one chunk takes microseconds), far below a test's
+// time limit.
+constexpr uint32_t kWaitBoundUs = 100000;
+
 }  // namespace trex
 
 // A resident DFA byte table (the handle behind trex_dfa_*).

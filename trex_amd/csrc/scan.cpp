@@ -154,7 +154,9 @@ int match_plan(trex_dfa* h, trex_plan* p, const uint8_t* d_bytes, bool image, ui
     if (h->n_states > 16) return fail(TREX_ERR_INVALID, "chunked matching needs a table of at most 16 states");
     if (p->n_chunks == 0) return TREX_OK;
     if (!d_bytes || !d_final_state || !d_accept) return fail(TREX_ERR_INVALID, "null buffer");
-    if (max_spins == 0) max_spins = 1u << 26;
+    // waits end after max_spins polls (0: no count limit) or after
+    // the wall-clock bound (trex_set_wait_bound_us, 100 ms by default), whichever comes first
+    uint64_t max_ticks = device_wait_ticks();
     hipStream_t s = (hipStream_t)stream;
     std::lock_guard<std::mutex> lock(p->mu);
 
@@ -182,7 +184,7 @@ int match_plan(trex_dfa* h, trex_plan* p, const uint8_t* d_bytes, bool image, ui
     uint32_t* error = ticket + 1;
     void* args[] = {&d_bytes, &p->tile_off.ptr, &p->begin.ptr, &p->end.ptr, &p->sid.ptr, &p->pred.ptr,
                     &p->flags.ptr, &n_chunks, &t2, &h->d_t1, &f2, &h->d_acc, &win4, &start, &dead, &n_states,
-                    &p->status.ptr, &p->agg.ptr, &p->jump.ptr, &ticket, &error, &max_spins, &p->hold,
+                    &p->status.ptr, &p->agg.ptr, &p->jump.ptr, &ticket, &error, &max_spins, &max_ticks, &p->hold,
                     &d_final_state, &d_accept};
     // persistent grid: enough waves to fill every CU, never more than the work
     const int64_t waves_needed = (n_chunks + 63) / 64;

@@ -23,11 +23,14 @@
 //    PREFIX is found; the chunk's prefix state is F(acc(p)). Waiting is only
 //    for predecessors' phase-1 results, which never wait on anything.
 //
-// Forward progress: chunks are claimed from an atomic ticket in index order,
-// and a chunk's predecessor always has a smaller index, so every chunk waited
-// on has been claimed by a running wave (the reference's blockIdx.x order,
-// cuda/monoid_scan.cu:141, gives no such guarantee). Spins are bounded by
-// `max_spins`; on expiry the kernel sets `error` and moves on, so every wave
+// Forward progress: chunks are claimed 64 at a time (one wave) from an atomic
+// ticket in index order, and a chunk's predecessor always has a smaller
+// index. A predecessor in the same wave is taken from registers (shuffles), so
+// every chunk waited on belongs to an earlier ticket, claimed by a running
+// wave that publishes without waiting on later tickets (the reference's
+// blockIdx.x order, cuda/monoid_scan.cu:141, gives no such guarantee). Waits
+// are bounded by a poll count (`max_spins`, 0 = none) and by wall-clock ticks
+// (`max_ticks`); on expiry the kernel sets `error` and moves on, so every wave
 // always exits.
 //
 // status[c]: 0 = not ready, 1 = aggregate in agg[c], 2 | state << 8 = prefix.
@@ -121,11 +124,12 @@ struct Scanner {
     }
 };
 
-// Always inlined into its two kernels. With 16 vectors in flight the byte
-// kernel's copy was compiled as a called function (280 VGPRs, 112-byte stack
-// for its arguments) and its first run did not finish; the cause was not
-// found, so the depth stays at 8, where both kernels inline (<= 106 VGPRs, no
-// stack) and pass every test.
+// Always inlined into its kernels, at 8 vectors in flight (<= 106 VGPRs, no
+// stack). A build with 16 in flight compiled the byte kernels' copy as a
+// called function (256 VGPRs, generic pointers, a private segment) and its
+// first GPU run did not finish; that build also let a lane wait on a chunk of
+// its own wave through memory, which the in-wave chains below no longer do.
+// The deeper prefetch measured no faster, so it stays off.
 // IMAGE: `bytes` is the chunk-interleaved image of the plan (tile t = the
 // ticket's 64 chunks at byte tile_off[t], vector i of lane k's chunk at
 // tile_off[t] + (64 i + k) * 16, so one wave load reads 1 KiB contiguously);
@@ -141,7 +145,8 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
                                            uint32_t dead, uint32_t n_states, uint32_t* __restrict__ status,
                                            uint64_t* __restrict__ agg, int32_t* __restrict__ jump,
                                            uint32_t* __restrict__ ticket, uint32_t* __restrict__ error,
-                                           uint32_t max_spins, uint32_t hold, uint8_t* __restrict__ out_state,
+                                           uint32_t max_spins, uint64_t max_ticks, uint32_t hold,
+                                           uint8_t* __restrict__ out_state,
                                            uint8_t* __restrict__ out_accept)
 {
     __shared__ __attribute__((aligned(16))) uint8_t t2[WINDOW ? 1024 * 16 : 16];
@@ -229,43 +234,45 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
             if (sc.w.outside(win4)) { sc.walking = true; sc.s = dead; }
         }
 
-        // ---- segmented scan of the wave's 64 chunk functions ----
-        // Lane k continues lane k-1's segment when its chunk directly follows
-        // lane k-1's chunk in the same string (pred == c - 1); every other
-        // lane starts a segment. Six shuffle steps (Hillis-Steele) leave in g
-        // the composition of the chunks from the segment's first lane up to
-        // this one. A first chunk, or any chunk whose function turned
-        // constant, enters as a constant function, so everything after it in
-        // the segment comes out constant: its final state is known here.
+        // ---- in-wave chains, resolved in registers ----
+        // A lane whose predecessor chunk lies in the same wave (pred >= base)
+        // takes that lane's function by shuffle instead of waiting for its
+        // publication: pointer jumping over the in-wave predecessor links
+        // (six rounds cover a chain of 64 lanes) leaves in g the composition
+        // of the chunks from the chain's root lane up to this one, and in
+        // `root` the root lane. A root's predecessor is a chunk of an earlier
+        // ticket (or none), so the look-back below only ever waits on other
+        // waves' chunks: no lane waits on a store of its own wave, whatever
+        // order the compiler gives the wave's publish stores and its loop. A
+        // first chunk, or any chunk whose function turned constant, enters as
+        // a constant function, so everything after it on the chain comes out
+        // constant: its final state is known here.
         Fn16 g = (pred < 0 || sc.walking) ? fn_const(sc.s) : sc.f;
-        const bool link = active && lane > 0u && pred == (int32_t)(c - 1);
-        bool closed = !link;   // the lanes folded into g reach back to a segment head
+        const bool inwave = active && pred >= (int32_t)base;
+        uint32_t root = inwave ? (uint32_t)(pred - (int32_t)base) : lane;
+        bool closed = !inwave;   // g reaches back to the chain's root
 #pragma unroll
-        for (uint32_t d = 1; d < 64; d <<= 1) {
+        for (int r = 0; r < 6; ++r) {
             Fn16 up;
-            up.d[0] = __shfl_up(g.d[0], d, 64);
-            up.d[1] = __shfl_up(g.d[1], d, 64);
-            up.d[2] = __shfl_up(g.d[2], d, 64);
-            up.d[3] = __shfl_up(g.d[3], d, 64);
-            const bool up_closed = __shfl_up((int)closed, d, 64) != 0;
-            if (lane >= d && !closed) {
+            up.d[0] = __shfl(g.d[0], (int)root, 64);
+            up.d[1] = __shfl(g.d[1], (int)root, 64);
+            up.d[2] = __shfl(g.d[2], (int)root, 64);
+            up.d[3] = __shfl(g.d[3], (int)root, 64);
+            const bool up_closed = __shfl((int)closed, (int)root, 64) != 0;
+            const uint32_t up_root = (uint32_t)__shfl((int)root, (int)root, 64);
+            if (!closed) {
                 g = fn_then(up, g);   // the earlier lanes' chunks apply first
                 closed = up_closed;
+                root = up_root;
             }
         }
-        const uint64_t heads = __ballot(!link);
-        const uint32_t head_lane = 63u - (uint32_t)__clzll(heads & ((2ull << lane) - 1ull));
-        const int32_t head_pred = __shfl(pred, (int)head_lane, 64);
+        const int32_t root_pred = __shfl(pred, (int)root, 64);
 
         // ---- publish ----
         // A constant g publishes PREFIX (its final state). Otherwise agg[c] =
-        // g, the function of the segment's chunks up to c, and jump[c] = the
-        // chunk before the segment: a look-back that meets c composes g and
-        // continues at jump[c], one step per segment instead of per chunk.
-        // Every lane's status is stored before any lane of the wave enters
-        // phase 2 (a segment head may look back into its own wave): one store
-        // for both kinds, then a convergent wave barrier, keeps the compiler
-        // from moving a lane's store behind the look-back loop.
+        // g, the function of the chain's chunks from its root up to c, and
+        // jump[c] = the root's predecessor: a look-back that meets c composes
+        // g and continues at jump[c], one step per chain instead of per chunk.
         // test hook: hold back this tile's publication, tiles later in each
         // group of 8 less, so successors find their predecessors unpublished
         if (hold != 0u) {
@@ -278,25 +285,26 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
         if (active) {
             if (!known) {
                 agg[c] = my_fn;
-                jump[c] = head_pred;
+                jump[c] = root_pred;
             }
             store_release(&status[c], known ? (kStatusPrefix | (kstate << 8)) : kStatusAggregate);
         }
-        __builtin_amdgcn_wave_barrier();
 
-        // ---- phase 2: the head of each unresolved segment looks back ----
-        // (a segment is unresolved only if its head is: a constant head makes
-        // every later lane of the segment constant)
+        // ---- phase 2: the root of each unresolved chain looks back ----
+        // (a chain is unresolved only if its root is: a constant root makes
+        // every later lane of the chain constant). Every chunk waited on
+        // belongs to an earlier ticket, whose wave publishes without waiting
+        // on this one. Waits are bounded by the spin count and by wall-clock
+        // ticks; on expiry the error word is set and the wave moves on.
         uint32_t incoming = 0;
-        if (active && !known && head_lane == lane) {
+        if (active && !known && root == lane) {
             uint64_t acc_fn = 0xfedcba9876543210ull;   // identity
             int32_t j = pred;
-            uint32_t spins = 0;
-            bool done = false;
-            while (!done) {
+            BoundedWait wait(max_spins, max_ticks);
+            while (true) {
                 const uint32_t st = load_relaxed(&status[j]);
                 if (st == 0u) {
-                    if (++spins > max_spins) {
+                    if (wait.expired()) {
                         atomicOr(error, 1u);
                         incoming = dead;
                         break;
@@ -307,20 +315,19 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
                 fence_acquire();   // agg[j] and jump[j] were written before status[j] was released
                 if ((st & 0xffu) == kStatusPrefix) {
                     incoming = nib(acc_fn, (st >> 8) & 15u);
-                    done = true;
-                } else {
-                    acc_fn = nib_then(agg[j], acc_fn);   // earlier chunks apply first
-                    j = jump[j];
-                    if (j < 0) {   // cannot happen: a string's first chunk publishes a prefix
-                        atomicOr(error, 2u);
-                        incoming = dead;
-                        break;
-                    }
+                    break;
+                }
+                acc_fn = nib_then(agg[j], acc_fn);   // earlier chunks apply first
+                j = jump[j];
+                if (j < 0) {   // cannot happen: a string's first chunk publishes a prefix
+                    atomicOr(error, 2u);
+                    incoming = dead;
+                    break;
                 }
             }
         }
-        // the state before each segment, from its head, then each lane's own
-        incoming = __shfl(incoming, (int)head_lane, 64);
+        // the state before each chain, from its root, then each lane's own
+        incoming = __shfl(incoming, (int)root, 64);
         const uint32_t state = known ? kstate : fn_apply(g, incoming);
         if (active && !known) store_release(&status[c], kStatusPrefix | (state << 8));
         if (active && (flags & 2u)) {
@@ -340,11 +347,12 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
         const uint8_t *__restrict__ t1g, const uint64_t *__restrict__ f2g, const uint8_t *__restrict__ accg,   \
         uint32_t win4, uint32_t start, uint32_t dead, uint32_t n_states, uint32_t *__restrict__ status,        \
         uint64_t *__restrict__ agg, int32_t *__restrict__ jump, uint32_t *__restrict__ ticket,                 \
-        uint32_t *__restrict__ error, uint32_t max_spins, uint32_t hold, uint8_t *__restrict__ out_state,      \
+        uint32_t *__restrict__ error, uint32_t max_spins, uint64_t max_ticks, uint32_t hold,                 \
+        uint8_t *__restrict__ out_state,                                                                      \
         uint8_t *__restrict__ out_accept
 #define TREX_CHUNK_ARGS                                                                                     \
     bytes, tile_off, cbegin, cend, csid, cpred, cflags, n_chunks, t2g, t1g, f2g, accg, win4, start, dead,   \
-        n_states, status, agg, jump, ticket, error, max_spins, hold, out_state, out_accept
+        n_states, status, agg, jump, ticket, error, max_spins, max_ticks, hold, out_state, out_accept
 
 // Chunks of any string, bytes as the plan's ranges index them (tile_off unused).
 extern "C" __global__ void __launch_bounds__(TREX_BLOCK) dfa16_chunks_window(TREX_CHUNK_PARAMS) {

@@ -344,6 +344,48 @@ NFA_VARIANTS = {"bytes": 1, "window": 2, "window5": 3, "window6": 4}
 NFA_VARIANT_NAMES = {v: k for k, v in NFA_VARIANTS.items()}
 
 
+# Strings longer than this are cut into chunks matched in parallel (the
+# look-back scan for DFA-16 tables, speculative chunks for NFA-64).
+LONG_STRING = 1 << 20
+
+
+def dfa_as_nfa(table: DfaByteTable):
+    """The NFA-64 form of a DFA byte table of at most 64 states, for the
+    chunked NFA path: one symbol per live byte (a byte that sends some
+    non-dead state elsewhere than the dead state), ``succ[c][s] = 1 <<
+    delta[s][b_c]``, every other byte rejecting (the empty set stands for
+    the dead state). Returns (succ, byte_to_sym, start mask, accept mask), or
+    None when the table has more than 64 states, no dead state, or more
+    live bytes than the NFA kernels hold."""
+    n, dead = int(table.n_states), int(table.dead)
+    if n > 64 or dead < 0:
+        return None
+    delta = np.asarray(table.delta, dtype=np.int64)[:n]
+    alive = np.array([s for s in range(n) if s != dead], dtype=np.int64)
+    live = np.flatnonzero((delta[alive] != dead).any(axis=0)) if alive.size else np.zeros(0, np.int64)
+    if live.size > 62:
+        return None
+    succ = np.zeros((max(live.size, 1), 64), np.uint64)
+    for c, b in enumerate(live):
+        succ[c, :n] = np.uint64(1) << delta[:, b].astype(np.uint64)
+    b2s = np.full(256, -1, np.int16)
+    b2s[live] = np.arange(live.size, dtype=np.int16)
+    accept = 0
+    for s in np.flatnonzero(np.asarray(table.accept)[:n]):
+        accept |= 1 << int(s)
+    return succ, b2s, 1 << int(table.start), accept
+
+
+def nfa_mask_to_state(mask: np.ndarray, dead: int) -> np.ndarray:
+    """Final DFA states from the final sets of :func:`dfa_as_nfa`'s automaton
+    (a set is one state, or empty for the dead state)."""
+    mask = np.asarray(mask, dtype=np.uint64)
+    out = np.full(mask.shape, dead, dtype=np.uint8)
+    nz = mask != 0
+    out[nz] = np.log2(mask[nz].astype(np.float64)).astype(np.uint8)
+    return out
+
+
 def nfa_byte_map(nm, unknown: str = "reject") -> np.ndarray:
     """byte -> symbol index of an :class:`NFAMatrices`; bytes outside the
     alphabet reject (``simulate_nfa``, src/nfa_matrices.py:149-152) or, with
@@ -368,6 +410,7 @@ class NfaEngine:
             raise ValueError("byte_to_sym must have 256 entries")
         self.start = int(start)
         self.accept = int(accept)
+        self._mix = None
         h = ctypes.c_void_p()
         _lib.call("trex_nfa_create", self.succ.ctypes.data if self.succ.size else None, self.succ.shape[0],
                   self.byte_to_sym.ctypes.data, self.start, self.accept, ctypes.byref(h))
@@ -392,10 +435,20 @@ class NfaEngine:
         _lib.call("trex_nfa_match_ranges", self.handle, batch.d_bytes.ptr, batch.d_begin.ptr, batch.d_end.ptr,
                   batch.n_strings, d_mask, d_accept, stream.handle if stream else None)
 
+    def can_split(self) -> bool:
+        """Whether long strings can be cut into speculative chunks (the mixed
+        queue's NFA roles, which take automata whose live bytes lie in one
+        aligned 32-byte window)."""
+        return self.info()["window_base"] >= 0
+
     def match_arrays(self, data: np.ndarray, offsets: np.ndarray):
-        """Host CSR batch -> (final mask uint64[n], accept uint8[n], kernel_ms, total_ms)."""
+        """Host CSR batch -> (final mask uint64[n], accept uint8[n], kernel_ms, total_ms).
+        A batch holding a string over ``LONG_STRING`` bytes goes through the
+        split path when the automaton allows it (:meth:`match_split`)."""
         data, offsets = _as_csr(data=data, offsets=offsets)
         n = len(offsets) - 1
+        if n > 0 and int(np.diff(offsets).max()) > LONG_STRING and self.can_split():
+            return self.match_split(data, offsets)
         mask = np.empty(n, dtype=np.uint64)
         acc = np.empty(n, dtype=np.uint8)
         kms, tms = ctypes.c_float(), ctypes.c_float()
@@ -404,10 +457,34 @@ class NfaEngine:
                       offsets.ctypes.data, n, mask.ctypes.data, acc.ctypes.data, ctypes.byref(kms), ctypes.byref(tms))
         return mask, acc, kms.value, tms.value
 
+    def match_split(self, data: np.ndarray, offsets: np.ndarray, params=None):
+        """Host CSR batch through the mixed queue with this one pattern: strings
+        over the split length are cut into chunks walked in parallel from the
+        set of all states (the first from the start set), each with
+        checkpoints, then one fix-up lane per string walks the chunks again
+        from the true entry set until it meets a checkpoint; shorter strings
+        are walked whole. The reference walks every string on one thread
+        (cuda/nfa_tc_evolution.cu:683-745). Returns (final mask, accept,
+        kernel_ms, total_ms)."""
+        import time
+        from trex_amd.mix import MixEngine, MixParams
+        if not self.can_split():
+            raise ValueError("the split path needs an automaton whose live bytes lie in one 32-byte window")
+        if self._mix is None:
+            self._mix = MixEngine([self])
+        t0 = time.perf_counter()
+        n = len(offsets) - 1
+        mask, acc, kms = self._mix.match_arrays(data, offsets, np.zeros(n, np.int32), params or MixParams(),
+                                                timed=True)
+        return mask, acc, kms, (time.perf_counter() - t0) * 1e3
+
     def match_strings(self, strings: Sequence):
         return self.match_arrays(*pack_strings(strings))
 
     def close(self) -> None:
+        if self._mix is not None:
+            self._mix.close()
+            self._mix = None
         if self.handle:
             _lib.lib().trex_nfa_destroy(self.handle)
             self.handle = None

@@ -140,9 +140,11 @@ class MixEngine:
         _lib.call("trex_mix_replay", self.handle, stream.handle)
 
     def match_arrays(self, data: np.ndarray, offsets: np.ndarray, pattern_of: np.ndarray,
-                     params: MixParams | None = None):
+                     params: MixParams | None = None, timed: bool = False):
         """Host CSR batch -> (final uint64[n], accept uint8[n]); final is the DFA
-        state or the NFA state set of each string."""
+        state or the NFA state set of each string. With ``timed``, also the
+        launch sequence's time in ms (HIP events on its stream)."""
+        from trex_amd.device import Event
         data = np.ascontiguousarray(data, dtype=np.uint8)
         offsets = np.ascontiguousarray(offsets, dtype=np.int64)
         n = len(offsets) - 1
@@ -150,13 +152,18 @@ class MixEngine:
         stream = Stream()
         d_bytes = DeviceBuffer.from_array(data if len(data) else np.zeros(16, np.uint8), stream)
         d_mask, d_acc = DeviceBuffer(8 * max(n, 1)), DeviceBuffer(max(n, 1))
+        e0, e1 = Event(), Event()
+        e0.record(stream)
         self.run(d_bytes.ptr, d_mask.ptr, d_acc.ptr, stream)
+        e1.record(stream)
         if self.error(stream):
             raise _lib.TrexError(-7, "trex_mix_run", "a bounded wait expired")
         mask = d_mask.download(np.uint64, n, stream)
         acc = d_acc.download(np.uint8, n, stream)
         for b in (d_bytes, d_mask, d_acc):
             b.free()
+        if timed:
+            return mask, acc, e0.elapsed_ms(e1)
         return mask, acc
 
     def close(self) -> None:

@@ -37,6 +37,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
+from trex_amd.engine import LONG_STRING, nfa_mask_to_state
 from trex_amd.kgram import auto_k
 from trex_amd.monoid import compute_monoid, monoid_byte_table
 from trex_amd.nfa_matrices import compile_nfa_matrices
@@ -47,10 +48,6 @@ from trex_amd.tables import MAX_TABLE_STATES, dfa_byte_table, tmap_byte_table
 
 CONFIGS = (None, "monoid", "monoid+kgram", "baseline", "nfa", "monoid+gpu", "batched+gpu", "kgram+gpu",
            "monoid_batch+gpu", "prefix+gpu", "fp16_tc+gpu")
-
-# strings longer than this are matched through the chunked look-back scan,
-# cut into chunks of lookback_chunk_len(total bytes)
-LONG_STRING = 1 << 20
 
 
 class OptimizedEngine:
@@ -71,6 +68,8 @@ class OptimizedEngine:
         self._nm = None
         self._table = None          # DfaByteTable walked on the GPU (DFA, monoid or tmap form)
         self._engine = None         # DfaEngine / NfaEngine, created on first match
+        self._nfa_engine = None     # NFA-64 form of a wider DFA for long strings (False: none)
+        self._lookback_target = None  # look-back chunk target of the table
         self._representation = None
         self._scan_backend = None
         self._selection_reason = None
@@ -234,28 +233,57 @@ class OptimizedEngine:
     def match_arrays(self, data: np.ndarray, offsets: np.ndarray):
         """CSR batch (uint8 bytes, int64 offsets) -> (final state, accept) arrays.
         Final states are DFA table states, or 64-bit state masks for NFAs."""
+        final, acc, _ = self._match(data, offsets)
+        return final, acc
+
+    def _match(self, data, offsets):
+        """(final, accept, kernel ms). Batches with a string over
+        ``LONG_STRING`` bytes are cut into chunks matched in parallel: DFAs of
+        at most 16 states by the look-back scan, NFAs (and wider DFAs in their
+        NFA-64 form) by speculative chunks and fix-ups; other automata walk
+        each string on one lane."""
         from trex_amd.engine import DfaEngine, _as_csr
         data, offsets = _as_csr(data=data, offsets=offsets)
         eng = self._gpu()
         lengths = np.diff(offsets)
-        if (isinstance(eng, DfaEngine) and self._table.n_states <= 16 and len(lengths)
-                and int(lengths.max()) > LONG_STRING):
-            return self._match_long(eng, data, offsets)
-        final, acc, _, _ = eng.match_arrays(data, offsets)
-        return final, acc
+        long = len(lengths) > 0 and int(lengths.max()) > LONG_STRING
+        if long and isinstance(eng, DfaEngine):
+            if self._table.n_states <= 16:
+                return self._match_long(eng, data, offsets)
+            nfa = self._nfa_form()
+            if nfa is not None:
+                mask, acc, kms, _ = nfa.match_split(data, offsets)
+                return nfa_mask_to_state(mask, self._table.dead), acc, kms
+        final, acc, kms, _ = eng.match_arrays(data, offsets)
+        return final, acc, kms
+
+    def _nfa_form(self):
+        """NfaEngine of the DFA table's NFA-64 form when it can be split, else None (cached)."""
+        if self._nfa_engine is None:
+            from trex_amd.engine import NfaEngine, dfa_as_nfa
+            form = dfa_as_nfa(self._table)
+            eng = NfaEngine(*form) if form is not None else None
+            self._nfa_engine = eng if eng is not None and eng.can_split() else False
+        return self._nfa_engine or None
 
     def _match_long(self, eng, data, offsets):
-        from trex_amd.device import DeviceBuffer
+        from trex_amd.device import DeviceBuffer, Event, Stream
         from trex_amd.engine import ChunkPlan, DeviceBatch, lookback_chunk_len, lookback_target_chunks
+        if self._lookback_target is None:       # a property of the table: computed once
+            self._lookback_target = lookback_target_chunks(self._table)
+        stream = Stream()
         batch = DeviceBatch(data, offsets, layout="csr")
         total = int(offsets[-1] - offsets[0]) if len(offsets) else 0
-        plan = ChunkPlan.for_batch(batch, lookback_chunk_len(total, lookback_target_chunks(self._table)))
+        plan = ChunkPlan.for_batch(batch, lookback_chunk_len(total, self._lookback_target))
         n = batch.n_strings
         d_f, d_a = DeviceBuffer(n), DeviceBuffer(n)
-        eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr)
-        if plan.error():
+        e0, e1 = Event(), Event()
+        e0.record(stream)
+        eng.match_plan(plan, batch.d_bytes.ptr, d_f.ptr, d_a.ptr, stream)
+        e1.record(stream)
+        if plan.error(stream):
             raise RuntimeError("look-back scan reported a wait timeout")
-        return d_f.download(np.uint8, n), d_a.download(np.uint8, n)
+        return d_f.download(np.uint8, n, stream), d_a.download(np.uint8, n, stream), e0.elapsed_ms(e1)
 
     def match_batch(self, strings: Sequence) -> list:
         """Match a list of strings (``str`` as latin-1, or bytes); returns list[bool]."""
@@ -275,13 +303,7 @@ class OptimizedEngine:
             return [], {"kernel_ms": 0.0, "total_ms": 0.0, "n_strings": 0}
         data, offsets = pack_strings(strings)
         t0 = time.perf_counter()
-        eng = self._gpu()
-        lengths = np.diff(offsets)
-        from trex_amd.engine import DfaEngine
-        if isinstance(eng, DfaEngine) and self._table.n_states <= 16 and int(lengths.max()) > LONG_STRING:
-            _, acc = self._match_long(eng, data, offsets)
-            kms = tms = (time.perf_counter() - t0) * 1e3
-        else:
-            _, acc, kms, tms = eng.match_arrays(data, offsets)
+        _, acc, kms = self._match(data, offsets)
+        tms = (time.perf_counter() - t0) * 1e3
         return [bool(x) for x in acc], {"kernel_ms": kms, "total_ms": tms,
                                         "total_seconds": time.perf_counter() - t0, "n_strings": len(strings)}
