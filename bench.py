@@ -42,9 +42,13 @@ Secondary legs (reported under "secondary"):
     and a saturated burst served vs one synchronous launch per request.
   * host_input: the Python API end to end on host strings, 10,000 str
     strings x 1,000 characters through OptimizedEngine.match_batch (packing,
-    copies, kernel, results as list[bool]), split into its stages.
+    copies, kernel, results as list[bool]), split into its stages;
+  * plain_dfa_cpu_path: the reference's CPU path on its plain DFA workload
+    (seeded regex over abcd, 1,024 x 1,024 B): the Python DFA.simulate
+    restatement, the C oracle on one and on all threads, and the GPU beside
+    them, bit-exact.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs dfa,nfa,scan,prob,mix,serve,host]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs dfa,nfa,scan,prob,mix,serve,host,plain]
 """
 
 from __future__ import annotations
@@ -128,8 +132,8 @@ def main() -> int:
     ap.add_argument("--prefetch", type=int, default=8, help="16-byte vectors in flight per lane (8 or 16)")
     ap.add_argument("--layout", default="interleaved", choices=["interleaved", "staggered", "csr"],
                     help="HBM layout of the DFA batch (the NFA leg reads a staggered copy when this is interleaved)")
-    ap.add_argument("--legs", default="dfa,nfa,scan,prob,mix,serve,host",
-                    help="comma list of dfa, nfa, scan, prob, mix, serve, host")
+    ap.add_argument("--legs", default="dfa,nfa,scan,prob,mix,serve,host,plain",
+                    help="comma list of dfa, nfa, scan, prob, mix, serve, host, plain")
     ap.add_argument("--prob-modes", default="f16x2,bf16x2,f16,bf16")
     ap.add_argument("--mix-strings", type=int, default=MIX_STRINGS, help="strings of the uneven-mix leg")
     ap.add_argument("--mix-tune", type=int, default=1, help="autotune the uneven-mix plan (0: defaults)")
@@ -221,6 +225,9 @@ def main() -> int:
     if "host" in legs:
         secondary["host_input"] = host_leg(args)
         ok &= secondary["host_input"].get("verified") in (None, True)
+    if "plain" in legs:
+        secondary["plain_dfa_cpu_path"] = plain_leg(args, stream)
+        ok &= secondary["plain_dfa_cpu_path"]["verified_bit_exact"]
 
     if not result:   # a run without the headline leg: report what ran
         result = {"metric": "secondary legs only", "value": None, "n_gpus": 1, "steps": args.steps,
@@ -532,6 +539,69 @@ def host_leg(args) -> dict:
            "reference_pack_ms_quoted": 88.0, "verified": verified}
     log(f"host input: pack {pack_ms:.2f} ms, match_arrays (copies + kernel) {arrays_ms:.2f} ms, "
         f"match_batch end to end {total_ms:.2f} ms (kernel {timing['kernel_ms']:.4f} ms), verified={verified}")
+    return out
+
+
+def plain_leg(args, stream) -> dict:
+    """The reference's CPU path on its own plain DFA workload: a seeded regex
+    over abcd (<= 16-state minimal DFA, bytes outside the alphabet -> dead
+    state), 1,024 strings x 1,024 bytes; final state and accept bit per
+    string. Times the Python restatement of DFA.simulate (one thread), the C
+    oracle's byte walk on one thread and on all threads (median of 3 after a
+    warm-up, the reference's method), and beside them the GPU on the same
+    batch (device-resident kernel time, and end to end from host arrays)."""
+    from oracle import cpu as ocpu
+    from oracle.dfa import dfa_simulate
+    from trex_amd.device import DeviceBuffer
+    from trex_amd.engine import DeviceBatch, DfaEngine
+    from trex_amd.tables import dfa_byte_table
+    from trex_amd.workloads import PLAIN_DFA_SEED, plain_dfa_workload
+
+    pattern, dfa, data, offsets = plain_dfa_workload()
+    tab = dfa_byte_table(dfa)
+    n = len(offsets) - 1
+    chars = int(offsets[-1])
+    strings = [bytes(data[offsets[i]:offsets[i + 1]]).decode("latin-1") for i in range(n)]
+
+    def median3(fn):
+        fn()
+        return float(np.median([_wall(fn) for _ in range(3)]))
+
+    py_res = []
+    py_ms = median3(lambda: py_res.__setitem__(slice(None), [dfa_simulate(dfa, s) for s in strings]))
+    threads = ocpu.default_threads()
+    c1_ms = median3(lambda: ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offsets, threads=1))
+    cn_ms = median3(lambda: ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offsets, threads=threads))
+    rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offsets)
+
+    eng = DfaEngine(tab)
+    batch = DeviceBatch(data, offsets, layout="csr")
+    d_f, d_a = DeviceBuffer(n), DeviceBuffer(n)
+    kms = time_launches(lambda: eng.match_batch_device(batch, d_f.ptr, d_a.ptr, stream), stream, 20)
+    gf, ga = d_f.download(np.uint8, n, stream), d_a.download(np.uint8, n, stream)
+    e2e = [eng.match_arrays(data, offsets) for _ in range(4)][1:]
+    e2e_ms = float(np.median([x[3] for x in e2e]))
+    same = bool(np.array_equal(gf, rf) and np.array_equal(ga, ra) and np.array_equal(e2e[-1][0], rf)
+                and [bool(x) for x in ra] == py_res)
+    for b in (d_f, d_a):
+        b.free()
+    batch.free()
+    eng.close()
+    out = {"workload": f"plain DFA, seed {PLAIN_DFA_SEED}: regex over abcd, {tab.n_states}-state table, "
+                       f"{n} x {chars // n} B",
+           "pattern": pattern, "accept_rate": round(float(ra.mean()), 4),
+           "python_dfa_simulate_mchars_s": round(chars / (py_ms * 1e-3) / 1e6, 2),
+           "c_oracle_1_thread_mchars_s": round(chars / (c1_ms * 1e-3) / 1e6, 1),
+           "c_oracle_threads": threads,
+           "c_oracle_all_threads_mchars_s": round(chars / (cn_ms * 1e-3) / 1e6, 1),
+           "cpu": cpu_model(), "os_cpu_count": os.cpu_count(),
+           "gpu_kernel_ms": round(kms, 4), "gpu_kernel_gchars_s": round(chars / (kms * 1e-3) / 1e9, 2),
+           "gpu_end_to_end_ms": round(e2e_ms, 4),
+           "reference_dfa_simulate_mchars_s_quoted": [18.4, 20.8],
+           "verified_bit_exact": same}
+    log(f"plain DFA (CPU path): Python DFA.simulate {out['python_dfa_simulate_mchars_s']} Mchar/s, C oracle "
+        f"{out['c_oracle_1_thread_mchars_s']} Mchar/s (1 thread) / {out['c_oracle_all_threads_mchars_s']} Mchar/s "
+        f"({threads} threads); GPU kernel {kms:.4f} ms, end to end {e2e_ms:.3f} ms; bit-exact={same}")
     return out
 
 
@@ -1140,9 +1210,7 @@ def serve_leg(args, stream) -> dict:
 
 
 def sweep(args, data, offsets, batch, engines, outs, stream) -> int:
-    """Kernel variants x prefetch x layout, plus the read probes."""
-    from trex_amd import _lib
-    from trex_amd.device import DeviceBuffer
+    """Kernel variants x prefetch x layout."""
     from trex_amd.engine import DeviceBatch
     n = len(offsets) - 1
     layouts = {batch.layout: batch}
@@ -1153,17 +1221,6 @@ def sweep(args, data, offsets, batch, engines, outs, stream) -> int:
     for i, eng in enumerate(engines):
         eng.match_batch_device(batch, outs[i][0].ptr, outs[i][1].ptr, stream)
         ref.append(outs[i][0].download(np.uint8, n, stream))
-    sink = DeviceBuffer(max(8192 * 256, n) * 4)
-    for mode, tag in ((0, ""), (4, " nt")):
-        ms = time_launches(lambda: _lib.call("trex_read_probe", batch.d_bytes.ptr, data.nbytes, None, None, 0, mode,
-                                             sink.ptr, stream.handle), stream, 5)
-        log(f"read probe coalesced stream{tag:3s}   : {ms:.4f} ms = {data.nbytes / (ms * 1e-3) / 1e9:.0f} GB/s")
-    for name, b in layouts.items():
-        for mode in ((3, 5) if name == "interleaved" else (1, 2)):
-            ms = time_launches(lambda: _lib.call("trex_read_probe", b.d_bytes.ptr, 16, b.d_begin.ptr, b.d_end.ptr,
-                                                 n, mode, sink.ptr, stream.handle), stream, 5)
-            log(f"read probe {name:11s} pf{16 if mode == 2 else 8:2d}{' nt' if mode == 5 else '   '}     : {ms:.4f} ms = "
-                f"{data.nbytes / (ms * 1e-3) / 1e9:.0f} GB/s")
     for name, b in layouts.items():
         for v, pf in [("k1_16", 16), ("k2_window", 8), ("k2_window", 16), ("k2_window_f", 8), ("k2_window_f", 16),
                       ("k2_window_fr", 8), ("k2_window_fr", 16)]:

@@ -451,16 +451,6 @@ int trex_plan_set_hold(trex_plan* plan, uint32_t hold);
  * counters and the achievable-bandwidth probe. No reference counterpart. */
 int trex_copy(void* d_dst, const void* d_src, size_t n_bytes, void* stream);
 
-/* HBM read-rate probes (no reference counterpart). mode 0: coalesced
- * grid-stride read of d_src[0..n_bytes), d_sink holds 8192 * 256 words;
- * mode 1 / 2: the per-string access pattern of the window kernels (8 / 16
- * vectors in flight) over 16-byte-aligned ranges [d_begin[i], d_end[i]),
- * d_sink holds n_strings words; mode 3: the same over interleaved-layout
- * ranges (trex_dfa_match_interleaved); mode 4 / 5: modes 0 / 3 with
- * non-temporal loads (as the DFA kernels issue them). */
-int trex_read_probe(const void* d_src, size_t n_bytes, const int64_t* d_begin, const int64_t* d_end,
-                    int64_t n_strings, int mode, uint32_t* d_sink, void* stream);
-
 /* ------------------------------------------------------------------------
  * Reference-compatible DFA entry points (process-global engines).
  * ------------------------------------------------------------------------ */

@@ -255,3 +255,20 @@ def test_config2_shape_slice_bit_exact(gpu_lib):
         a = d_a.download(np.uint8, n)
         rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offs)
         assert np.array_equal(f, rf) and np.array_equal(a, ra), f"pattern {p}"
+
+
+def test_plain_dfa_workload(gpu_lib):
+    """The reference CPU path's plain DFA workload (seeded regex over abcd,
+    <= 16 states, 1,024 x 1,024 B): final state and accept bit of every string
+    equal the C oracle's, the accept bits equal DFA.simulate, and the final
+    states equal the dict walk's."""
+    from oracle.dfa import dfa_final_state
+    from trex_amd.workloads import plain_dfa_workload
+    pattern, dfa, data, offsets = plain_dfa_workload()
+    tab = dfa_byte_table(dfa)
+    assert tab.n_states <= 16 and len(offsets) == 1025 and int(offsets[-1]) == 1 << 20
+    eng = DfaEngine(tab)
+    f, a = _check(eng, tab, data, offsets)
+    strings = [bytes(data[offsets[i]:offsets[i + 1]]).decode("latin-1") for i in range(1024)]
+    assert [bool(x) for x in a] == [dfa.simulate(s) for s in strings] == [dfa_simulate(dfa, s) for s in strings]
+    assert [int(x) for x in f[:128]] == [dfa_final_state(dfa, s, tab.dead) for s in strings[:128]]

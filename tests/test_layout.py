@@ -88,3 +88,18 @@ def test_pack_strings_rejects_characters_beyond_latin1():
     from trex_amd.engine import pack_strings
     with pytest.raises(UnicodeEncodeError):
         pack_strings(["ok", "Ā"])
+
+
+def test_plain_dfa_workload_shape():
+    """The plain DFA workload is seeded and deterministic: a regex over abcd
+    whose table has at most 16 states, 1,024 strings of 1,024 bytes over the
+    alphabet."""
+    from trex_amd.tables import dfa_byte_table
+    from trex_amd.workloads import plain_dfa_workload
+    p1, d1, data1, off1 = plain_dfa_workload()
+    p2, _, data2, off2 = plain_dfa_workload()
+    assert p1 == p2 and np.array_equal(data1, data2) and np.array_equal(off1, off2)
+    assert set(p1) <= set("abcd()|*")
+    assert dfa_byte_table(d1).n_states <= 16
+    assert len(off1) == 1025 and np.all(np.diff(off1) == 1024)
+    assert set(np.unique(data1).tolist()) == {97, 98, 99, 100}

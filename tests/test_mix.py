@@ -195,6 +195,48 @@ def test_gpu_mix_matches_oracle(gpu_lib, params, form):
     assert len(bad_idx) == 0, (bad_idx[:10], pattern_of[bad_idx[:10]], np.diff(offsets)[bad_idx[:10]])
 
 
+TUNED = MixParams(dfa_split=64 << 10, dfa_chunk=64 << 10, nfa_split=32 << 10, nfa_chunk=32 << 10, ck=64)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["nibble", "slice"])
+def test_gpu_mix_long_strings_tuned_split(gpu_lib, form):
+    """Strings of 2-4 MiB at the tuned split and chunk lengths (DFA 64 KiB
+    chunks chained by the look-back, NFA 32 KiB speculative chunks with
+    fix-ups, a permutation NFA whose chunks never meet a checkpoint, and a
+    DFA that never synchronises), beside short strings of every pattern:
+    bit-exact against the oracle."""
+    from trex_amd.engine import DfaEngine
+    from trex_amd.regex_to_dfa import compile_regex
+    from trex_amd.tables import dfa_byte_table
+    rng = np.random.default_rng(41)
+    pats, refs = _patterns(rng, 2, 3, permutation_nfa=1)
+    # even number of '@' (0x40) over the window: a permutation automaton
+    even = dfa_byte_table(compile_regex("([A-_]*@[A-_]*@)*[A-_]*"))
+    pats.append(DfaEngine(even))
+    refs.append(("dfa", even))
+    n_p = len(pats)
+    long_len = rng.integers(2 << 20, 4 << 20, size=2 * n_p)
+    short_len = rng.integers(0, 70_000, size=300)
+    lengths = np.concatenate([long_len, short_len]).astype(np.int64)
+    pattern_of = np.concatenate([np.repeat(np.arange(n_p), 2), rng.integers(0, n_p, size=300)]).astype(np.int32)
+    perm = rng.permutation(len(lengths))
+    lengths, pattern_of = lengths[perm], pattern_of[perm]
+    offsets = np.zeros(len(lengths) + 1, np.int64)
+    np.cumsum(lengths, out=offsets[1:])
+    data = rng.integers(0x40, 0x60, size=int(offsets[-1]), dtype=np.uint8)
+    bad = rng.choice(len(data), size=12, replace=False)
+    data[bad] = rng.integers(0, 0x40, size=12, dtype=np.uint8)
+    m = MixEngine(pats)
+    m.set_form(form)
+    mask, acc = m.match_arrays(data, offsets, pattern_of, TUNED)
+    info = m.info()
+    assert info["chunks"] > 2 * n_p * 32 and info["fixups"] >= 6
+    ref_mask, ref_acc = _oracle(refs, data, offsets, pattern_of)
+    bad_idx = np.flatnonzero((mask != ref_mask) | (acc != ref_acc))
+    assert len(bad_idx) == 0, (bad_idx[:10], pattern_of[bad_idx[:10]], np.diff(offsets)[bad_idx[:10]])
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("form", ["nibble", "slice"])
 def test_gpu_mix_graph_replay(gpu_lib, form):

@@ -2,12 +2,16 @@
 that every request's result is the same served in a mix of concurrent
 requests as computed alone (bit-exact for DFA / NFA results, and also for the
 probabilistic log-likelihoods, whose per-sequence arithmetic does not depend on
-the other sequences of a launch)."""
+the other sequences of a launch), and that every served result agrees with the
+oracle: DFA / NFA bit-exact against the C walk, log-likelihoods within 1e-3
+relative of the FP64 forward pass."""
 
 from __future__ import annotations
 
 import numpy as np
 import pytest
+
+from oracle_checks import assert_loglik_matches_oracle, assert_walk_matches_oracle
 
 
 def test_arena_first_fit_and_merge():
@@ -89,10 +93,12 @@ def test_gpu_served_equals_alone(gpu_lib, mix_form):
                     a, b = e.loglik_ragged(np.concatenate(parts) if offs[-1] else np.zeros(0, np.uint8), offs, **kw)
                     ll[idx], lm[idx] = a, b
             assert np.array_equal(r.result[0], ll) and np.array_equal(r.result[1], lm)
+            assert_loglik_matches_oracle(e, r.data, r.offsets, r.result[0])
         else:
             final, acc, _, _ = e.match_arrays(r.data, r.offsets)
             assert np.array_equal(r.result[0].astype(final.dtype), final), r.pattern
             assert np.array_equal(r.result[1], acc)
+            assert_walk_matches_oracle(e, r.data, r.offsets, r.result[0], r.result[1])
         assert r.arrival <= r.admitted <= r.launched <= r.done
 
 
@@ -126,6 +132,7 @@ def test_gpu_expired_waits_fail_the_tick_not_the_server(gpu_lib):
         assert r.error is None
         final, acc, _, _ = engines[0].match_arrays(r.data, r.offsets)
         assert np.array_equal(r.result[0], final) and np.array_equal(r.result[1], acc)
+        assert_walk_matches_oracle(engines[0], r.data, r.offsets, r.result[0], r.result[1])
 
 
 @pytest.mark.gpu
@@ -183,7 +190,9 @@ def test_gpu_restart_from_table_file(gpu_lib, tmp_path):
         if isinstance(e, ProbNfaEngine):
             ll, lm = e.loglik_ragged(r.data, r.offsets)
             assert np.array_equal(r.result[0], ll) and np.array_equal(r.result[1], lm)
+            assert_loglik_matches_oracle(e, r.data, r.offsets, r.result[0])
         else:
             final, acc, _, _ = e.match_arrays(r.data, r.offsets)
             assert np.array_equal(r.result[0].astype(final.dtype), final)
             assert np.array_equal(r.result[1], acc)
+            assert_walk_matches_oracle(e, r.data, r.offsets, r.result[0], r.result[1])

@@ -10,6 +10,7 @@ import pytest
 import trex_amd.optimized_engine as oe
 from oracle.dfa import dfa_simulate, simulate_monoid
 from oracle.nfa import simulate_nfa
+from oracle_checks import assert_loglik_matches_oracle, assert_walk_matches_oracle
 from trex_amd.table_cache import (CompiledPattern, ProbModel, TableCache, TableFileError, _pack, _unpack,
                                   default_cache, load_tables, pattern_key, save_tables)
 from trex_amd.workloads import dirichlet_pnfa
@@ -260,11 +261,14 @@ def test_gpu_restored_engines_match_originals(gpu_lib, tmp_path):
             fa, aa = a.match_arrays(data, offsets)[:2]
             fb, ab = b.match_arrays(data, offsets)[:2]
             assert np.array_equal(fa, fb) and np.array_equal(aa, ab)
+            # the original's tables walked by the oracle decide the restored results
+            assert_walk_matches_oracle(a, data, offsets, fb, ab)
         else:
             assert isinstance(a, ProbNfaEngine) and b.info()["mode"] == "bf16x2"
             la, ma = a.loglik_ragged(sym, offsets)
             lb, mb = b.loglik_ragged(sym, offsets)
             assert np.array_equal(la, lb) and np.array_equal(ma, mb)
+            assert_loglik_matches_oracle(a, sym, offsets, lb)
     for e in originals + restored:
         e.close()
 
@@ -277,4 +281,15 @@ def test_gpu_cache_hit_engine_matches_fresh(gpu_lib, tmp_path):
         fresh = oe.OptimizedEngine("(ab|cd|ef)*(x|y)(ab|cd)*z", config=config, cache=False)
         oe.OptimizedEngine("(ab|cd|ef)*(x|y)(ab|cd)*z", config=config, cache=tmp_path)
         hit = oe.OptimizedEngine("(ab|cd|ef)*(x|y)(ab|cd)*z", config=config, cache=TableCache(tmp_path))
-        assert hit.match(strings) == fresh.match(strings)
+        got = hit.match(strings)
+        assert got == fresh.match(strings)
+        # each configuration's reference semantics, from the fresh compile
+        if config == "nfa":
+            want = [simulate_nfa(fresh._nm, s) for s in strings]
+        elif config == "monoid_batch+gpu":
+            want = [simulate_monoid(fresh._md, s) for s in strings]
+        elif config == "prefix+gpu":
+            want = [dfa_simulate(fresh._dfa, "".join(c for c in s if c in fresh._dfa.alphabet)) for s in strings]
+        else:
+            want = [dfa_simulate(fresh._dfa, s) for s in strings]
+        assert got == want, config

@@ -126,7 +126,6 @@ _SIGNATURES = {
     "trex_dfa_match_plan_image": (c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, c_void_p,
                                           ctypes.c_uint32, c_void_p]),
     "trex_copy": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
-    "trex_read_probe": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p]),
     "trex_dfa_match": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
     "trex_dfa_match_host": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, f32p, f32p]),
     # reference-compatible

@@ -245,25 +245,6 @@ int trex_dfa_match_host(trex_dfa* h, const uint8_t* bytes, const int64_t* offset
     return TREX_OK;
 }
 
-int trex_read_probe(const void* d_src, size_t n_bytes, const int64_t* d_begin, const int64_t* d_end,
-                    int64_t n_strings, int mode, uint32_t* d_sink, void* stream) {
-    if (((uintptr_t)d_src & 15) || (n_bytes & 15)) return fail(TREX_ERR_INVALID, "probe buffer must be 16-byte aligned");
-    hipStream_t s = (hipStream_t)stream;
-    if (mode < 0 || mode > 5) return fail(TREX_ERR_INVALID, "probe mode %d: 0-5", mode);
-    if (mode == 0 || mode == 4) {
-        int64_t n16 = (int64_t)(n_bytes / 16);
-        const unsigned grid = 8192;   // d_sink must hold grid * 256 words
-        void* args[] = {&d_src, &n16, &d_sink};
-        return launch(mode == 4 ? "read_probe_stream_nt" : "read_probe_stream", grid, kBlock, 0, s, args);
-    }
-    if (!d_begin || !d_end || n_strings <= 0) return fail(TREX_ERR_INVALID, "string probe needs ranges");
-    const unsigned grid = (unsigned)((n_strings + kBlock - 1) / kBlock);
-    void* args[] = {&d_src, &d_begin, &d_end, &n_strings, &d_sink};
-    const char* name = mode == 5 ? "read_probe_il_nt_pf8" : mode == 3 ? "read_probe_il_pf8"
-                     : mode == 2 ? "read_probe_strings_pf16" : "read_probe_strings_pf8";
-    return launch(name, grid, kBlock, 0, s, args);
-}
-
 int trex_copy(void* d_dst, const void* d_src, size_t n_bytes, void* stream) {
     if (n_bytes % 16 != 0) return fail(TREX_ERR_INVALID, "copy size must be a multiple of 16");
     if (((uintptr_t)d_dst | (uintptr_t)d_src) & 15) return fail(TREX_ERR_INVALID, "copy buffers must be 16-byte aligned");

@@ -275,99 +275,6 @@ TREX_K1_KERNEL(dfa_k1, -1, false)
 TREX_K1_KERNEL(dfa_k1_il, -1, true)
 
 // ---------------------------------------------------------------------------
-// Read probes: the HBM read rate of (a) a fully coalesced grid-stride stream
-// and (b) the per-string access pattern of the window kernels with no table
-// work (XOR of every word), over the same begin/end ranges the DFA kernels
-// take. Each lane writes one word so the loads are kept.
-// ---------------------------------------------------------------------------
-template <bool NT>
-__device__ __forceinline__ void read_probe_stream_body(const uint4* __restrict__ src, int64_t n16,
-                                                       uint32_t* __restrict__ sink)
-{
-    const int64_t stride = (int64_t)gridDim.x * TREX_BLOCK;
-    uint32_t x = 0;
-    for (int64_t i = (int64_t)blockIdx.x * TREX_BLOCK + threadIdx.x; i < n16; i += stride) {
-        const uint4 v = NT ? load16_nt(src, i) : src[i];
-        x ^= v.x ^ v.y ^ v.z ^ v.w;
-    }
-    sink[(int64_t)blockIdx.x * TREX_BLOCK + threadIdx.x] = x;
-}
-
-extern "C" __global__ void __launch_bounds__(TREX_BLOCK)
-read_probe_stream(const uint4* __restrict__ src, int64_t n16, uint32_t* __restrict__ sink)
-{
-    read_probe_stream_body<false>(src, n16, sink);
-}
-
-extern "C" __global__ void __launch_bounds__(TREX_BLOCK)
-read_probe_stream_nt(const uint4* __restrict__ src, int64_t n16, uint32_t* __restrict__ sink)
-{
-    read_probe_stream_body<true>(src, n16, sink);
-}
-
-template <int PF>
-__device__ __forceinline__ void read_probe_strings_body(const uint8_t* __restrict__ bytes,
-                                                        const int64_t* __restrict__ begin,
-                                                        const int64_t* __restrict__ endp, int64_t n_strings,
-                                                        uint32_t* __restrict__ sink)
-{
-    const int64_t sid = (int64_t)blockIdx.x * TREX_BLOCK + threadIdx.x;
-    if (sid >= n_strings) return;
-    const int64_t p = begin[sid];
-    const int64_t nvec = (endp[sid] - p) >> 4;   // probe ranges are 16-byte aligned
-    uint32_t x = 0;
-    stream_body<PF>(reinterpret_cast<const uint4*>(bytes + p), nvec,
-                    [&](uint4 v) { x ^= v.x ^ v.y ^ v.z ^ v.w; });
-    sink[sid] = x;
-}
-
-extern "C" TREX_STRING_KERNEL
-read_probe_strings_pf8(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
-                       const int64_t* __restrict__ end, int64_t n_strings, uint32_t* __restrict__ sink)
-{
-    read_probe_strings_body<8>(bytes, begin, end, n_strings, sink);
-}
-
-extern "C" __global__ void __launch_bounds__(TREX_BLOCK)
-read_probe_strings_pf16(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
-                        const int64_t* __restrict__ end, int64_t n_strings, uint32_t* __restrict__ sink)
-{
-    read_probe_strings_body<16>(bytes, begin, end, n_strings, sink);
-}
-
-// The interleaved layout's access pattern (one 1 KiB row per wave load),
-// with ordinary or non-temporal loads.
-template <bool NT>
-__device__ __forceinline__ void read_probe_il_body(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
-                                                   const int64_t* __restrict__ endp, int64_t n_strings,
-                                                   uint32_t* __restrict__ sink)
-{
-    const int64_t sid = (int64_t)blockIdx.x * TREX_BLOCK + threadIdx.x;
-    if (sid >= n_strings) return;
-    const int64_t p = begin[sid];
-    const int64_t nvec = (endp[sid] - p) >> 4;
-    uint32_t x = 0;
-    auto eat = [&](uint4 v) { x ^= v.x ^ v.y ^ v.z ^ v.w; };
-    if constexpr (NT) stream_body_nt<8, kIlGroup>(reinterpret_cast<const uint4*>(bytes + p), nvec, eat);
-    else stream_body<8, kIlGroup>(reinterpret_cast<const uint4*>(bytes + p), nvec, eat);
-    sink[sid] = x;
-}
-
-extern "C" TREX_STRING_KERNEL
-read_probe_il_pf8(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
-                  const int64_t* __restrict__ endp, int64_t n_strings, uint32_t* __restrict__ sink)
-{
-    read_probe_il_body<false>(bytes, begin, endp, n_strings, sink);
-}
-
-extern "C" TREX_STRING_KERNEL
-read_probe_il_nt_pf8(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ begin,
-                     const int64_t* __restrict__ endp, int64_t n_strings, uint32_t* __restrict__ sink)
-{
-    read_probe_il_body<true>(bytes, begin, endp, n_strings, sink);
-}
-
-// ---------------------------------------------------------------------------
 // Plain 16-byte copy, the reference point for HBM traffic counters.
 // n16 = number of 16-byte vectors.
 // ---------------------------------------------------------------------------

@@ -9,6 +9,8 @@
 * :func:`random_nfa64` draws a Boolean NFA with each per-symbol edge present
   with probability ``p``, start state 0 and ``n_accept`` random accept states.
 * :func:`uniform_batch` / :func:`loguniform_batch` build CSR byte batches.
+* :func:`plain_dfa_workload` is the CPU-path plain DFA workload: one seeded
+  regex over ``abcd`` and 1,024 strings of 1,024 bytes over that alphabet.
 """
 
 from __future__ import annotations
@@ -97,3 +99,18 @@ def ragged_batch(rng: np.random.Generator, lengths: np.ndarray, lo: int, hi: int
 
 def loguniform_lengths(rng: np.random.Generator, n: int, lo: int, hi: int) -> np.ndarray:
     return np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), size=n))).astype(np.int64)
+
+
+PLAIN_DFA_SEED = 1
+
+
+def plain_dfa_workload(seed: int = PLAIN_DFA_SEED, n_strings: int = 1024, length: int = 1024):
+    """The plain DFA workload of the reference's CPU path: a seeded regex over
+    ``abcd`` (concatenation / alternation / star, depth <= 4) whose minimal
+    complete DFA has at most 16 states (bytes outside the alphabet go to the
+    dead state), and ``n_strings`` strings of ``length`` bytes uniform over
+    the alphabet. Returns (pattern, dfa, data, offsets)."""
+    rng = np.random.default_rng(seed)
+    pattern, dfa = random_dfa_pattern(rng, ALPHABET_ABCD)
+    data, offsets = uniform_batch(rng, n_strings, length, ord("a"), ord("d") + 1)
+    return pattern, dfa, data, offsets
