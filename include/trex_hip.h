@@ -92,6 +92,24 @@ int trex_event_sync(void* ev);
 /* Later work on `stream` waits for `ev` (no host wait). */
 int trex_stream_wait_event(void* stream, void* ev);
 
+/* HIP graphs of captured launches (no reference counterpart: it launches each
+ * kernel from the host, cuda/monoid_batch.cu:647-658). trex_graph_begin starts
+ * capturing `stream` (not the default stream); every asynchronous trex_* call
+ * on it until trex_graph_end is recorded instead of run, and trex_graph_end
+ * instantiates the recording. trex_graph_launch replays it on any stream with
+ * the same device pointers and parameters. Entry points that copy from host
+ * memory, synchronise or allocate (the *_host forms, the first call of a
+ * look-back plan or mixed queue that sizes its buffers) cannot be captured:
+ * run them once before capturing, or use trex_dfa_match_chunked_static, which
+ * allocates nothing. trex_graph_abort ends a capture that failed part way and
+ * drops what it recorded. */
+typedef struct trex_graph trex_graph;
+int trex_graph_begin(void* stream);
+int trex_graph_end(void* stream, trex_graph** out);
+int trex_graph_abort(void* stream);
+int trex_graph_launch(trex_graph* g, void* stream);
+int trex_graph_destroy(trex_graph* g);
+
 /* Synthetic input in HBM (no reference counterpart): bytes [0, n_bytes) of
  * d_out (16-byte aligned, n_bytes a multiple of 16) become lo + (r & (span-1)),
  * r = byte (i % 8) of splitmix64(seed ^ ((i / 8) * 0x9E3779B97F4A7C15)). */
@@ -434,6 +452,11 @@ int trex_plan_build_image(const trex_plan* plan, const uint8_t* src, int64_t src
  * buffer's size, refused when smaller than trex_plan_image_size). */
 int trex_dfa_match_plan_image(trex_dfa* h, trex_plan* plan, const uint8_t* d_image, int64_t image_bytes,
                               uint8_t* d_final_state, uint8_t* d_accept, uint32_t max_spins, void* stream);
+/* Phase 1 of the plan's later launches with 1 or 2 dependent chains per lane:
+ * 2 walks the two halves of each chunk side by side (the second half from
+ * the identity function) and joins them with one composition. Same results;
+ * a tuning choice. No counterpart in the reference. */
+int trex_plan_set_chains(trex_plan* plan, int chains);
 /* Error word of the plan's last launch (0 = none; bit 0 = a wait expired),
  * read synchronously after the work queued on `stream`. */
 int trex_plan_error(trex_plan* plan, uint32_t* error, void* stream);
@@ -484,8 +507,9 @@ int monoid_batch_engine_set_kgram(const uint8_t* fused_compose, const uint8_t* k
 /* cuda/monoid_batch.cu:647-658 */
 int monoid_batch_engine_dispatch(const uint8_t* raw_concat, const int* offsets, int* results, int B,
                                  int total_chars, float* kernel_ms, float* total_ms);
-/* cuda/monoid_batch.cu:660-671 (long strings; same kernels, the split of long
- * strings is internal) */
+/* cuda/monoid_batch.cu:660-671 (long strings): with M <= 16 every string is
+ * cut into chunks chained by the decoupled look-back (trex_dfa_match_plan, one
+ * launch); wider monoids walk each string on one lane. */
 int monoid_batch_engine_dispatch_prefix(const uint8_t* raw_concat, const int* offsets, int* results, int B,
                                         int total_chars, float* kernel_ms, float* total_ms);
 /* cuda/monoid_batch.cu:640-645 */

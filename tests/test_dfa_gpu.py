@@ -272,3 +272,46 @@ def test_plain_dfa_workload(gpu_lib):
     strings = [bytes(data[offsets[i]:offsets[i + 1]]).decode("latin-1") for i in range(1024)]
     assert [bool(x) for x in a] == [dfa.simulate(s) for s in strings] == [dfa_simulate(dfa, s) for s in strings]
     assert [int(x) for x in f[:128]] == [dfa_final_state(dfa, s, tab.dead) for s in strings[:128]]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("regex,alpha", [("(a|b)*abb", "ab"), ("(b*ab*ab*)*b*", "ab"), ("(a|b|c|d)*a(b|c)", "abcd"),
+                                         ("(a|c|(b|d)(b|d))*", "abcd"), ("(a|b)*a(a|b)(a|b)", "ab")])
+def test_reference_compat_monoid_engine_batch_tables(gpu_lib, regex, alpha):
+    """precompute_batch_tables' output (the tables the reference's bridge hands
+    to monoid_batch_engine_init, src/monoid.py:195-287) through the reference
+    entry points, both dispatch forms, against simulate_monoid: short strings,
+    strings with characters outside the alphabet (the identity), and strings
+    long enough for the prefix dispatch to split."""
+    import ctypes
+    from oracle.dfa import simulate_monoid
+    from trex_amd import _lib
+    from trex_amd.monoid import compute_monoid, precompute_batch_tables
+    from trex_amd.simulation import DFAMatrices
+    dm = DFAMatrices(compile_regex(regex))
+    md = compute_monoid(dm)
+    t = precompute_batch_tables(md, dm)
+    rng = np.random.default_rng(len(regex))
+    chars = np.array(list(alpha + "xZ"))
+    p = np.r_[np.full(len(alpha), 0.98 / len(alpha)), 0.01, 0.01]
+    lengths = np.r_[[0, 1, 2, 3], rng.integers(0, 300, 400), [70_000, 200_001]]
+    strings = ["".join(rng.choice(chars, int(n), p=p)) for n in lengths]
+    want = [int(simulate_monoid(md, s)) for s in strings]
+    assert 0 < sum(want) < len(want)
+    _lib.call("monoid_batch_engine_init", t["M"], t["sigma_ext"], t["identity_idx"],
+              t["char_compose"].ctypes.data_as(_lib.u8p), t["raw_char_map"].ctypes.data_as(_lib.u8p),
+              t["accept"].ctypes.data_as(_lib.u8p), t["monoid_compose"].ctypes.data_as(_lib.u8p), 1 << 22, 1 << 12)
+    if t["kgram_compose"] is not None:
+        _lib.call("monoid_batch_engine_set_kgram", t["fused_compose"].ctypes.data_as(_lib.u8p),
+                  t["kgram_compose"].ctypes.data_as(_lib.u8p), t["kgram_k"], t["sigma_k"], t["sigma"])
+    data, offs = pack_strings(strings)
+    offs32 = offs.astype(np.int32)
+    km, tm = ctypes.c_float(), ctypes.c_float()
+    try:
+        for entry in ("monoid_batch_engine_dispatch", "monoid_batch_engine_dispatch_prefix"):
+            res = np.full(len(strings), -1, np.int32)
+            _lib.call(entry, data.ctypes.data_as(_lib.u8p), offs32.ctypes.data_as(_lib.intp),
+                      res.ctypes.data_as(_lib.intp), len(strings), int(offs[-1]), ctypes.byref(km), ctypes.byref(tm))
+            assert res.tolist() == want, entry
+    finally:
+        _lib.lib().monoid_batch_engine_destroy()

@@ -250,7 +250,7 @@ def test_plan_byte_kernel_two_windows(gpu_lib, chunk_len):
     assert np.array_equal(f, rf) and np.array_equal(a, ra)
 
 
-def _run_plan_image(tab, data, offsets, chunk_len):
+def _run_plan_image(tab, data, offsets, chunk_len, chains=None):
     """Match through the plan's chunk-interleaved image; also returns the
     plain (CSR) results of the same plan."""
     from trex_amd.device import DeviceBuffer
@@ -258,6 +258,8 @@ def _run_plan_image(tab, data, offsets, chunk_len):
     eng = DfaEngine(tab)
     batch = DeviceBatch(data, offsets, layout="csr")
     plan = ChunkPlan.for_batch(batch, chunk_len)
+    if chains is not None:
+        plan.set_chains(chains)
     image = plan.build_image(data)
     assert plan.image_size() >= len(data) and plan.image_size() % 1024 == 0
     d_img = DeviceBuffer(image.nbytes)
@@ -276,9 +278,10 @@ def _run_plan_image(tab, data, offsets, chunk_len):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("chains", [1, 2])
 @pytest.mark.parametrize("pattern,chunk_len", [("random32", 1024), ("random32", 8192), (EVEN_A, 64),
                                                (EVEN_A, 4096), ("(a|B)*aBB", 512)])
-def test_plan_image_matches_oracle(gpu_lib, pattern, chunk_len):
+def test_plan_image_matches_oracle(gpu_lib, pattern, chunk_len, chains):
     """The chunk-interleaved image gives the sequential walk's results (and
     the CSR launch's) over ragged lengths, partial vectors, empty strings,
     synchronising and permutation automata and the byte kernel."""
@@ -298,7 +301,7 @@ def test_plan_image_matches_oracle(gpu_lib, pattern, chunk_len):
     np.cumsum(lengths, out=offs[1:])
     data = np.frombuffer(alphabet, np.uint8)[rng.integers(0, len(alphabet), int(offs[-1]))].copy()
     data[rng.integers(0, len(data), 8)] = 0x7e
-    fi, ai, fc, ac = _run_plan_image(tab, data, offs, chunk_len)
+    fi, ai, fc, ac = _run_plan_image(tab, data, offs, chunk_len, chains)
     rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offs)
     assert np.array_equal(fi, rf) and np.array_equal(ai, ra)
     assert np.array_equal(fc, rf) and np.array_equal(ac, ra)

@@ -112,6 +112,11 @@ _SIGNATURES = {
     "trex_mix_set_form": (c_int, [c_void_p, c_int]),
     "trex_mix_set_grid": (c_int, [c_void_p, c_int64]),
     "trex_mix_stats": (c_int, [c_void_p, POINTER(ctypes.c_uint32), c_void_p]),
+    "trex_graph_begin": (c_int, [c_void_p]),
+    "trex_graph_end": (c_int, [c_void_p, vpp]),
+    "trex_graph_abort": (c_int, [c_void_p]),
+    "trex_graph_launch": (c_int, [c_void_p, c_void_p]),
+    "trex_graph_destroy": (c_int, [c_void_p]),
     "trex_mix_capture": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_uint32, c_void_p]),
     "trex_mix_replay": (c_int, [c_void_p, c_void_p]),
     # chunk plans
@@ -122,6 +127,7 @@ _SIGNATURES = {
     "trex_plan_error": (c_int, [c_void_p, POINTER(ctypes.c_uint32), c_void_p]),
     "trex_plan_image_size": (c_int, [c_void_p, i64p]),
     "trex_plan_set_hold": (c_int, [c_void_p, ctypes.c_uint32]),
+    "trex_plan_set_chains": (c_int, [c_void_p, c_int]),
     "trex_plan_build_image": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int64]),
     "trex_dfa_match_plan_image": (c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, c_void_p,
                                           ctypes.c_uint32, c_void_p]),

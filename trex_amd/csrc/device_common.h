@@ -43,6 +43,43 @@ __device__ __forceinline__ void stream_body(const uint4* __restrict__ v, int64_t
     for (; i < nvec; ++i) consume(v[i * VSTRIDE]);
 }
 
+// Two streams consumed side by side: pairs (a[i], b[i]) for i < nb, then
+// a[nb] alone when `extra_a` (a holds nb or nb + 1 vectors). PF vectors of
+// each stream are in flight while the previous block is consumed, so the two
+// consumers' dependent chains interleave.
+template <int PF, int VSTRIDE = 1, class F2, class F1>
+__device__ __forceinline__ void stream_body2(const uint4* __restrict__ va, const uint4* __restrict__ vb, int64_t nb,
+                                             bool extra_a, F2&& both, F1&& only_a) {
+    int64_t i = 0;
+    if (nb >= PF) {
+        uint4 a[PF], b[PF];
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            a[k] = va[(int64_t)k * VSTRIDE];
+            b[k] = vb[(int64_t)k * VSTRIDE];
+        }
+        for (i = PF; i + PF <= nb; i += PF) {
+            uint4 a2[PF], b2[PF];
+#pragma unroll
+            for (int k = 0; k < PF; ++k) {
+                a2[k] = va[(i + k) * VSTRIDE];
+                b2[k] = vb[(i + k) * VSTRIDE];
+            }
+#pragma unroll
+            for (int k = 0; k < PF; ++k) both(a[k], b[k]);
+#pragma unroll
+            for (int k = 0; k < PF; ++k) {
+                a[k] = a2[k];
+                b[k] = b2[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PF; ++k) both(a[k], b[k]);
+    }
+    for (; i < nb; ++i) both(va[i * VSTRIDE], vb[i * VSTRIDE]);
+    if (extra_a) only_a(va[nb * VSTRIDE]);
+}
+
 // v[i] by a non-temporal load (data read once: no reuse in the caches).
 __device__ __forceinline__ uint4 load16_nt(const uint4* __restrict__ v, int64_t i) {
     typedef unsigned int v4u __attribute__((ext_vector_type(4)));

@@ -304,6 +304,75 @@ int compat_dispatch(CompatEngine& eng, const uint8_t* raw_concat, const int* off
     return TREX_OK;
 }
 
+// The prefix (long-string) dispatch: for tables of at most 16 states every
+// string is cut into chunks chained by the decoupled look-back (one launch;
+// chunk length the power of two nearest total / 65,536 within 1-64 KiB, about
+// one wave of 64 chunks per SIMD); wider tables walk each string on one lane.
+int compat_dispatch_prefix(CompatEngine& eng, const uint8_t* raw_concat, const int* offsets, int* results, int B,
+                           int total_chars, float* kernel_ms, float* total_ms) {
+    if (!eng.h) return TREX_ERR_NO_DEVICE;
+    if (eng.h->n_states > 16 || B <= 0) return compat_dispatch(eng, raw_concat, offsets, results, B, total_chars,
+                                                               kernel_ms, total_ms);
+    if (total_chars < 0) return fail(TREX_ERR_INVALID, "negative size");
+    if (B > eng.max_batch || total_chars > eng.max_total_chars)
+        return fail(TREX_ERR_CAPACITY, "batch of %d strings / %d chars exceeds init capacity %d / %d", B, total_chars,
+                    eng.max_batch, eng.max_total_chars);
+    if (!offsets || !results) return fail(TREX_ERR_INVALID, "null buffer");
+    const int64_t base = offsets[0], total = (int64_t)offsets[B] - base;
+    if (offsets[B] > total_chars || total < 0) return fail(TREX_ERR_INVALID, "offsets out of range");
+    if (total > 0 && !raw_concat) return fail(TREX_ERR_INVALID, "null bytes");
+    std::vector<int64_t> begin((size_t)B), end((size_t)B);
+    for (int i = 0; i < B; ++i) {
+        if (offsets[i + 1] < offsets[i]) return fail(TREX_ERR_INVALID, "offsets decrease at %d", i);
+        begin[i] = offsets[i] - base;
+        end[i] = offsets[i + 1] - base;
+    }
+    int64_t chunk = 1024;
+    while (chunk < 65536 && (double)total / 65536.0 >= (double)chunk * 1.41421356) chunk <<= 1;
+    trex_plan* plan = nullptr;
+    TREX_TRY(trex_plan_create(begin.data(), end.data(), B, chunk, &plan));
+    trex_dfa* h = eng.h;
+    int rc = TREX_OK;
+    {
+        std::lock_guard<std::mutex> lock(h->mu);
+        rc = h->in_bytes.reserve((size_t)total + 16);
+        if (!rc) rc = h->out_state.reserve((size_t)B);
+        if (!rc) rc = h->out_accept.reserve((size_t)B);
+        if (!rc) rc = ensure_host_dispatch(h->stream, h->ev);
+        hipStream_t s = h->stream;
+        eng.st.resize((size_t)B);
+        eng.acc.resize((size_t)B);
+        hipError_t e = hipSuccess;
+        if (!rc) e = hipEventRecord(h->ev[0], s);
+        if (!rc && e == hipSuccess && total > 0)
+            e = hipMemcpyAsync(h->in_bytes.ptr, raw_concat + base, (size_t)total, hipMemcpyHostToDevice, s);
+        if (!rc && e == hipSuccess) e = hipEventRecord(h->ev[1], s);
+        if (!rc && e != hipSuccess) rc = hip_fail(e, "prefix dispatch upload");
+        if (!rc) rc = trex_dfa_match_plan(h, plan, h->in_bytes.as<uint8_t>(), h->out_state.as<uint8_t>(),
+                                          h->out_accept.as<uint8_t>(), 0, s);
+        if (!rc) e = hipEventRecord(h->ev[2], s);
+        if (!rc && e == hipSuccess)
+            e = hipMemcpyAsync(eng.acc.data(), h->out_accept.ptr, (size_t)B, hipMemcpyDeviceToHost, s);
+        if (!rc && e == hipSuccess) e = hipEventRecord(h->ev[3], s);
+        if (!rc && e == hipSuccess) e = hipEventSynchronize(h->ev[3]);
+        if (!rc && e != hipSuccess) rc = hip_fail(e, "prefix dispatch");
+        if (!rc) {
+            float k = 0.f, t = 0.f;
+            hipEventElapsedTime(&k, h->ev[1], h->ev[2]);
+            hipEventElapsedTime(&t, h->ev[0], h->ev[3]);
+            if (kernel_ms) *kernel_ms = k;
+            if (total_ms) *total_ms = t;
+        }
+    }
+    uint32_t err = 0;
+    if (!rc) rc = trex_plan_error(plan, &err, h->stream);
+    trex_plan_destroy(plan);
+    if (rc) return rc;
+    if (err) return fail(TREX_ERR_TIMEOUT, "look-back launch reported error word %u", err);
+    for (int i = 0; i < B; ++i) results[i] = eng.acc[i];
+    return TREX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -367,7 +436,8 @@ int monoid_batch_engine_dispatch(const uint8_t* raw_concat, const int* offsets, 
 
 int monoid_batch_engine_dispatch_prefix(const uint8_t* raw_concat, const int* offsets, int* results, int B,
                                         int total_chars, float* kernel_ms, float* total_ms) {
-    return monoid_batch_engine_dispatch(raw_concat, offsets, results, B, total_chars, kernel_ms, total_ms);
+    std::lock_guard<std::mutex> lock(g_compat_mu);
+    return compat_dispatch_prefix(g_monoid, raw_concat, offsets, results, B, total_chars, kernel_ms, total_ms);
 }
 
 void monoid_batch_engine_destroy(void) {

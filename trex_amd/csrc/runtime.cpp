@@ -336,6 +336,65 @@ int trex_stream_wait_event(void* stream, void* ev) {
     return TREX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Graphs of captured launches
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct trex_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int trex_graph_begin(void* stream) {
+    if (!stream) return fail(TREX_ERR_INVALID, "graph capture needs a non-default stream");
+    TREX_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    return TREX_OK;
+}
+
+int trex_graph_end(void* stream, trex_graph** out) {
+    if (!stream || !out) return fail(TREX_ERR_INVALID, "null argument");
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    TREX_HIP(hipStreamEndCapture((hipStream_t)stream, &g), "hipStreamEndCapture");
+    trex_graph* h = new trex_graph();
+    h->graph = g;
+    const hipError_t e = hipGraphInstantiate(&h->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        trex_graph_destroy(h);
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    *out = h;
+    return TREX_OK;
+}
+
+int trex_graph_abort(void* stream) {
+    if (!stream) return TREX_OK;
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing((hipStream_t)stream, &st) == hipSuccess && st != hipStreamCaptureStatusNone) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture((hipStream_t)stream, &g);
+        if (g) hipGraphDestroy(g);
+    }
+    return TREX_OK;
+}
+
+int trex_graph_launch(trex_graph* g, void* stream) {
+    if (!g || !g->exec) return fail(TREX_ERR_INVALID, "null graph");
+    TREX_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream), "hipGraphLaunch");
+    return TREX_OK;
+}
+
+int trex_graph_destroy(trex_graph* g) {
+    if (!g) return TREX_OK;
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    delete g;
+    return TREX_OK;
+}
+
 int trex_device_info(char* name, int n, int* cu_count, int64_t* hbm_bytes) {
     int dev = 0;
     TREX_HIP(hipGetDevice(&dev), "hipGetDevice");

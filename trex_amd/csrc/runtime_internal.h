@@ -54,6 +54,10 @@ uint64_t device_wait_ticks();
 // time limit.
 constexpr uint32_t kWaitBoundUs = 100000;
 
+// Dependent chains per lane in phase 1 of the look-back scan by default: 1
+// (each chunk walked front to back) or 2 (its two halves side by side).
+constexpr int kScanChains = 1;
+
 }  // namespace trex
 
 // A resident DFA byte table (the handle behind trex_dfa_*).

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -25,6 +26,7 @@ struct trex_plan {
     std::vector<int64_t> h_tile_off;       // byte offset of each tile in the image, n_tiles + 1 entries
     DevBuf tile_off;                       // the same offsets on the device
     uint32_t hold = 0;                     // test hook: delayed publication (trex_plan_set_hold)
+    int chains = kScanChains;              // dependent chains per lane in phase 1 (trex_plan_set_chains)
     std::mutex mu;                         // one launch at a time per plan
 };
 
@@ -189,8 +191,9 @@ int match_plan(trex_dfa* h, trex_plan* p, const uint8_t* d_bytes, bool image, ui
     // persistent grid: enough waves to fill every CU, never more than the work
     const int64_t waves_needed = (n_chunks + 63) / 64;
     const int64_t blocks = std::min<int64_t>((waves_needed + 3) / 4, (int64_t)device_cu_count() * 8);
-    const char* kernel = window ? (image ? "dfa16_chunks_window_img" : "dfa16_chunks_window")
-                                : (image ? "dfa16_chunks_bytes_img" : "dfa16_chunks_bytes");
+    char kernel[48];
+    std::snprintf(kernel, sizeof(kernel), "dfa16_chunks_%s%s%s", window ? "window" : "bytes", image ? "_img" : "",
+                  p->chains == 2 ? "_c2" : "");
     return launch(kernel, (unsigned)std::max<int64_t>(blocks, 1), kBlock, 0, s, args);
 }
 
@@ -201,6 +204,14 @@ extern "C" {
 int trex_dfa_match_plan(trex_dfa* h, trex_plan* p, const uint8_t* d_bytes, uint8_t* d_final_state,
                         uint8_t* d_accept, uint32_t max_spins, void* stream) {
     return match_plan(h, p, d_bytes, false, d_final_state, d_accept, max_spins, stream);
+}
+
+int trex_plan_set_chains(trex_plan* p, int chains) {
+    if (!p) return fail(TREX_ERR_INVALID, "null plan");
+    if (chains != 1 && chains != 2) return fail(TREX_ERR_INVALID, "chains must be 1 or 2");
+    std::lock_guard<std::mutex> lock(p->mu);
+    p->chains = chains;
+    return TREX_OK;
 }
 
 int trex_plan_set_hold(trex_plan* p, uint32_t hold) {

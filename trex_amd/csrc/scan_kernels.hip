@@ -40,6 +40,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "device_common.h"
 
 #define TREX_BLOCK 256
@@ -122,6 +124,35 @@ struct Scanner {
             settle();
         }
     }
+    // A vector of this scanner and one of `o`, a second scanner over later
+    // bytes of the same chunk: when both walk in every lane, the two walks
+    // are interleaved word by word (two independent dependent chains).
+    __device__ __forceinline__ void vector2(const uint4& a, Scanner& o, const uint4& b) {
+        if (__ballot(!walking || !o.walking) == 0ull) {
+            walk_word(a.x); o.walk_word(b.x);
+            walk_word(a.y); o.walk_word(b.y);
+            walk_word(a.z); o.walk_word(b.z);
+            walk_word(a.w); o.walk_word(b.w);
+        } else {
+            vector(a);
+            o.vector(b);
+        }
+    }
+    // This scanner followed by `o` (which started composing from the identity
+    // on the bytes right after this one's): o's constant function, or o's
+    // function applied to this one's state or composed after its function.
+    __device__ __forceinline__ void join(const Scanner& o) {
+        if (o.walking) {
+            walking = true;
+            s = o.s;
+        } else if (walking) {
+            s = fn_apply(o.f, s);
+        } else {
+            f = fn_then(f, o.f);
+        }
+        w.any_hi |= o.w.any_hi;
+        w.all &= o.w.all;
+    }
 };
 
 // Always inlined into its kernels, at 8 vectors in flight (<= 106 VGPRs, no
@@ -134,7 +165,7 @@ struct Scanner {
 // ticket's 64 chunks at byte tile_off[t], vector i of lane k's chunk at
 // tile_off[t] + (64 i + k) * 16, so one wave load reads 1 KiB contiguously);
 // otherwise the bytes the plan's ranges index.
-template <bool WINDOW, bool IMAGE>
+template <bool WINDOW, bool IMAGE, int CHAINS>
 __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, const int64_t* __restrict__ tile_off,
                                            const int64_t* __restrict__ cbegin,
                                            const int64_t* __restrict__ cend, const int32_t* __restrict__ csid,
@@ -207,14 +238,38 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
         sc.f = fn_identity_on(n_states);
 
         // ---- phase 1: walk or compose this chunk's bytes ----
+        // CHAINS == 2: the body's first half goes through `sc`, its second
+        // half through `sb`, which composes from the identity (and walks once
+        // its function is constant), side by side; then sb is joined to sc.
+        Scanner<WINDOW> sb;
+        sb.t2 = t2;
+        sb.t1 = t1;
+        sb.f2 = f2;
+        sb.f1 = f1;
+        sb.w.init();
+        sb.walking = false;
+        sb.s = start;
+        sb.f = fn_identity_on(n_states);
         auto vector = [&](const uint4& a) { sc.vector(a); };
+        auto vector2 = [&](const uint4& a, const uint4& b) { sc.vector2(a, sb, b); };
+        // the body (nvec vectors, VSTRIDE apart) through one or two chains
+        auto body = [&](const uint4* v, int64_t nvec, auto stride_tag) {
+            constexpr int VS = decltype(stride_tag)::value;
+            if constexpr (CHAINS == 2) {
+                const int64_t na = (nvec + 1) >> 1, nb = nvec - na;
+                stream_body2<kScanPF / 2, VS>(v, v + na * VS, nb, na > nb, vector2, vector);
+                sc.join(sb);
+            } else {
+                stream_body<kScanPF, VS>(v, nvec, vector);
+            }
+        };
         if constexpr (IMAGE) {
             // vectors of this lane's chunk are 1 KiB apart; the tail bytes sit
             // at the start of the next vector slot
             const int64_t len = end - p;
             const uint8_t* mine = bytes + tile_off[base >> 6] + 16u * lane;
             const int64_t nvec = len >> 4;
-            stream_body<kScanPF, 64>(reinterpret_cast<const uint4*>(mine), nvec, vector);
+            body(reinterpret_cast<const uint4*>(mine), nvec, std::integral_constant<int, 64>{});
             const uint8_t* tail = mine + nvec * 1024;
             for (int64_t k = 0; k < (len & 15); ++k) sc.byte(tail[k]);
         } else {
@@ -225,7 +280,7 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
             // previous ones are walked (one lane's loads are its own chunk's
             // next bytes)
             const int64_t nvec = (end - p) >> 4;
-            stream_body<kScanPF>(reinterpret_cast<const uint4*>(bytes + p), nvec, vector);
+            body(reinterpret_cast<const uint4*>(bytes + p), nvec, std::integral_constant<int, 1>{});
             p += nvec << 4;
             for (; p < end; ++p) sc.byte(bytes[p]);
         }
@@ -354,20 +409,26 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
     bytes, tile_off, cbegin, cend, csid, cpred, cflags, n_chunks, t2g, t1g, f2g, accg, win4, start, dead,   \
         n_states, status, agg, jump, ticket, error, max_spins, max_ticks, hold, out_state, out_accept
 
-// Chunks of any string, bytes as the plan's ranges index them (tile_off unused).
-extern "C" __global__ void __launch_bounds__(TREX_BLOCK) dfa16_chunks_window(TREX_CHUNK_PARAMS) {
-    chunk_scan<true, false>(TREX_CHUNK_ARGS);
-}
-extern "C" __global__ void __launch_bounds__(TREX_BLOCK) dfa16_chunks_bytes(TREX_CHUNK_PARAMS) {
-    chunk_scan<false, false>(TREX_CHUNK_ARGS);
-}
-// The same over the plan's chunk-interleaved image.
-extern "C" __global__ void __launch_bounds__(TREX_BLOCK) dfa16_chunks_window_img(TREX_CHUNK_PARAMS) {
-    chunk_scan<true, true>(TREX_CHUNK_ARGS);
-}
-extern "C" __global__ void __launch_bounds__(TREX_BLOCK) dfa16_chunks_bytes_img(TREX_CHUNK_PARAMS) {
-    chunk_scan<false, true>(TREX_CHUNK_ARGS);
-}
+// Chunks of any string, bytes as the plan's ranges index them (tile_off
+// unused); _c2: each chunk's body walked as two chains.
+#define TREX_CHUNK_KERNELS(SUFFIX, CHAINS)                                                                   \
+    extern "C" __global__ void __launch_bounds__(TREX_BLOCK) dfa16_chunks_window##SUFFIX(TREX_CHUNK_PARAMS) { \
+        chunk_scan<true, false, CHAINS>(TREX_CHUNK_ARGS);                                                    \
+    }                                                                                                        \
+    extern "C" __global__ void __launch_bounds__(TREX_BLOCK) dfa16_chunks_bytes##SUFFIX(TREX_CHUNK_PARAMS) {  \
+        chunk_scan<false, false, CHAINS>(TREX_CHUNK_ARGS);                                                   \
+    }                                                                                                        \
+    /* the same over the plan's chunk-interleaved image */                                                  \
+    extern "C" __global__ void __launch_bounds__(TREX_BLOCK)                                                 \
+        dfa16_chunks_window_img##SUFFIX(TREX_CHUNK_PARAMS) {                                                 \
+        chunk_scan<true, true, CHAINS>(TREX_CHUNK_ARGS);                                                     \
+    }                                                                                                        \
+    extern "C" __global__ void __launch_bounds__(TREX_BLOCK)                                                 \
+        dfa16_chunks_bytes_img##SUFFIX(TREX_CHUNK_PARAMS) {                                                  \
+        chunk_scan<false, true, CHAINS>(TREX_CHUNK_ARGS);                                                    \
+    }
+TREX_CHUNK_KERNELS(, 1)
+TREX_CHUNK_KERNELS(_c2, 2)
 
 // Sets dst[0 .. n) = value (status words, tickets, error flags).
 extern "C" __global__ void __launch_bounds__(TREX_BLOCK)

@@ -147,3 +147,56 @@ class PinnedBuffer:
             self.free()
         except Exception:
             pass
+
+
+class Graph:
+    """Asynchronous launches recorded on a stream and replayed as one HIP graph
+    (``trex_graph_*``)::
+
+        with Graph.capture(stream) as g:
+            engine.match_batch_device(batch, d_final, d_accept, stream)
+        g.launch(stream)          # the same launches, same pointers
+
+    Calls that copy from host memory, synchronise or allocate cannot be
+    recorded; a failed capture is ended and dropped."""
+
+    def __init__(self):
+        self.handle = None
+
+    @classmethod
+    def capture(cls, stream: Stream) -> "_Capture":
+        return _Capture(cls(), stream)
+
+    def launch(self, stream: Stream | None = None) -> None:
+        if not self.handle:
+            raise RuntimeError("graph not captured")
+        _lib.call("trex_graph_launch", self.handle, stream.handle if stream else None)
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.lib().trex_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _Capture:
+    def __init__(self, graph: Graph, stream: Stream):
+        self.graph, self.stream = graph, stream
+
+    def __enter__(self) -> Graph:
+        _lib.call("trex_graph_begin", self.stream.handle)
+        return self.graph
+
+    def __exit__(self, exc_type, exc, tb) -> bool:
+        if exc_type is not None:
+            _lib.lib().trex_graph_abort(self.stream.handle)
+            return False
+        h = ctypes.c_void_p()
+        _lib.call("trex_graph_end", self.stream.handle, ctypes.byref(h))
+        self.graph.handle = h.value
+        return False

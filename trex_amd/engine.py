@@ -727,6 +727,11 @@ class ChunkPlan:
     def for_batch(cls, batch: "DeviceBatch", chunk_len: int | None = None) -> "ChunkPlan":
         return cls(batch.begin, batch.end, chunk_len)
 
+    def set_chains(self, chains: int) -> None:
+        """Dependent chains per lane in the scan's first phase (1 or 2: the
+        two halves of each chunk walked side by side)."""
+        _lib.call("trex_plan_set_chains", self.handle, int(chains))
+
     def set_hold(self, hold: int) -> None:
         """Test hook: later launches hold back each tile's publication,
         in reverse order within groups of 8 tiles (0 = off)."""
