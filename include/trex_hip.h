@@ -469,6 +469,31 @@ int trex_plan_error(trex_plan* plan, uint32_t* error, void* stream);
  * off; at most 2000. */
 int trex_plan_set_hold(trex_plan* plan, uint32_t hold);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): trex_plan_create + trex_dfa_match_plan in one call that
+ * allocates nothing. The string ranges are in DEVICE memory (d_begin/d_end:
+ * n_strings int64 offsets into d_bytes, which holds bytes_len bytes); the
+ * chunk table (chunks of chunk_len bytes, cut as trex_plan_create cuts them)
+ * is built on the device, and it and the look-back state live in a static
+ * device array of TREX_STATIC_SCRATCH_BYTES that each GPU the library runs on
+ * holds until the process ends. Limits, refused with TREX_ERR_CAPACITY: at
+ * most TREX_STATIC_MAX_STRINGS strings, and n_strings + ceil(bytes_len /
+ * chunk_len) at most TREX_STATIC_MAX_CHUNKS. All calls on one GPU share its
+ * array: calls that overlap on one GPU race. d_error: one device word, set by
+ * the call to 0 or to bits: 1 a look-back wait expired, 2 a chain without a
+ * first chunk, 4 a range with begin < 0, end < begin or end > bytes_len, 8
+ * more chunks than the bound (overlapping ranges); with bit 4 or 8 set no
+ * string is matched and the outputs are left unwritten. Asynchronous on
+ * `stream`. */
+#define TREX_STATIC_MAX_STRINGS 65536
+#define TREX_STATIC_MAX_CHUNKS 131072
+#define TREX_STATIC_SCRATCH_BYTES 6291968
+int trex_dfa_match_chunked_static(trex_dfa* h, const uint8_t* d_bytes, int64_t bytes_len, const int64_t* d_begin,
+                                  const int64_t* d_end, int64_t n_strings, int64_t chunk_len,
+                                  uint8_t* d_final_state, uint8_t* d_accept, uint32_t* d_error, uint32_t max_spins,
+                                  void* stream);
+
 /* Plain device copy with the engine's streaming-load kernel (16-byte aligned
  * buffers, size a multiple of 16): the reference point for HBM traffic
  * counters and the achievable-bandwidth probe. No reference counterpart. */

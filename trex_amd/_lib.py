@@ -129,6 +129,9 @@ _SIGNATURES = {
     "trex_plan_set_hold": (c_int, [c_void_p, ctypes.c_uint32]),
     "trex_plan_set_chains": (c_int, [c_void_p, c_int]),
     "trex_plan_build_image": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int64]),
+    # the allocation-free look-back (static device scratch)
+    "trex_dfa_match_chunked_static": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_int64,
+                                              c_void_p, c_void_p, c_void_p, ctypes.c_uint32, c_void_p]),
     "trex_dfa_match_plan_image": (c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, c_void_p,
                                           ctypes.c_uint32, c_void_p]),
     "trex_copy": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),

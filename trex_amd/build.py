@@ -24,6 +24,7 @@ ARCH = "gfx950"
 
 KERNEL_DIR = PKG / "kernels"
 LIB = PKG / "libtrexhip.so"
+OBJ_DIR = PKG / "_build"   # intermediate objects of the library (not shipped)
 
 # Packed FP32 instructions are kept out of every kernel.
 HIP_DEVICE_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}",
@@ -62,10 +63,25 @@ def build_kernels(jobs: int | None = None) -> list[Path]:
     return built
 
 
+def build_static_objects() -> list[Path]:
+    """HIP sources of the library itself (host code plus static device
+    arrays, ``csrc/static/*.hip``), compiled by hipcc into objects that are
+    linked into ``libtrexhip.so``."""
+    OBJ_DIR.mkdir(exist_ok=True)
+    objs = []
+    for src in sorted((CSRC / "static").glob("*.hip")):
+        obj = OBJ_DIR / (src.stem + ".o")
+        _run([ROCM / "bin" / "hipcc", "-x", "hip", "-c", "-fPIC", *HIP_DEVICE_FLAGS, "-Wall", "-Wextra",
+              "-Wno-unused-parameter", f"-I{ROOT / 'include'}", f"-I{CSRC}", "-o", obj, src])
+        objs.append(obj)
+    return objs
+
+
 def build_runtime() -> Path:
     sources = sorted(CSRC.glob("*.cpp"))
+    objs = build_static_objects()
     tmp = LIB.with_suffix(".so.tmp")
-    _run(["g++", *HOST_FLAGS, "-shared", "-o", tmp, *sources,
+    _run(["g++", *HOST_FLAGS, "-shared", "-o", tmp, *sources, *objs,
           f"-L{ROCM / 'lib'}", "-lamdhip64", f"-Wl,-rpath,{ROCM / 'lib'}", "-lpthread", "-ldl"])
     os.replace(tmp, LIB)
     return LIB

@@ -437,3 +437,109 @@ fill_u32(uint32_t* __restrict__ dst, int64_t n, uint32_t value)
     const int64_t stride = (int64_t)gridDim.x * TREX_BLOCK;
     for (int64_t i = (int64_t)blockIdx.x * TREX_BLOCK + threadIdx.x; i < n; i += stride) dst[i] = value;
 }
+
+// ---------------------------------------------------------------------------
+// Chunk table built on the device, for look-back launches whose string ranges
+// are in device memory (the allocation-free entry point): the same chunks as
+// trex_plan_create cuts (chunk_len bytes, the last one shorter, one empty
+// chunk for an empty string), in string order, so each chunk's predecessor is
+// the chunk just before it.
+// ---------------------------------------------------------------------------
+
+// error bits of the table kernels (the look-back's own: bit 0 a wait
+// expired, bit 1 a chain without a first chunk)
+constexpr uint32_t kTableBadRange = 4u;    // a range with begin < 0, end < begin or end > bytes_len
+constexpr uint32_t kTableOverBound = 8u;   // more chunks than the launch was sized for
+
+// One workgroup of 1024 threads: first[s] = index of string s's first chunk,
+// first[n_strings] = the chunk count; clears the ticket and sets the error word
+// (to 0, or to the bits above).
+extern "C" __global__ void __launch_bounds__(1024)
+chunk_table_count(const int64_t* __restrict__ begin, const int64_t* __restrict__ end, int64_t n_strings,
+                  int64_t bytes_len, int64_t chunk_len, int64_t n_bound, int64_t* __restrict__ first,
+                  uint32_t* __restrict__ ticket, uint32_t* __restrict__ error)
+{
+    __shared__ int64_t wave_sum[16];
+    __shared__ int64_t carry_s;
+    __shared__ uint32_t bad_s;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    if (t == 0) { carry_s = 0; bad_s = 0; *ticket = 0; }
+    __syncthreads();
+    for (int64_t s0 = 0; s0 < n_strings; s0 += 1024) {
+        const int64_t s = s0 + t;
+        int64_t cnt = 0;
+        if (s < n_strings) {
+            const int64_t b = begin[s], e = end[s];
+            if (b < 0 || e < b || e > bytes_len) {
+                atomicOr(&bad_s, kTableBadRange);
+                cnt = 1;
+            } else {
+                const int64_t len = e - b;
+                cnt = len <= chunk_len ? 1 : (len + chunk_len - 1) / chunk_len;
+            }
+        }
+        // inclusive scan within the wave, then across the 16 waves
+        int64_t x = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int64_t y = __shfl_up(x, d, 64);
+            if ((int)lane >= d) x += y;
+        }
+        if (lane == 63u) wave_sum[wave] = x;
+        __syncthreads();
+        int64_t before = carry_s;
+        for (uint32_t w = 0; w < wave; ++w) before += wave_sum[w];
+        if (s < n_strings) first[s] = before + x - cnt;
+        __syncthreads();   // every thread has read carry_s and wave_sum
+        if (t == 1023u) carry_s = before + x;
+        __syncthreads();
+    }
+    if (t == 0) {
+        const int64_t total = carry_s;
+        first[n_strings] = total;
+        uint32_t bits = bad_s;
+        if (total > n_bound) bits |= kTableOverBound;
+        *error = bits;
+    }
+}
+
+// Chunk slots 0 .. n_bound: the chunks of the table above, then empty chunks
+// that publish at once and write no result (the launch is sized by a bound
+// computed on the host). Also clears the look-back status. After an error of
+// the count kernel every slot is empty, so nothing is matched.
+extern "C" __global__ void __launch_bounds__(TREX_BLOCK)
+chunk_table_fill(const int64_t* __restrict__ begin, const int64_t* __restrict__ end, int64_t n_strings,
+                 int64_t chunk_len, int64_t n_bound, const int64_t* __restrict__ first,
+                 const uint32_t* __restrict__ error, int64_t* __restrict__ cbegin, int64_t* __restrict__ cend,
+                 int32_t* __restrict__ csid, int32_t* __restrict__ cpred, uint32_t* __restrict__ cflags,
+                 uint32_t* __restrict__ status)
+{
+    const bool ok = (*error & (kTableBadRange | kTableOverBound)) == 0u;
+    const int64_t total = ok ? first[n_strings] : 0;
+    const int64_t stride = (int64_t)gridDim.x * TREX_BLOCK;
+    for (int64_t c = (int64_t)blockIdx.x * TREX_BLOCK + threadIdx.x; c < n_bound; c += stride) {
+        int64_t b = 0, e = 0;
+        int32_t sid = 0, pred = -1;
+        uint32_t fl = 0;
+        if (c < total) {
+            // the last string whose first chunk is <= c (first[] strictly increases)
+            int64_t lo = 0, hi = n_strings - 1;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi + 1) >> 1;
+                if (first[mid] <= c) lo = mid; else hi = mid - 1;
+            }
+            const int64_t k = c - first[lo], cnt = first[lo + 1] - first[lo];
+            b = begin[lo] + k * chunk_len;
+            e = min(end[lo], b + chunk_len);
+            sid = (int32_t)lo;
+            pred = k == 0 ? -1 : (int32_t)(c - 1);
+            fl = (k == 0 ? 1u : 0u) | (k == cnt - 1 ? 2u : 0u);
+        }
+        cbegin[c] = b;
+        cend[c] = e;
+        csid[c] = sid;
+        cpred[c] = pred;
+        cflags[c] = fl;
+        status[c] = 0u;
+    }
+}
