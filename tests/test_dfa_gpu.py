@@ -272,6 +272,14 @@ def test_plain_dfa_workload(gpu_lib):
     strings = [bytes(data[offsets[i]:offsets[i + 1]]).decode("latin-1") for i in range(1024)]
     assert [bool(x) for x in a] == [dfa.simulate(s) for s in strings] == [dfa_simulate(dfa, s) for s in strings]
     assert [int(x) for x in f[:128]] == [dfa_final_state(dfa, s, tab.dead) for s in strings[:128]]
+    # uniform text over abcd leaves every string of this workload in the dead
+    # state; strings that walk the pattern's live states reach all of them
+    from trex_amd.workloads import live_walk_batch
+    data2, offs2 = live_walk_batch(tab, np.random.default_rng(5), 1024, 64, "abcd")
+    f2, a2 = _check(eng, tab, data2, offs2)
+    assert len(np.unique(f2)) == tab.n_states and a2.any()
+    strings2 = [bytes(data2[offs2[i]:offs2[i + 1]]).decode("latin-1") for i in range(1024)]
+    assert [bool(x) for x in a2] == [dfa_simulate(dfa, s) for s in strings2]
 
 
 @pytest.mark.gpu

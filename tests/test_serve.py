@@ -67,7 +67,7 @@ def test_gpu_served_equals_alone(gpu_lib, mix_form):
     kinds = ["mix"] * 5 + ["prob"]
     reqs = _requests(rng, len(engines), kinds, 60, 64, 512, 2.0, 300_000)
     server = Server(engines, hbm_budget=256 << 20, small_bytes=64 << 10, prob_chunk_min=1024, prob_chunk_len=128,
-                    mix_form=mix_form)
+                    prob_long_chunk=8192, prob_long_chunk_len=512, mix_form=mix_form)
     with server:
         for r in reqs:
             server.submit(r)
@@ -76,6 +76,8 @@ def test_gpu_served_equals_alone(gpu_lib, mix_form):
     assert server.ticks["small"] >= 1 and server.ticks["bulk"] >= 1
     assert any(r.lane == "prob_small" and np.any(np.diff(r.offsets) > server.prob_chunk_min) for r in reqs), \
         "no small probabilistic request exercises the parallel-in-time path"
+    assert any(r.lane == "prob_small" and np.any(np.diff(r.offsets) > server.prob_long_chunk) for r in reqs), \
+        "no small probabilistic request takes the longer chunks"
     for r in reqs:
         e = engines[r.pattern]
         if isinstance(e, ProbNfaEngine):
@@ -83,9 +85,11 @@ def test_gpu_served_equals_alone(gpu_lib, mix_form):
             # depends only on itself and the fixed chunk length)
             lengths = np.diff(r.offsets)
             chunked = (lengths > server.prob_chunk_min) if r.lane == "prob_small" else np.zeros(len(lengths), bool)
+            longer = chunked & (lengths > server.prob_long_chunk)
             ll = np.empty(len(lengths))
             lm = np.empty(len(lengths))
-            for sel, kw in ((chunked, {"chunk_len": server.prob_chunk_len}), (~chunked, {})):
+            for sel, kw in ((chunked & ~longer, {"chunk_len": server.prob_chunk_len}),
+                            (longer, {"chunk_len": server.prob_long_chunk_len}), (~chunked, {})):
                 idx = np.flatnonzero(sel)
                 if len(idx):
                     parts = [r.data[r.offsets[i]:r.offsets[i + 1]] for i in idx]

@@ -138,7 +138,11 @@ std::vector<uint16_t> chunk_fragments(const std::vector<uint16_t>& q, int n_sym)
 }
 
 // Pinned staging of one chunked call's tables; reused once the copy that
-// read it has run (its event has fired).
+// read it has run (its event has fired). kStageSlots of kStageBytes each are
+// made with the handle (a table of 64 Ki chunks fits); a larger table or more
+// calls in flight add or grow slots, doubling.
+constexpr int kStageSlots = 8;
+constexpr size_t kStageBytes = 1u << 20;
 struct StageSlot {
     uint8_t* host = nullptr;
     size_t cap = 0;
@@ -237,6 +241,17 @@ int trex_pnfa_create(const float* T, int n_symbols, const float* start, const fl
         const std::vector<uint16_t> cf = chunk_fragments(h->q16, n_symbols), cfb = chunk_fragments(h->qb16, n_symbols);
         rc = upload((void**)&h->d_cf16, cf.data(), cf.size() * 2);
         if (!rc) rc = upload((void**)&h->d_cfb16, cfb.data(), cfb.size() * 2);
+    }
+    // pinned staging of the chunked path, made here: hipHostMalloc and
+    // hipHostFree can wait for the whole device, so a server that keeps the
+    // engine resident must not pay them while other work runs
+    for (int i = 0; i < kStageSlots && !rc; ++i) {
+        h->stages.emplace_back();
+        StageSlot& st = h->stages.back();
+        hipError_t e = hipEventCreateWithFlags(&st.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&st.host, kStageBytes, hipHostMallocDefault);
+        if (e != hipSuccess) rc = hip_fail(e, "pinned staging");
+        else st.cap = kStageBytes;
     }
     if (rc) { trex_pnfa_destroy(h); return rc; }
     *out = h;
@@ -412,11 +427,12 @@ int trex_pnfa_loglik_chunked(trex_pnfa* h, const uint8_t* d_symbols, const int64
     }
     slot->used = false;
     if (slot->cap < table_bytes) {
+        const size_t want = std::max(table_bytes, 2 * slot->cap);
         if (slot->host) hipHostFree(slot->host);
         slot->host = nullptr;
         slot->cap = 0;
-        TREX_HIP(hipHostMalloc((void**)&slot->host, table_bytes, hipHostMallocDefault), "hipHostMalloc");
-        slot->cap = table_bytes;
+        TREX_HIP(hipHostMalloc((void**)&slot->host, want, hipHostMallocDefault), "hipHostMalloc");
+        slot->cap = want;
     }
     int64_t* cbeg = reinterpret_cast<int64_t*>(slot->host + o_cbeg);
     int32_t* clen = reinterpret_cast<int32_t*>(slot->host + o_clen);

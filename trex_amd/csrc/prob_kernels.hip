@@ -87,10 +87,13 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
     // One state row per sequence, updated in place: a row is read and then
     // rewritten by the one wave that owns its sequence's symbol at that step.
     __shared__ __attribute__((aligned(16))) float state[kSeqPerGroup][kRow];
-    __shared__ __attribute__((aligned(16))) float dscale[kMaxSymbols][kStates];      // column factors, high part
-    __shared__ __attribute__((aligned(16))) float dscale_lo[kMaxSymbols][kStates];   // and the FP32 rest
-    __shared__ __attribute__((aligned(16))) float dcq[kMaxSymbols][kStates];         // colsum(round T), high
-    __shared__ __attribute__((aligned(16))) float dcq_lo[kMaxSymbols][kStates];      // and rest
+    // per-symbol rows padded to kRow floats: lanes of one read that name
+    // different symbols start 16 bytes apart in the banks instead of on the
+    // same ones (a 64-float row is exactly one turn of the 64 banks)
+    __shared__ __attribute__((aligned(16))) float dscale[kMaxSymbols][kRow];      // column factors, high part
+    __shared__ __attribute__((aligned(16))) float dscale_lo[kMaxSymbols][kRow];   // and the FP32 rest
+    __shared__ __attribute__((aligned(16))) float dcq[kMaxSymbols][kRow];         // colsum(round T), high
+    __shared__ __attribute__((aligned(16))) float dcq_lo[kMaxSymbols][kRow];      // and rest
     __shared__ uint8_t syms[kSeqPerGroup][kChunk + 4];
     __shared__ int32_t estep[2][kSeqPerGroup];   // exponent taken out at a step, by step parity
     __shared__ int32_t expo[kSeqPerGroup];

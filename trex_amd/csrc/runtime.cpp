@@ -93,6 +93,16 @@ void load_module() {
         }
         g_modules.push_back(m);
     }
+    // Stream-ordered allocations (hipMallocAsync: the chunked probabilistic
+    // path's workspace, mixed-queue plan buffers) come from the device's
+    // default pool; by default it returns memory to the system at every
+    // synchronisation and maps it again on the next allocation. Keep it.
+    int dev = 0;
+    hipMemPool_t pool = nullptr;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool) {
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     g_module_status = TREX_OK;
 }
 }  // namespace

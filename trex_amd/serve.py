@@ -25,12 +25,17 @@ Boolean NFA-64 and probabilistic NFA-64 engines) and serves match requests
   placed), so a small-request tick always finds room beside it. A
   probabilistic request is a tick of its own (one ragged call, at most
   ``prob_grid`` workgroups per kernel launch, set on every probabilistic
-  engine the server holds); each probabilistic lane has ``prob_slots`` slots
-  with a stream each, so a few long sequential launches overlap one another
-  while the compute units they hold stay bounded. In small probabilistic
-  requests, sequences longer than ``prob_chunk_min`` symbols take the
-  parallel-in-time path (chunks of ``prob_chunk_len`` multiplied concurrently,
-  then chained), whose latency does not grow with the sequence length;
+  engine the server holds); the bulk probabilistic lane has ``prob_slots``
+  slots and the small one ``prob_small_slots``, with a stream each, so a few
+  long sequential launches overlap one another while the compute units they
+  hold stay bounded, and small requests do not queue behind one another. In
+  small probabilistic requests, sequences longer than ``prob_chunk_min``
+  symbols take the parallel-in-time path (chunks multiplied concurrently, then
+  chained), whose latency does not grow with the sequence length: chunks of
+  ``prob_chunk_len`` symbols, or ``prob_long_chunk_len`` for sequences longer
+  than ``prob_long_chunk`` (fewer chunks to chain);
+* ticks complete in whatever order their events fire, so a slow tick never
+  holds back the slots of faster ones;
 * no call that synchronises the whole device (``hipFree``, a synchronous
   copy) runs while serving: tick buffers are reserved up front and grow
   geometrically, and mixed-queue plans are copied asynchronously;
@@ -40,11 +45,11 @@ Boolean NFA-64 and probabilistic NFA-64 engines) and serves match requests
   and then every request of the tick gets its result and completion time, and
   its arena space is released.
 
-The server uses seven streams (a copy stream, two DFA/NFA lanes, two slots
-in each probabilistic lane); streams that share a hardware queue run their
-work in order, so a process that serves should start with
-``GPU_MAX_HW_QUEUES`` of at least 8 (the HIP runtime reads it once, before the
-first GPU call; the default stream needs one too).
+The server uses nine streams (a copy stream, two DFA/NFA lanes, two slots
+in the bulk probabilistic lane and four in the small one); streams that
+share a hardware queue run their work in order, so a process that serves
+should start with ``GPU_MAX_HW_QUEUES`` of at least 8 (the HIP runtime reads
+it once, before the first GPU call; the default stream needs one too).
 
 The reference has no serving path: one synchronous launch per request and
 pattern through a process-global engine (cuda/monoid_batch.cu:596-671).
@@ -171,10 +176,11 @@ class Server:
                  bulk_grid: int | None = None, small_budget: int | None = None, prob_slots: int = 2,
                  bulk_slots: int = 1, prob_grid: int = 32, reserve_strings: int = 1 << 18,
                  prob_chunk_min: int = 1024, prob_chunk_len: int = 256, mix_max_spins: int = 0,
-                 mix_form: str = "nibble"):
+                 mix_form: str = "nibble", prob_small_slots: int = 4, prob_long_chunk: int = 1 << 14,
+                 prob_long_chunk_len: int = 1024):
         hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4)   # HIP's default is 4
         if hwq < 8:
-            warnings.warn(f"GPU_MAX_HW_QUEUES is {hwq}: the server's seven streams share hardware queues, so "
+            warnings.warn(f"GPU_MAX_HW_QUEUES is {hwq}: the server's nine streams share hardware queues, so "
                           "small requests wait behind long launches; start the process with 8 or more",
                           RuntimeWarning, stacklevel=2)
         self.engines = list(engines)
@@ -193,7 +199,14 @@ class Server:
         self.mix_index = mix_index
         self.small_bytes = int(small_bytes)
         self.prob_chunk_min = int(prob_chunk_min)
-        self.prob_chunk_len = int(prob_chunk_len)   # fixed, so a sequence's result does not depend on its neighbours
+        # chunk lengths of the parallel-in-time path depend only on the
+        # sequence's own length, so a result does not depend on its neighbours:
+        # prob_chunk_len up to prob_long_chunk symbols, prob_long_chunk_len
+        # beyond (fewer chunks to chain, so a 1 MiB sequence stays within a
+        # few milliseconds)
+        self.prob_chunk_len = int(prob_chunk_len)
+        self.prob_long_chunk = int(prob_long_chunk)
+        self.prob_long_chunk_len = int(prob_long_chunk_len)
         self.mix_max_spins = int(mix_max_spins)     # bound of every wait in a mixed launch (0: the library's)
         self.arena = Arena(hbm_budget)
         self.small_arena = Arena(small_budget if small_budget is not None else max(64 * self.small_bytes, 256 << 20))
@@ -204,7 +217,8 @@ class Server:
                       for name in ("small", "bulk", "prob_small", "prob_bulk")}
         for name, lane in self.lanes.items():
             if name.startswith("prob"):
-                lane["slots"] = [_Slot(None, Stream()) for _ in range(prob_slots)]
+                n_slots = prob_small_slots if name == "prob_small" else prob_slots
+                lane["slots"] = [_Slot(None, Stream()) for _ in range(n_slots)]
             else:
                 stream = Stream()
                 lane["slots"] = [_Slot(mix_engines, stream) for _ in range(bulk_slots if name == "bulk" else 2)]
@@ -336,7 +350,7 @@ class Server:
     # -- lane workers ------------------------------------------------------------------
     def _worker(self, name: str) -> None:
         """One thread per lane: launch ticks while a slot is free, complete them
-        in launch order; every wait releases the GIL. A request joins a tick
+        as their events fire; every wait releases the GIL. A request joins a tick
         once its bytes are on the device: the host polls the copy's event, so
         no stream holds a wait for another (a wait blocks the hardware queue it
         sits in, which other streams may share)."""
@@ -384,10 +398,13 @@ class Server:
                 inflight.append(slot)
                 self.ticks[name] += 1
                 continue
-            if inflight and (len(inflight) == len(lane["slots"]) and not lane["queue"]):
-                inflight[0].event.synchronize()   # every slot busy, nothing else to watch
-            if inflight and inflight[0].event.query():
-                done = inflight.popleft()
+            if inflight and len(inflight) == 1 and not lane["queue"]:
+                inflight[0].event.synchronize()   # one tick running, nothing else to watch
+            # complete whichever ticks have finished, in any order: a slow tick
+            # does not hold back the slots of faster ones behind it
+            done = next((sl for sl in inflight if sl.event.query()), None)
+            if done is not None:
+                inflight.remove(done)
                 try:
                     self._complete(name, done)
                 except Exception as e:
@@ -446,9 +463,14 @@ class Server:
             if chunk_min > 0:
                 lengths = end[k0:k0 + m] - begin[k0:k0 + m]
                 n_long = int(np.count_nonzero(lengths > chunk_min))
-            if n_long:
-                eng.loglik_chunked_device(base, begin[k0:k0 + n_long], end[k0:k0 + n_long], slot.h_f64.ptr + 8 * k0,
-                                          slot.h_f64.ptr + 8 * (n + k0), self.prob_chunk_len, stream)
+            if n_long:   # longest first: those over prob_long_chunk take the longer chunks
+                lengths = end[k0:k0 + n_long] - begin[k0:k0 + n_long]
+                n_longer = int(np.count_nonzero(lengths > self.prob_long_chunk))
+                for a, b, clen in ((0, n_longer, self.prob_long_chunk_len), (n_longer, n_long, self.prob_chunk_len)):
+                    if b > a:
+                        eng.loglik_chunked_device(base, begin[k0 + a:k0 + b], end[k0 + a:k0 + b],
+                                                  slot.h_f64.ptr + 8 * (k0 + a), slot.h_f64.ptr + 8 * (n + k0 + a),
+                                                  clen, stream)
             if n_long < m:
                 k1 = k0 + n_long
                 eng.loglik_ranges_device(base, slot.d_begin.ptr + 8 * k1, d_end + 8 * k1, m - n_long,

@@ -114,3 +114,27 @@ def plain_dfa_workload(seed: int = PLAIN_DFA_SEED, n_strings: int = 1024, length
     pattern, dfa = random_dfa_pattern(rng, ALPHABET_ABCD)
     data, offsets = uniform_batch(rng, n_strings, length, ord("a"), ord("d") + 1)
     return pattern, dfa, data, offsets
+
+
+def live_walk_batch(table, rng: np.random.Generator, n_strings: int, max_len: int, alphabet: str,
+                    stray: float = 0.03):
+    """Strings that keep a DFA byte table (``DfaByteTable``) out of its dead
+    state: each byte is drawn among the alphabet bytes that lead from the
+    current state to a live one (any alphabet byte with probability
+    ``stray``, or when none is live), lengths uniform in [0, max_len]. Uniform
+    random text dies within a few bytes for most patterns; these strings reach
+    the table's other states and its accepting ones. Returns (data, offsets)."""
+    alpha = np.frombuffer(alphabet.encode("latin-1"), np.uint8)
+    out = []
+    for _ in range(n_strings):
+        s = table.start
+        chars = []
+        for _ in range(int(rng.integers(0, max_len + 1))):
+            live = [b for b in alpha if table.delta[s, b] != table.dead]
+            b = int(rng.choice(live)) if live and rng.random() >= stray else int(rng.choice(alpha))
+            chars.append(b)
+            s = int(table.delta[s, b])
+        out.append(bytes(chars))
+    offsets = np.zeros(n_strings + 1, np.int64)
+    np.cumsum([len(x) for x in out], out=offsets[1:])
+    return np.frombuffer(b"".join(out), np.uint8).copy(), offsets
