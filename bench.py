@@ -152,6 +152,8 @@ def main() -> int:
     ap.add_argument("--scan-sweep", action="store_true", help="also time the look-back scan for several chunk lengths")
     ap.add_argument("--scan-chains", type=int, default=0,
                     help="dependent chains per lane in the look-back scan's first phase (0: library default)")
+    ap.add_argument("--scan-gaps", default="",
+                    help="comma-separated tile gaps (bytes) to also time the look-back scan's chunk image with")
     ap.add_argument("--multi-sweep", action="store_true",
                     help="also time the multi-pattern pass for every table form and patterns-per-launch")
     ap.add_argument("--copy-probe", action="store_true",
@@ -340,27 +342,41 @@ def dfa_leg(args, data, offsets, batch, stream, result: dict, copy_gbs) -> int:
         from trex_amd.engine import DeviceBatch
         csr_batch = DeviceBatch(data, offsets, layout="csr")
         csr_outs = (DeviceBuffer(n), DeviceBuffer(n))
-        c_evs = [(Event(), Event()) for _ in range(N_PATTERNS * 2)]
-        csr_same = True
-        for r in range(2):
-            for i, eng in enumerate(engines):
-                a, b = c_evs[r * N_PATTERNS + i]
-                a.record(stream)
-                eng.match_batch_device(csr_batch, csr_outs[0].ptr, csr_outs[1].ptr, stream)
-                b.record(stream)
-                if r == 1:
-                    csr_same &= bool(np.array_equal(csr_outs[0].download(np.uint8, n, stream),
-                                                    outs[i][0].download(np.uint8, n, stream))
-                                     and np.array_equal(csr_outs[1].download(np.uint8, n, stream),
-                                                        outs[i][1].download(np.uint8, n, stream)))
-        stream.synchronize()
-        c_ms = float(np.mean([a.elapsed_ms(b) for a, b in c_evs[N_PATTERNS:]]))
-        c_ach = alg_bytes / (c_ms * 1e-3) / 1e9
-        csr_res = {"layout": "csr", "kernel": kernel_name(variants[0], args.prefetch, "csr"),
-                   "kernel_avg_ms": round(c_ms, 5), "achieved_gbs": round(c_ach, 1),
-                   "frac": round(c_ach / HBM_PEAK_GBS, 4), "same_results_as_interleaved": csr_same}
-        log(f"dfa on packed CSR: kernel avg {c_ms:.4f} ms, {c_ach:.0f} GB/s ({c_ach / HBM_PEAK_GBS:.1%}), "
-            f"same results: {csr_same}")
+        csr_res = {"layout": "csr"}
+        # read order of the packed strings: as stored (rotation 0), then the
+        # library's default (rotated per wave for tables that synchronise)
+        for mode, key in ((0, "unrotated"), (-1, "default")):
+            c_evs = [(Event(), Event()) for _ in range(N_PATTERNS * 2)]
+            csr_same = True
+            rotated = []
+            for eng in engines:
+                eng.set_rotation(mode)
+                rotated.append(eng.rotation()[0])
+            for r in range(2):
+                for i, eng in enumerate(engines):
+                    a, b = c_evs[r * N_PATTERNS + i]
+                    a.record(stream)
+                    eng.match_batch_device(csr_batch, csr_outs[0].ptr, csr_outs[1].ptr, stream)
+                    b.record(stream)
+                    if r == 1:
+                        csr_same &= bool(np.array_equal(csr_outs[0].download(np.uint8, n, stream),
+                                                        outs[i][0].download(np.uint8, n, stream))
+                                         and np.array_equal(csr_outs[1].download(np.uint8, n, stream),
+                                                            outs[i][1].download(np.uint8, n, stream)))
+            stream.synchronize()
+            for eng in engines:
+                eng.set_rotation(-1)
+            c_ms = float(np.mean([a.elapsed_ms(b) for a, b in c_evs[N_PATTERNS:]]))
+            c_ach = alg_bytes / (c_ms * 1e-3) / 1e9
+            kname = "dfa16_k2f_rot_pf8" if all(rotated) else kernel_name(variants[0], args.prefetch, "csr")
+            csr_res[key] = {"kernel": kname, "patterns_rotated": int(sum(rotated)),
+                            "kernel_avg_ms": round(c_ms, 5), "achieved_gbs": round(c_ach, 1),
+                            "frac": round(c_ach / HBM_PEAK_GBS, 4), "same_results_as_interleaved": csr_same}
+            log(f"dfa on packed CSR ({key} read order, {sum(rotated)} of {len(engines)} patterns rotated): "
+                f"kernel avg {c_ms:.4f} ms, {c_ach:.0f} GB/s ({c_ach / HBM_PEAK_GBS:.1%}), same results: {csr_same}")
+        csr_res.update({k: csr_res["default"][k] for k in ("kernel", "kernel_avg_ms", "achieved_gbs", "frac")})
+        csr_res["same_results_as_interleaved"] = (csr_res["default"]["same_results_as_interleaved"]
+                                                  and csr_res["unrotated"]["same_results_as_interleaved"])
         csr_outs[0].free()
         csr_outs[1].free()
         csr_batch.free()
@@ -392,6 +408,8 @@ def dfa_leg(args, data, offsets, batch, stream, result: dict, copy_gbs) -> int:
                 ok &= m_same
                 if not m_same:
                     log(f"pattern {i}: multi-pattern pass differs from the oracle")
+        if csr_res is not None:   # the CSR launches' results equal the interleaved ones checked above
+            ok &= bool(csr_res["same_results_as_interleaved"])
         verified = ok
         # pure-Python restatement of DFA.simulate on a 256-string sample (1 MiB)
         sample = [bytes(data[offsets[j]:offsets[j + 1]]).decode("latin-1") for j in range(256)]
@@ -411,6 +429,9 @@ def dfa_leg(args, data, offsets, batch, stream, result: dict, copy_gbs) -> int:
         b.free()
 
     kernel = kernel_name(variants[0], args.prefetch, args.layout)
+    if (args.layout != "interleaved" and kernel == "dfa16_k2f_pf8"
+            and all(eng.rotation()[0] for eng in engines)):
+        kernel = "dfa16_k2f_rot_pf8"
     result.update({
         "metric": "Gchars/s (DFA-16, NFA-64) + %HBM roofline; p50/p99 latency served vs plain",
         "value": round(gchars, 2),
@@ -464,7 +485,7 @@ def scan_leg(args, data, stream) -> dict:
     batch = DeviceBatch(data, offsets, layout="csr")
     d_f, d_a = DeviceBuffer(n_long), DeviceBuffer(n_long)
     if args.scan_sweep:
-        for clen in (4096, 8192, 12288, 16384, 20480, 24576, 32768, 65536):
+        for clen in (4096, 8192, 12288, 16384, 16400, 16640, 20480, 24576, 32768, 65536):
             pl = ChunkPlan.for_batch(batch, clen)
             if args.scan_chains:
                 pl.set_chains(args.scan_chains)
@@ -539,6 +560,30 @@ def scan_leg(args, data, stream) -> dict:
         log(f"look-back scan (chunk image): bit-exact={same}")
         if not same:
             log("PARITY FAILURE (look-back scan, chunk image)")
+    if args.scan_gaps:
+        # the image under other gaps after each tile (0 = tiles back to back)
+        gaps = {}
+        for gap in (int(g) for g in args.scan_gaps.split(",")):
+            plan.set_image_gap(gap)
+            d_img.free()
+            image = plan.build_image(data)
+            d_img = DeviceBuffer(image.nbytes)
+            d_img.upload(image, stream=stream)
+            stream.synchronize()
+            del image
+            gms = time_launches(lambda: eng.match_plan_image(plan, d_img.ptr, d_img.nbytes, d_f.ptr, d_a.ptr,
+                                                             stream), stream, max(3, args.steps))
+            gerr = plan.error(stream)
+            ok = None
+            if not args.no_verify:
+                ok = bool(np.array_equal(d_f.download(np.uint8, n_long), rf)
+                          and np.array_equal(d_a.download(np.uint8, n_long), ra) and gerr == 0)
+            gaps[gap] = {"kernel_avg_ms": round(gms, 4), "frac": round(alg_bytes / (gms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "error_word": int(gerr), "verified_bit_exact": ok}
+            log(f"look-back scan over the chunk image, tile gap {gap} B: {gms:.4f} ms per launch, "
+                f"{alg_bytes / (gms * 1e-3) / 1e9:.0f} GB/s, error word {gerr}, bit-exact={ok}")
+        out["image_gaps"] = gaps
+        plan.set_image_gap(256)
     # an automaton that never synchronises: every chunk keeps a permutation as
     # its function and resolves through the look-back chain
     from trex_amd.tables import DfaByteTable

@@ -148,6 +148,21 @@ int trex_dfa_set_variant(trex_dfa* h, int variant);
 /* 16-byte vectors each lane keeps in flight while streaming a string (8 or 16). */
 int trex_dfa_set_prefetch(trex_dfa* h, int vectors);
 
+/* Read order of contiguous strings (trex_dfa_match / _ranges with the
+ * two-bytes-per-lookup function table, 8 vectors in flight): with rotation,
+ * the lanes of wave w read each string's 16-byte vectors from
+ * (16 x (w mod 16)) mod n onwards, then the first ones, composing the
+ * transition function of the part read first until it turns constant. Equal
+ * strings packed back to back are then not all read at the same offset
+ * modulo 4 KiB at once, which crowds a few HBM channels. Same results.
+ * mode -1 (default): on when the table synchronises (some word over its
+ * window bytes sends every state that can occur to one state); 0 off; 1 on.
+ * No counterpart in the reference (its kernels read strings as packed,
+ * cuda/monoid_batch.cu:58-95). trex_dfa_rotation reports whether launches
+ * use it and whether the table synchronises. */
+int trex_dfa_set_rotation(trex_dfa* h, int mode);
+int trex_dfa_rotation(const trex_dfa* h, int* active, int* synchronizes);
+
 /* Device-pointer dispatch, asynchronous on `stream`: final state and accept
  * bit per string. d_offsets: int64[n_strings + 1]. */
 int trex_dfa_match(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_offsets, int64_t n_strings,
@@ -437,7 +452,9 @@ int trex_dfa_match_plan(trex_dfa* h, trex_plan* plan, const uint8_t* d_bytes, ui
  * wave load reads 1 KiB contiguously. Tile t holds the plan's chunks 64t ..
  * 64t+63 (processing order); vector i (16 bytes) of the tile's k-th chunk
  * lies at tile_off[t] + (64 i + k) * 16, a chunk's last partial vector
- * zero-padded, each tile as long as its longest chunk. The image form of
+ * zero-padded, each tile as long as its longest chunk and followed by a gap
+ * (256 bytes by default, trex_plan_set_image_gap) so that the tiles the
+ * resident waves stream at once start on different HBM channels. The image form of
  * the HBM layout for long strings, as interleaved rows are for batches of
  * similar lengths (same matching as monoid_r3_kernel, cuda/monoid_scan.cu:
  * 128-222; no counterpart of the layout in the reference).
@@ -457,6 +474,11 @@ int trex_dfa_match_plan_image(trex_dfa* h, trex_plan* plan, const uint8_t* d_ima
  * the identity function) and joins them with one composition. Same results;
  * a tuning choice. No counterpart in the reference. */
 int trex_plan_set_chains(trex_plan* plan, int chains);
+/* Bytes left after each tile of the plan's image (a multiple of 16, at most
+ * 65536; 256 by default). Changes trex_plan_image_size and the layout of
+ * images built afterwards; an image must be built under the gap it is
+ * matched with. A tuning choice with no counterpart in the reference. */
+int trex_plan_set_image_gap(trex_plan* plan, int64_t gap);
 /* Error word of the plan's last launch (0 = none; bit 0 = a wait expired),
  * read synchronously after the work queued on `stream`. */
 int trex_plan_error(trex_plan* plan, uint32_t* error, void* stream);

@@ -323,3 +323,90 @@ def test_reference_compat_monoid_engine_batch_tables(gpu_lib, regex, alpha):
             assert res.tolist() == want, entry
     finally:
         _lib.lib().monoid_batch_engine_destroy()
+
+
+def _mod15_table():
+    """sum of (byte & 15) mod 15 over the window 0x40..0x5f (a permutation on
+    every window byte: never synchronises), with a dead state for the rest."""
+    delta = np.full((16, 256), 15, np.uint8)
+    for b in range(0x40, 0x60):
+        delta[:15, b] = (np.arange(15) + (b & 15)) % 15
+    accept = np.zeros(16, np.uint8)
+    accept[0] = 1
+    return DfaByteTable(delta, accept, 0, 15, 16)
+
+
+def _rotation_tables():
+    rng = np.random.default_rng(77)
+    tabs = {f"random{i}": dfa_byte_table(random_dfa_pattern(rng, ALPHABET_32)[1]) for i in range(3)}
+    tabs["suffix_ABC"] = dfa_byte_table(compile_regex("(A|B|C|D)*ABC"))   # dead state only off the alphabet
+    tabs["mod15"] = _mod15_table()
+    return tabs
+
+
+@pytest.mark.parametrize("name", ["random0", "random1", "random2", "suffix_ABC", "mod15"])
+def test_rotated_read_order_matches_oracle(gpu_lib, name):
+    """The per-wave rotated read order (phase A composed from the rotation
+    point, phase B walked, joined) gives the oracle's results with rotation
+    off, on and automatic: 2,048 packed 4 KiB strings (every rotation class),
+    then ragged strings at unaligned offsets with stray bytes in heads,
+    bodies and tails, over synchronising automata, one whose dead state no
+    window byte reaches, and a permutation that never synchronises."""
+    tab = _rotation_tables()[name]
+    eng = DfaEngine(tab)
+    assert eng.info()["variant"] == "k2_window_f"
+    rng = np.random.default_rng(len(name))
+    alpha = np.frombuffer(b"ABCD", np.uint8) if name == "suffix_ABC" else np.arange(0x40, 0x60, dtype=np.uint8)
+    packed = alpha[rng.integers(0, len(alpha), 2048 * 4096)]
+    packed_offs = np.arange(2049, dtype=np.int64) * 4096
+    lengths = rng.integers(0, 5000, 1500)
+    lengths[:12] = [0, 1, 15, 16, 17, 255, 256, 257, 4095, 4096, 4097, 40000]
+    offs = np.zeros(len(lengths) + 1, np.int64)
+    np.cumsum(lengths, out=offs[1:])
+    ragged = np.concatenate([alpha[rng.integers(0, len(alpha), 7)], alpha[rng.integers(0, len(alpha), int(offs[-1]))]])
+    offs += 7
+    hits = rng.integers(0, len(ragged), 400)
+    ragged[hits] = rng.integers(0, 256, 400, dtype=np.uint8)
+    packed_stray = packed.copy()
+    hits = rng.integers(0, len(packed_stray), 300)
+    packed_stray[hits] = np.where(rng.random(300) < 0.5, 0x3f, 0x60).astype(np.uint8)
+    for data, o in ((packed, packed_offs), (packed_stray, packed_offs), (ragged, offs)):
+        results = []
+        for mode in (0, 1, -1):
+            eng.set_rotation(mode)
+            results.append(_check(eng, tab, data, o))
+        for f, a in results[1:]:
+            assert np.array_equal(f, results[0][0]) and np.array_equal(a, results[0][1])
+    eng.set_rotation(1)
+    assert eng.rotation()[0]
+    eng.set_rotation(0)
+    assert not eng.rotation()[0]
+    eng.set_variant("k2_window")   # the rotated order is a k2f form only
+    eng.set_rotation(1)
+    _check(eng, tab, ragged, offs)
+
+
+def test_rotation_auto_follows_synchronisation(gpu_lib):
+    """Automatic rotation is on exactly for tables that synchronise, and the
+    library's synchronisation test agrees with the host's."""
+    from trex_amd import _lib
+    from trex_amd.engine import synchronizes
+    tabs = _rotation_tables()
+    rng = np.random.default_rng(5)
+    for i in range(12):
+        tabs[f"extra{i}"] = dfa_byte_table(random_dfa_pattern(rng, ALPHABET_32)[1])
+    seen = set()
+    for name, tab in tabs.items():
+        eng = DfaEngine(tab)
+        if eng.info()["window_base"] < 0:
+            continue
+        active, sync = eng.rotation()
+        want = synchronizes(tab.delta, tab.n_states, tab.dead)
+        assert sync == want, name
+        assert active == sync, name
+        seen.add(sync)
+        eng.set_rotation(0)
+        assert eng.rotation() == (False, sync)
+        with pytest.raises(_lib.TrexError):
+            eng.set_rotation(2)
+    assert seen == {True, False}

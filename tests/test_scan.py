@@ -250,7 +250,7 @@ def test_plan_byte_kernel_two_windows(gpu_lib, chunk_len):
     assert np.array_equal(f, rf) and np.array_equal(a, ra)
 
 
-def _run_plan_image(tab, data, offsets, chunk_len, chains=None):
+def _run_plan_image(tab, data, offsets, chunk_len, chains=None, gap=None):
     """Match through the plan's chunk-interleaved image; also returns the
     plain (CSR) results of the same plan."""
     from trex_amd.device import DeviceBuffer
@@ -260,8 +260,12 @@ def _run_plan_image(tab, data, offsets, chunk_len, chains=None):
     plan = ChunkPlan.for_batch(batch, chunk_len)
     if chains is not None:
         plan.set_chains(chains)
+    if gap is not None:
+        plan.set_image_gap(gap)
     image = plan.build_image(data)
-    assert plan.image_size() >= len(data) and plan.image_size() % 1024 == 0
+    n_tiles = -(-plan.n_chunks // 64)
+    g = 256 if gap is None else gap
+    assert plan.image_size() >= len(data) + n_tiles * g and (plan.image_size() - n_tiles * g) % 1024 == 0
     d_img = DeviceBuffer(image.nbytes)
     d_img.upload(image)
     n = len(offsets) - 1
@@ -278,13 +282,14 @@ def _run_plan_image(tab, data, offsets, chunk_len, chains=None):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("chains", [1, 2])
+@pytest.mark.parametrize("chains,gap", [(1, None), (2, None), (1, 0), (1, 4112)])
 @pytest.mark.parametrize("pattern,chunk_len", [("random32", 1024), ("random32", 8192), (EVEN_A, 64),
                                                (EVEN_A, 4096), ("(a|B)*aBB", 512)])
-def test_plan_image_matches_oracle(gpu_lib, pattern, chunk_len, chains):
+def test_plan_image_matches_oracle(gpu_lib, pattern, chunk_len, chains, gap):
     """The chunk-interleaved image gives the sequential walk's results (and
     the CSR launch's) over ragged lengths, partial vectors, empty strings,
-    synchronising and permutation automata and the byte kernel."""
+    synchronising and permutation automata and the byte kernel, with one or
+    two chains per lane and with the default, no and an odd tile gap."""
     rng = np.random.default_rng(chunk_len + len(pattern))
     if pattern == "random32":
         _, dfa = random_dfa_pattern(rng, ALPHABET_32)
@@ -301,33 +306,42 @@ def test_plan_image_matches_oracle(gpu_lib, pattern, chunk_len, chains):
     np.cumsum(lengths, out=offs[1:])
     data = np.frombuffer(alphabet, np.uint8)[rng.integers(0, len(alphabet), int(offs[-1]))].copy()
     data[rng.integers(0, len(data), 8)] = 0x7e
-    fi, ai, fc, ac = _run_plan_image(tab, data, offs, chunk_len, chains)
+    fi, ai, fc, ac = _run_plan_image(tab, data, offs, chunk_len, chains, gap)
     rf, ra = ocpu.dfa_walk(tab.delta, tab.accept, tab.start, data, offs)
     assert np.array_equal(fi, rf) and np.array_equal(ai, ra)
     assert np.array_equal(fc, rf) and np.array_equal(ac, ra)
 
 
 @pytest.mark.gpu
-def test_plan_image_size_equal_lengths(gpu_lib):
+@pytest.mark.parametrize("gap", [0, 256, 4112])
+def test_plan_image_size_equal_lengths(gpu_lib, gap):
     """Strings of one length that is a multiple of the chunk length: the image
-    is exactly the bytes (no padding)."""
+    is exactly the bytes plus one gap per tile (no padding)."""
+    from trex_amd import _lib
     from trex_amd.engine import ChunkPlan
     L, C = 1 << 20, 8192
     begin = np.arange(4, dtype=np.int64) * L
     plan = ChunkPlan(begin, begin + L, C)
-    assert plan.n_chunks == 512 and plan.image_size() == 4 << 20
+    if gap != 256:
+        assert plan.image_size() == (4 << 20) + 8 * 256   # the default gap
+        plan.set_image_gap(gap)
+    assert plan.n_chunks == 512 and plan.image_size() == (4 << 20) + 8 * gap
     # the layout itself: chunks of equal length run first chunks first, then
     # the others in string order; vector i of tile t's k-th chunk sits at
-    # t * C * 64 + (64 i + k) * 16
+    # t * (C * 64 + gap) + (64 i + k) * 16, and the gaps stay zero
     data = np.random.default_rng(0).integers(0, 256, 4 * L, dtype=np.uint8)
     image = plan.build_image(data)
+    tiles = image[:plan.image_size()].reshape(8, C * 64 + gap)
+    assert not tiles[:, C * 64:].any()
     starts = [s * L for s in range(4)] + [s * L + j * C for s in range(4) for j in range(1, L // C)]
-    rows = image.reshape(512 // 64, C // 16, 64, 16).transpose(0, 2, 1, 3).reshape(512, C)
+    rows = tiles[:, :C * 64].reshape(8, C // 16, 64, 16).transpose(0, 2, 1, 3).reshape(512, C)
     want = np.stack([data[a:a + C] for a in starts])
     assert np.array_equal(rows, want)
-    from trex_amd import _lib
     with pytest.raises(_lib.TrexError):
         plan.build_image(data[:-1])          # a chunk would lie outside the source
+    for bad in (-16, 8, 65552):
+        with pytest.raises(_lib.TrexError, match="image gap"):
+            plan.set_image_gap(bad)
 
 
 @pytest.mark.gpu

@@ -60,6 +60,8 @@ _SIGNATURES = {
     "trex_dfa_info": (c_int, [c_void_p, intp, intp, intp]),
     "trex_dfa_set_variant": (c_int, [c_void_p, c_int]),
     "trex_dfa_set_prefetch": (c_int, [c_void_p, c_int]),
+    "trex_dfa_set_rotation": (c_int, [c_void_p, c_int]),
+    "trex_dfa_rotation": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int)]),
     "trex_dfa_match_ranges": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
                                       c_void_p]),
     "trex_dfa_multi_create": (c_int, [c_void_p, c_int, vpp]),
@@ -128,6 +130,7 @@ _SIGNATURES = {
     "trex_plan_image_size": (c_int, [c_void_p, i64p]),
     "trex_plan_set_hold": (c_int, [c_void_p, ctypes.c_uint32]),
     "trex_plan_set_chains": (c_int, [c_void_p, c_int]),
+    "trex_plan_set_image_gap": (c_int, [c_void_p, c_int64]),
     "trex_plan_build_image": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int64]),
     # the allocation-free look-back (static device scratch)
     "trex_dfa_match_chunked_static": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_int64,

@@ -44,6 +44,58 @@ int find_window(const uint8_t* delta, int n, int dead) {
     return window < 0 ? 0 : window;
 }
 
+// States a window walk can be in, and whether they synchronise: the states
+// other than `dead`, plus `dead` when a window byte sends another state to
+// it. Bytes of the window that send every such state to `dead` are left out
+// (they synchronise anything but do not occur in matched text). Sets
+// h->synchronizes, and h->rot_init: the identity on those states, every
+// other slot (absent states, and `dead` when no window byte reaches it and
+// it is not the start) started at the start state, so that the rotated
+// kernel's composed function turns constant once those states merge.
+void analyse_window(trex_dfa* h, const uint8_t* delta) {
+    const int n = h->n_states, dead = h->dead, base = h->window_base;
+    std::vector<int> live_bytes;
+    for (int b = base; b < base + 32; ++b) {
+        bool kills = true;
+        for (int s = 0; s < n && kills; ++s) kills = s == dead || delta[s * 256 + b] == dead;
+        if (!kills) live_bytes.push_back(b);
+    }
+    // a byte that kills every state makes the composed function constant
+    // (dead) whatever its slots hold, so only live bytes decide whether the
+    // dead state's slot is needed
+    bool dead_reachable = false;
+    for (int b : live_bytes)
+        for (int s = 0; s < n; ++s) dead_reachable = dead_reachable || (s != dead && delta[s * 256 + b] == dead);
+    const bool dead_absent = dead >= 0 && !dead_reachable && h->start != dead;
+    std::vector<int> states;
+    for (int s = 0; s < n; ++s)
+        if (!(s == dead && dead_absent)) states.push_back(s);
+    uint8_t init[16];
+    for (int s = 0; s < 16; ++s) init[s] = (uint8_t)h->start;
+    for (int s : states) init[s] = (uint8_t)s;
+    memcpy(h->rot_init, init, 16);
+    // pairs that some word sends to one state: backward closure from the diagonal
+    bool good[16][16] = {};
+    for (int s = 0; s < 16; ++s) good[s][s] = true;
+    for (bool changed = true; changed;) {
+        changed = false;
+        for (int p : states)
+            for (int q : states) {
+                if (good[p][q]) continue;
+                for (int b : live_bytes)
+                    if (good[delta[p * 256 + b]][delta[q * 256 + b]]) {
+                        good[p][q] = good[q][p] = true;
+                        changed = true;
+                        break;
+                    }
+            }
+    }
+    bool all = true;
+    for (int p : states)
+        for (int q : states) all = all && good[p][q];
+    h->synchronizes = all || live_bytes.empty();
+}
+
 bool variant_allowed(const trex_dfa* h, int v) {
     switch (v) {
         case TREX_DFA_K1_GENERIC: return true;
@@ -100,6 +152,7 @@ int trex_dfa_create(const uint8_t* delta, const uint8_t* accept, int n_states, i
         if (!rc) rc = upload((void**)&h->d_f2, f2.data(), f2.size() * sizeof(uint64_t));
     }
     if (rc) { trex_dfa_destroy(h); return rc; }
+    if (h->window_base >= 0) analyse_window(h, delta);
     if (h->window_base >= 0) h->auto_variant = TREX_DFA_K2_WINDOW_F;
     else if (n_states <= 16) h->auto_variant = TREX_DFA_K1_16;
     else h->auto_variant = TREX_DFA_K1_GENERIC;
@@ -141,6 +194,21 @@ int trex_dfa_set_variant(trex_dfa* h, int variant) {
     return TREX_OK;
 }
 
+int trex_dfa_set_rotation(trex_dfa* h, int mode) {
+    if (!h) return fail(TREX_ERR_INVALID, "null handle");
+    if (mode < -1 || mode > 1) return fail(TREX_ERR_INVALID, "rotation mode %d: -1 (auto), 0 or 1", mode);
+    h->rotation = mode;
+    return TREX_OK;
+}
+
+int trex_dfa_rotation(const trex_dfa* h, int* active, int* synchronizes) {
+    if (!h) return fail(TREX_ERR_INVALID, "null handle");
+    const bool usable = h->window_base >= 0 && h->n_states <= 16;
+    if (active) *active = usable && (h->rotation == 1 || (h->rotation == -1 && h->synchronizes)) ? 1 : 0;
+    if (synchronizes) *synchronizes = h->synchronizes ? 1 : 0;
+    return TREX_OK;
+}
+
 int trex_dfa_set_prefetch(trex_dfa* h, int vectors) {
     if (!h) return fail(TREX_ERR_INVALID, "null handle");
     if (vectors != 8 && vectors != 16) return fail(TREX_ERR_INVALID, "prefetch must be 8 or 16 vectors");
@@ -171,6 +239,15 @@ int dfa_dispatch(trex_dfa* h, const uint8_t* d_bytes, const int64_t* d_begin, co
         uint32_t dead = (uint32_t)h->dead;
         uint8_t* table = h->variant == TREX_DFA_K2_WINDOW ? h->d_t2 : h->d_f2;
         const char* form = h->variant == TREX_DFA_K2_WINDOW ? "k2w" : h->variant == TREX_DFA_K2_WINDOW_F ? "k2f" : "k2fr";
+        int rot = 0;
+        trex_dfa_rotation(h, &rot, nullptr);
+        if (!il && rot && h->variant == TREX_DFA_K2_WINDOW_F && h->prefetch == 8) {
+            // contiguous strings, read order rotated per wave (spreads HBM channels)
+            void* args[] = {&d_bytes, &d_begin, &d_end, &n_strings, &h->d_f2, &h->d_t2, &h->d_t1, &h->d_acc,
+                            &win4, &start, &dead, &h->rot_init[0], &h->rot_init[1], &h->rot_init[2],
+                            &h->rot_init[3], &d_final_state, &d_accept};
+            return launch("dfa16_k2f_rot_pf8", grid, kBlock, 0, s, args);
+        }
         std::snprintf(name, sizeof(name), "dfa16_%s%s_pf%d", form, il_tag, h->prefetch == 16 ? 16 : 8);
         void* args[] = {&d_bytes, &d_begin, &d_end, &n_strings, &table, &h->d_t1, &h->d_acc,
                         &win4, &start, &dead, &d_final_state, &d_accept};

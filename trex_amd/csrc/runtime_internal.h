@@ -74,6 +74,9 @@ struct trex_dfa {
     uint8_t* d_t2 = nullptr;  // [1024][16] pair table t2[pair * 16 + s] (window variants only)
     uint8_t* d_f2 = nullptr;  // [1024] uint64 pair functions, nibble s = successor of s
     int prefetch = 8;         // 16-byte vectors in flight per lane (8 or 16)
+    int rotation = -1;        // rotated read order of contiguous strings: -1 auto, 0 off, 1 on
+    bool synchronizes = false;  // some window word sends every state that can occur to one state
+    uint32_t rot_init[4] = {0, 0, 0, 0};  // starting function of the rotated kernel's composition
     // staging for trex_dfa_match_host
     trex::DevBuf in_bytes, in_offsets, out_state, out_accept;
     hipStream_t stream = nullptr;

@@ -14,6 +14,11 @@
 
 using namespace trex;
 
+namespace {
+// Bytes left after each tile of a plan's chunk-interleaved image (default).
+constexpr int64_t kImageTileGap = 256;
+}  // namespace
+
 struct trex_plan {
     int64_t n_strings = 0;
     int64_t n_chunks = 0;
@@ -25,10 +30,35 @@ struct trex_plan {
     std::vector<int64_t> h_begin, h_end;   // chunk ranges in processing order (host copy)
     std::vector<int64_t> h_tile_off;       // byte offset of each tile in the image, n_tiles + 1 entries
     DevBuf tile_off;                       // the same offsets on the device
+    int64_t image_gap = kImageTileGap;     // bytes left after each tile (trex_plan_set_image_gap)
     uint32_t hold = 0;                     // test hook: delayed publication (trex_plan_set_hold)
     int chains = kScanChains;              // dependent chains per lane in phase 1 (trex_plan_set_chains)
     std::mutex mu;                         // one launch at a time per plan
 };
+
+namespace {
+
+// Image tiles: 64 chunks each, as many 16-byte vectors per lane as the
+// tile's longest chunk needs (chunks are sorted longest first, so a tile pads
+// little), each followed by image_gap bytes. Every wave streams its own tile
+// front to back; with tiles a power of two apart the resident waves would all
+// read the same offset modulo that power at once and crowd a few HBM
+// channels, which the gap staggers (as the gap between interleaved groups of
+// the batch layout does).
+int layout_tiles(trex_plan* p) {
+    const int64_t n = p->n_chunks, n_tiles = (n + 63) / 64;
+    p->h_tile_off.assign((size_t)n_tiles + 1, 0);
+    for (int64_t t = 0; t < n_tiles; ++t) {
+        int64_t longest = 0;
+        for (int64_t k = t * 64; k < std::min(n, t * 64 + 64); ++k)
+            longest = std::max(longest, p->h_end[k] - p->h_begin[k]);
+        p->h_tile_off[t + 1] = p->h_tile_off[t] + (longest + 15) / 16 * 16 * 64 + p->image_gap;
+    }
+    hipError_t e = hipMemcpy(p->tile_off.ptr, p->h_tile_off.data(), p->h_tile_off.size() * 8, hipMemcpyHostToDevice);
+    return e == hipSuccess ? TREX_OK : hip_fail(e, "plan upload");
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -97,24 +127,10 @@ int trex_plan_create(const int64_t* begin, const int64_t* end, int64_t n_strings
     if (!rc) rc = p->agg.reserve(m * 8);
     if (!rc) rc = p->jump.reserve(m * 4);
     if (!rc) rc = p->control.reserve(16);
-    // image tiles: 64 chunks each, as many 16-byte vectors per lane as the
-    // tile's longest chunk needs (chunks are sorted longest first, so a tile
-    // pads little)
-    const int64_t n_tiles = (n + 63) / 64;
-    p->h_tile_off.assign((size_t)n_tiles + 1, 0);
-    for (int64_t t = 0; t < n_tiles; ++t) {
-        int64_t longest = 0;
-        for (int64_t k = t * 64; k < std::min(n, t * 64 + 64); ++k) longest = std::max(longest, oe[k] - ob[k]);
-        p->h_tile_off[t + 1] = p->h_tile_off[t] + (longest + 15) / 16 * 16 * 64;
-    }
-    if (!rc) rc = p->tile_off.reserve(p->h_tile_off.size() * 8);
-    if (!rc) {
-        hipError_t e = hipMemcpy(p->tile_off.ptr, p->h_tile_off.data(), p->h_tile_off.size() * 8,
-                                 hipMemcpyHostToDevice);
-        if (e != hipSuccess) rc = hip_fail(e, "plan upload");
-    }
     p->h_begin = ob;
     p->h_end = oe;
+    if (!rc) rc = p->tile_off.reserve(((n + 63) / 64 + 1) * 8);
+    if (!rc) rc = layout_tiles(p);
     if (!rc && n > 0) {
         hipError_t e = hipMemcpy(p->begin.ptr, ob.data(), (size_t)n * 8, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(p->end.ptr, oe.data(), (size_t)n * 8, hipMemcpyHostToDevice);
@@ -212,6 +228,15 @@ int trex_plan_set_chains(trex_plan* p, int chains) {
     std::lock_guard<std::mutex> lock(p->mu);
     p->chains = chains;
     return TREX_OK;
+}
+
+int trex_plan_set_image_gap(trex_plan* p, int64_t gap) {
+    if (!p) return fail(TREX_ERR_INVALID, "null plan");
+    if (gap < 0 || gap > 65536 || gap % 16 != 0)
+        return fail(TREX_ERR_INVALID, "image gap %lld: a multiple of 16 in [0, 65536]", (long long)gap);
+    std::lock_guard<std::mutex> lock(p->mu);
+    p->image_gap = gap;
+    return layout_tiles(p);
 }
 
 int trex_plan_set_hold(trex_plan* p, uint32_t hold) {

@@ -98,6 +98,18 @@ class DfaEngine:
     def set_prefetch(self, vectors: int) -> None:
         _lib.call("trex_dfa_set_prefetch", self.handle, int(vectors))
 
+    def set_rotation(self, mode: int) -> None:
+        """Read order of contiguous strings: -1 (default) rotated per wave when
+        the table synchronises, 0 never, 1 always (``trex_dfa_set_rotation``)."""
+        _lib.call("trex_dfa_set_rotation", self.handle, int(mode))
+
+    def rotation(self) -> tuple[bool, bool]:
+        """(launches over contiguous strings use the rotated read order,
+        the table synchronises)."""
+        a, sy = ctypes.c_int(), ctypes.c_int()
+        _lib.call("trex_dfa_rotation", self.handle, ctypes.byref(a), ctypes.byref(sy))
+        return bool(a.value), bool(sy.value)
+
     # -- matching ------------------------------------------------------------
     def match_device(self, d_bytes: int, d_offsets: int, n_strings: int, d_final: int, d_accept: int,
                      stream: Stream | None = None) -> None:
@@ -732,6 +744,11 @@ class ChunkPlan:
         two halves of each chunk walked side by side)."""
         _lib.call("trex_plan_set_chains", self.handle, int(chains))
 
+    def set_image_gap(self, gap: int) -> None:
+        """Bytes after each tile of the chunk-interleaved image (a multiple
+        of 16; 256 by default). Build the image again after changing it."""
+        _lib.call("trex_plan_set_image_gap", self.handle, int(gap))
+
     def set_hold(self, hold: int) -> None:
         """Test hook: later launches hold back each tile's publication,
         in reverse order within groups of 8 tiles (0 = off)."""
@@ -746,7 +763,8 @@ class ChunkPlan:
     def build_image(self, data: np.ndarray) -> np.ndarray:
         """The plan's chunk-interleaved image of ``data`` (the host bytes its
         ranges index): tile t = chunks 64t .. 64t+63 in processing order,
-        vector i of the tile's k-th chunk at tile_off[t] + (64 i + k) * 16."""
+        vector i of the tile's k-th chunk at tile_off[t] + (64 i + k) * 16,
+        each tile followed by the plan's image gap."""
         data = np.ascontiguousarray(data, dtype=np.uint8)
         out = np.empty(max(self.image_size(), 16), np.uint8)
         _lib.call("trex_plan_build_image", self.handle, data.ctypes.data, data.nbytes, out.ctypes.data, out.nbytes)
