@@ -43,6 +43,31 @@ __device__ __forceinline__ void stream_body(const uint4* __restrict__ v, int64_t
     for (; i < nvec; ++i) consume(v[i * VSTRIDE]);
 }
 
+// As stream_body, but whole blocks of PF vectors go to `block` (given the
+// array) and only the last nvec % PF vectors go to `one`: a consumer that
+// tests a wave-wide condition once per block (instead of once per vector)
+// keeps each block's code straight-line, so the compiler can issue all of a
+// block's table reads ahead of the dependent chain that uses them.
+template <int PF, int VSTRIDE = 1, class FB, class F1>
+__device__ __forceinline__ void stream_blocks(const uint4* __restrict__ v, int64_t nvec, FB&& block, F1&& one) {
+    int64_t i = 0;
+    if (nvec >= PF) {
+        uint4 a[PF];
+#pragma unroll
+        for (int k = 0; k < PF; ++k) a[k] = v[(int64_t)k * VSTRIDE];
+        for (i = PF; i + PF <= nvec; i += PF) {
+            uint4 b[PF];
+#pragma unroll
+            for (int k = 0; k < PF; ++k) b[k] = v[(i + k) * VSTRIDE];
+            block(a);
+#pragma unroll
+            for (int k = 0; k < PF; ++k) a[k] = b[k];
+        }
+        block(a);
+    }
+    for (; i < nvec; ++i) one(v[i * VSTRIDE]);
+}
+
 // Two streams consumed side by side: pairs (a[i], b[i]) for i < nb, then
 // a[nb] alone when `extra_a` (a holds nb or nb + 1 vectors). PF vectors of
 // each stream are in flight while the previous block is consumed, so the two
@@ -138,6 +163,47 @@ struct WinCheck {
 // Pair index sym0 | sym1 << 5 of the low / high two bytes of a word (sym = byte & 31).
 __device__ __forceinline__ uint32_t pair_lo(uint32_t x) { return (x & 31u) | (((x >> 8) & 31u) << 5); }
 __device__ __forceinline__ uint32_t pair_hi(uint32_t x) { return ((x >> 16) & 31u) | (((x >> 24) & 31u) << 5); }
+
+// The eight pair functions (nibble s = successor of s) of a 16-byte vector:
+// table reads only, independent of the state.
+__device__ __forceinline__ void pair_fns(const uint64_t* __restrict__ f2, const uint4& x, uint64_t (&e)[8]) {
+    e[0] = f2[pair_lo(x.x)]; e[1] = f2[pair_hi(x.x)];
+    e[2] = f2[pair_lo(x.y)]; e[3] = f2[pair_hi(x.y)];
+    e[4] = f2[pair_lo(x.z)]; e[5] = f2[pair_hi(x.z)];
+    e[6] = f2[pair_lo(x.w)]; e[7] = f2[pair_hi(x.w)];
+}
+
+// PF vectors walked from state s through the pair functions, with the next
+// vector's eight table reads issued before the current vector's dependent
+// chain (a shift and a mask per pair), so LDS latency overlaps the chain even
+// with one wave per SIMD (the compiler's own schedule issued four reads and
+// waited on each in turn). Also records the window check of every word.
+template <int PF, class WC>
+__device__ __forceinline__ uint32_t walk_pairs(const uint64_t* __restrict__ f2, const uint4 (&a)[PF], uint32_t s,
+                                               WC& w) {
+    uint64_t e[8];
+    pair_fns(f2, a[0], e);
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+        uint64_t n[8];
+        if (k + 1 < PF) {
+            // the next vector's reads go out once the previous vector's chain
+            // is done (an empty asm ties the words to the state): one vector
+            // ahead, not the whole block (which took ~220 VGPRs)
+            uint4 x = a[k + 1];
+            asm volatile("" : "+v"(x.x), "+v"(x.y), "+v"(x.z), "+v"(x.w) : "v"(s));
+            pair_fns(f2, x, n);
+        }
+        w.add(a[k].x); w.add(a[k].y); w.add(a[k].z); w.add(a[k].w);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s = (uint32_t)(e[j] >> (s << 2)) & 15u;
+        if (k + 1 < PF) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) e[j] = n[j];
+        }
+    }
+    return s;
+}
 
 // ---------------------------------------------------------------------------
 // 16-state functions
@@ -237,6 +303,12 @@ __device__ __forceinline__ uint32_t load_relaxed(const uint32_t* p) {
 __device__ __forceinline__ void fence_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 __device__ __forceinline__ void store_release(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+// For a status word that carries everything its readers use (a prefix state):
+// no release, so no write-back of the whole L2 ahead of the store (which an
+// agent-scope release costs on every publishing wave).
+__device__ __forceinline__ void store_relaxed(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // A wait bounded twice: by a count of unsuccessful polls (max_spins, 0 = no

@@ -281,26 +281,35 @@ __device__ __forceinline__ void rot_kernel(const uint8_t* __restrict__ bytes, co
     bool walking = false;
     uint32_t sa = 0;
     Fn16 f = init;
-    stream_body<PF>(v + r, nvec - r, [&](uint4 x) {
-        if (__ballot(!walking) == 0ull) {
-            sa = window_vec<StepK2F>(f2, sa, x, w);
-        } else {
-            const uint32_t words[4] = {x.x, x.y, x.z, x.w};
+    auto mixed = [&](const uint4& x) {   // some lane of the wave still composes
+        const uint32_t words[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                w.add(words[k]);
-                if (walking) {
-                    sa = StepK2F::word(f2, sa, words[k]);
-                } else {
-                    f = fn_then(f, load_row(t2, pair_lo(words[k])));
-                    f = fn_then(f, load_row(t2, pair_hi(words[k])));
-                }
-            }
-            if (!walking && fn_is_const(f)) {
-                walking = true;
-                sa = f.d[0] & 0xffu;
+        for (int k = 0; k < 4; ++k) {
+            w.add(words[k]);
+            if (walking) {
+                sa = StepK2F::word(f2, sa, words[k]);
+            } else {
+                f = fn_then(f, load_row(t2, pair_lo(words[k])));
+                f = fn_then(f, load_row(t2, pair_hi(words[k])));
             }
         }
+        if (!walking && fn_is_const(f)) {
+            walking = true;
+            sa = f.d[0] & 0xffu;
+        }
+    };
+    // the wave-wide mode test once per block of PF vectors: a walking wave
+    // runs each block straight-line, as the unrotated kernel does
+    stream_blocks<PF>(v + r, nvec - r, [&](const uint4 (&a)[PF]) {
+        if (__ballot(!walking) == 0ull) {
+            sa = walk_pairs<PF>(reinterpret_cast<const uint64_t*>(f2), a, sa, w);
+        } else {
+#pragma unroll
+            for (int k = 0; k < PF; ++k) mixed(a[k]);
+        }
+    }, [&](const uint4& x) {
+        if (__ballot(!walking) == 0ull) sa = window_vec<StepK2F>(f2, sa, x, w);
+        else mixed(x);
     });
     // phase B: vectors 0 .. r-1 from the head's state
     stream_body<PF>(v, r, [&](uint4 x) { s = window_vec<StepK2F>(f2, s, x, w); });

@@ -243,8 +243,12 @@ __device__ __forceinline__ void dfa_chunk(const MixArgs& A, const MixPattern& P,
     const bool known = fn_is_const(g);
     const uint32_t kstate = g.d[0] & 0xffu;
     if (active) {
-        if (!known) A.agg[item] = fn_pack(own);
-        store_release(&A.status[item], known ? (kStatusPrefix | (kstate << 8)) : kStatusAggregate);
+        if (known) {
+            store_relaxed(&A.status[item], kStatusPrefix | (kstate << 8));   // the word says it all
+        } else {
+            A.agg[item] = fn_pack(own);
+            store_release(&A.status[item], kStatusAggregate);   // agg first
+        }
     }
     uint32_t incoming = 0;
     if (active && !known && root == lane) {
@@ -262,11 +266,11 @@ __device__ __forceinline__ void dfa_chunk(const MixArgs& A, const MixPattern& P,
                 __builtin_amdgcn_s_sleep(2);
                 continue;
             }
-            fence_acquire();   // agg[j] was written before status[j] was released
-            if ((st & 0xffu) == kStatusPrefix) {
+            if ((st & 0xffu) == kStatusPrefix) {   // the word holds the state
                 incoming = nib(acc_fn, (st >> 8) & 15u);
                 break;
             }
+            fence_acquire();   // agg[j] was written before status[j] was released
             acc_fn = nib_then(A.agg[j], acc_fn);   // the earlier chunk applies first
             j = (A.iflags[j] & 1u) ? -1 : A.ia[j];
             if (j < 0) {   // cannot happen: a string's first chunk publishes a prefix
@@ -278,7 +282,7 @@ __device__ __forceinline__ void dfa_chunk(const MixArgs& A, const MixPattern& P,
     }
     incoming = __shfl(incoming, (int)root, 64);
     const uint32_t state = known ? kstate : fn_apply(g, incoming);
-    if (active && !known) store_release(&A.status[item], kStatusPrefix | (state << 8));
+    if (active && !known) store_relaxed(&A.status[item], kStatusPrefix | (state << 8));
     if (active && A.ib[item] >= 0) {
         A.out_mask[A.ib[item]] = state;
         A.out_accept[A.ib[item]] = L.acc[state];

@@ -23,11 +23,12 @@
 //    PREFIX is found; the chunk's prefix state is F(acc(p)). Waiting is only
 //    for predecessors' phase-1 results, which never wait on anything.
 //
-// Forward progress: chunks are claimed 64 at a time (one wave) from an atomic
-// ticket in index order, and a chunk's predecessor always has a smaller
-// index. A predecessor in the same wave is taken from registers (shuffles), so
-// every chunk waited on belongs to an earlier ticket, claimed by a running
-// wave that publishes without waiting on later tickets (the reference's
+// Forward progress: chunks are claimed 256 at a time (one workgroup, 64 per
+// wave) from an atomic ticket in index order, and a chunk's predecessor
+// always has a smaller index. A predecessor in the same wave is taken from
+// registers (shuffles), so every chunk waited on belongs to an earlier wave of
+// the same claim or to an earlier claim, held by a running wave that
+// publishes without waiting on later chunks (the reference's
 // blockIdx.x order, cuda/monoid_scan.cu:141, gives no such guarantee). Waits
 // are bounded by a poll count (`max_spins`, 0 = none) and by wall-clock ticks
 // (`max_ticks`); on expiry the kernel sets `error` and moves on, so every wave
@@ -112,6 +113,25 @@ struct Scanner {
             s = (uint32_t)(d >> (s << 2)) & 15u;
         }
     }
+    __device__ __forceinline__ void walk_vector(const uint4& a) {
+        walk_word(a.x); walk_word(a.y); walk_word(a.z); walk_word(a.w);
+    }
+    // A block of PF vectors: the mode test is made once for the block and the
+    // whole wave, so a walking wave runs the block as straight-line code.
+    template <int PF>
+    __device__ __forceinline__ void block(const uint4 (&a)[PF]) {
+        if (__ballot(!walking) == 0ull) {
+            if constexpr (WINDOW) {
+                s = walk_pairs<PF>(f2, a, s, w);
+            } else {
+#pragma unroll
+                for (int k = 0; k < PF; ++k) walk_vector(a[k]);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < PF; ++k) vector(a[k]);
+        }
+    }
     // One 16-byte vector. The mode test is made once per vector and for the
     // whole wave: when every active lane walks, the branch-free path lets the
     // vector's table reads issue together (a per-lane test around each word
@@ -162,7 +182,7 @@ struct Scanner {
 // its own wave through memory, which the in-wave chains below no longer do.
 // The deeper prefetch measured no faster, so it stays off.
 // IMAGE: `bytes` is the chunk-interleaved image of the plan (tile t = the
-// ticket's 64 chunks at byte tile_off[t], vector i of lane k's chunk at
+// wave's 64 chunks at byte tile_off[t], vector i of lane k's chunk at
 // tile_off[t] + (64 i + k) * 16, so one wave load reads 1 KiB contiguously);
 // otherwise the bytes the plan's ranges index.
 template <bool WINDOW, bool IMAGE, int CHAINS>
@@ -209,12 +229,21 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
     __syncthreads();
 
     const uint32_t lane = threadIdx.x & 63u;
+    __shared__ uint32_t group_base;
     while (true) {
-        // one ticket per wave: 64 consecutive chunks
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(ticket, 64u);
-        base = __shfl(base, 0, 64);
-        if ((int64_t)base >= n_chunks) break;
+        // one ticket per workgroup: TREX_BLOCK consecutive chunks, 64 per
+        // wave. One device-scope atomic on one address per workgroup rather
+        // than per wave: such atomics are served one after another, and a
+        // ticket per wave kept the last waves of a launch waiting for theirs
+        // (about 70 ns each; 0.28 ms for 4,096 waves). Chunks of earlier
+        // waves of the same workgroup are held by waves that run alongside.
+        __syncthreads();
+        if (threadIdx.x == 0) group_base = atomicAdd(ticket, (uint32_t)TREX_BLOCK);
+        __syncthreads();
+        const uint32_t gb = group_base;
+        if ((int64_t)gb >= n_chunks) break;   // the same for the whole workgroup
+        const uint32_t base = gb + (threadIdx.x & ~63u);
+        if ((int64_t)base >= n_chunks) continue;   // this wave's share is past the end
         const int64_t c = (int64_t)base + lane;
         const bool active = c < n_chunks;
 
@@ -260,7 +289,7 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
                 stream_body2<kScanPF / 2, VS>(v, v + na * VS, nb, na > nb, vector2, vector);
                 sc.join(sb);
             } else {
-                stream_body<kScanPF, VS>(v, nvec, vector);
+                stream_blocks<kScanPF, VS>(v, nvec, [&](const uint4 (&a)[kScanPF]) { sc.block(a); }, vector);
             }
         };
         if constexpr (IMAGE) {
@@ -338,11 +367,13 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
         const uint32_t kstate = g.d[0] & 0xffu;
         const uint64_t my_fn = known ? 0ull : fn_pack(g);
         if (active) {
-            if (!known) {
+            if (known) {
+                store_relaxed(&status[c], kStatusPrefix | (kstate << 8));   // the word says it all
+            } else {
                 agg[c] = my_fn;
                 jump[c] = root_pred;
+                store_release(&status[c], kStatusAggregate);   // agg and jump first
             }
-            store_release(&status[c], known ? (kStatusPrefix | (kstate << 8)) : kStatusAggregate);
         }
 
         // ---- phase 2: the root of each unresolved chain looks back ----
@@ -367,11 +398,11 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
                     __builtin_amdgcn_s_sleep(2);
                     continue;
                 }
-                fence_acquire();   // agg[j] and jump[j] were written before status[j] was released
-                if ((st & 0xffu) == kStatusPrefix) {
+                if ((st & 0xffu) == kStatusPrefix) {   // the word holds the state
                     incoming = nib(acc_fn, (st >> 8) & 15u);
                     break;
                 }
+                fence_acquire();   // agg[j] and jump[j] were written before status[j] was released
                 acc_fn = nib_then(agg[j], acc_fn);   // earlier chunks apply first
                 j = jump[j];
                 if (j < 0) {   // cannot happen: a string's first chunk publishes a prefix
@@ -384,7 +415,7 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
         // the state before each chain, from its root, then each lane's own
         incoming = __shfl(incoming, (int)root, 64);
         const uint32_t state = known ? kstate : fn_apply(g, incoming);
-        if (active && !known) store_release(&status[c], kStatusPrefix | (state << 8));
+        if (active && !known) store_relaxed(&status[c], kStatusPrefix | (state << 8));
         if (active && (flags & 2u)) {
             const int32_t sid = csid[c];
             out_state[sid] = (uint8_t)state;
