@@ -66,6 +66,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+LDS_PEAK_TBS = 157.3   # 256 CUs x 256 B per clock x 2.4 GHz
 # HBM bytes per launch of the headline kernel on this workload, from two rocprofv3
 # counter runs (FETCH_SIZE x 2.000 + WRITE_SIZE, both calibrated against a plain
 # 1 GiB copy); the command lines and counts are in the profile named with each.
@@ -841,6 +842,15 @@ def nfa_leg(args, data, offsets, batch, stream) -> dict:
                         "note": "LDS table lookups (12 per byte with 6/5-bit slices, 13 with 5-bit, 16 with nibbles) bound "
                                 "the kernel before HBM does"},
            "verified_bit_exact": None, "cpu_baseline": None}
+    reads = {"window6": 12, "window5": 13, "window": 16}.get(default)
+    if reads:
+        # the bound that holds: LDS bytes per character against the LDS's
+        # 256 B per clock per CU (256 CUs at 2.4 GHz)
+        lds_tbs = reads * 8 * gchars / 1e3
+        out["lds_roofline"] = {"reads_per_char": reads, "bytes_per_read": 8, "achieved": round(lds_tbs, 1),
+                               "peak": LDS_PEAK_TBS, "unit": "TB/s", "frac": round(lds_tbs / LDS_PEAK_TBS, 4),
+                               "note": "bank conflicts add about 2.1 cycles to each 2-cycle wave read "
+                                       "(rocprof SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE)"}
     if not args.no_verify:
         from oracle import cpu as ocpu
         from trex_amd.engine import nfa_byte_map

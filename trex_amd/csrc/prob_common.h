@@ -55,19 +55,37 @@ __device__ __forceinline__ float column_sum(float v) {
 }
 
 // Eight FP32 values -> MFMA operand: the high half (FP32 truncated to the
-// operand's mantissa, converted exactly) and, when split, the rounded rest.
+// operand's precision, converted exactly) and, when split, the rounded rest.
+// FP16 high halves come two at a time from the round-toward-zero pack, which
+// truncates in the FP16 subnormal range too (a mantissa mask does not), so
+// the rest is always x minus the exact high half.
 template <bool BF16, bool SPLIT>
 __device__ __forceinline__ void split8(const float (&x)[8], typename Lp<BF16>::vec& hi, typename Lp<BF16>::vec& lo) {
     typedef typename Lp<BF16>::scalar scalar;
-    constexpr unsigned kMask = BF16 ? 0xffff0000u : 0xffffe000u;
+    if constexpr (!BF16 && SPLIT) {
+        typedef __fp16 h2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        if constexpr (SPLIT) {
-            const float h = __uint_as_float(__float_as_uint(x[e]) & kMask);
-            hi[e] = (scalar)h;
-            lo[e] = (scalar)(x[e] - h);
-        } else {
-            hi[e] = (scalar)x[e];
+        for (int p = 0; p < 4; ++p) {
+            const h2 t = __builtin_amdgcn_cvt_pkrtz(x[2 * p], x[2 * p + 1]);
+            const uint32_t bits = __builtin_bit_cast(uint32_t, t);
+            const _Float16 h0 = __builtin_bit_cast(_Float16, (uint16_t)(bits & 0xffffu));
+            const _Float16 h1 = __builtin_bit_cast(_Float16, (uint16_t)(bits >> 16));
+            hi[2 * p] = h0;
+            hi[2 * p + 1] = h1;
+            lo[2 * p] = (_Float16)(x[2 * p] - (float)h0);
+            lo[2 * p + 1] = (_Float16)(x[2 * p + 1] - (float)h1);
+        }
+    } else {
+        constexpr unsigned kMask = BF16 ? 0xffff0000u : 0xffffe000u;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            if constexpr (SPLIT) {
+                const float h = __uint_as_float(__float_as_uint(x[e]) & kMask);
+                hi[e] = (scalar)h;
+                lo[e] = (scalar)(x[e] - h);
+            } else {
+                hi[e] = (scalar)x[e];
+            }
         }
     }
 }

@@ -17,7 +17,8 @@
 //    the mass the FP32 one does, then split into a high and a low half (two
 //    MFMAs per tile, about 22 bits of state) or rounded once;
 //  * each step, every wave gathers the sequences whose current symbol it owns
-//    into 16-column tiles (the n-th set bit of a ballot mask -> column n) whose
+//    into 16-column tiles (the n-th set bit of a ballot mask -> column n, by
+//    one ds_permute of the ranks) whose
 //    k dimension runs over all of the wave's matrices, each column's state
 //    placed in its own symbol's k block and zeros elsewhere; it runs
 //    v_mfma_f32_16x16x32_{f16,bf16} with FP32 accumulation, rescales each
@@ -45,31 +46,6 @@ constexpr int kRow = kStates + 4;   // padded LDS row (floats) against bank conf
 constexpr int kSeqPerGroup = 64;
 constexpr int kMaxSymbols = 32;
 constexpr int kChunk = 64;          // symbols staged per sequence at a time
-
-// Position of the n-th set bit of m (n < popcount(m)).
-__device__ __forceinline__ int nth_set_bit(uint64_t m, int n) {
-    int base = 0;
-    uint32_t w = (uint32_t)m;
-    const int lo_cnt = __popc(w);
-    if (n >= lo_cnt) {
-        n -= lo_cnt;
-        w = (uint32_t)(m >> 32);
-        base = 32;
-    }
-#pragma unroll
-    for (int half = 16; half >= 1; half >>= 1) {
-        const uint32_t low = w & ((1u << half) - 1u);
-        const int c = __popc(low);
-        if (n >= c) {
-            n -= c;
-            w >>= half;
-            base += half;
-        } else {
-            w = low;
-        }
-    }
-    return base;
-}
 
 template <bool BF16, bool SPLIT, int SPW>
 __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __restrict__ colscale,
@@ -193,12 +169,19 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
         // One tile holds up to 16 sequences on any of this wave's symbols: the
         // k dimension runs over the SPW symbols' matrices (SPW x 64), and each
         // column's state sits only in its own symbol's k block.
-        const uint64_t m = __ballot(cj < n_sym && cj / SPW == wave);
+        const bool sel = cj < n_sym && cj / SPW == wave;
+        const uint64_t m = __ballot(sel);
         const int cnt = __popcll(m);
+        // column n of the wave's tiles = the n-th selected sequence: each lane
+        // (a sequence) pushes its index to lane rank-among-the-selected, the
+        // others to lanes past cnt (a permutation, one ds_permute)
+        const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        const int dst = sel ? below : cnt + (lane - below);
+        const int colseq = __builtin_amdgcn_ds_permute(dst << 2, lane);
         for (int g = 0; g * 16 < cnt; ++g) {
             const int n = g * 16 + (lane & 15);
             const bool valid = n < cnt;
-            const int seq = nth_set_bit(m, valid ? n : g * 16);
+            const int seq = __shfl(colseq, valid ? n : g * 16, 64);
             const int c = syms[seq][tc];
             const int cn = syms[seq][tc + 1];
             const int myq = valid ? c - wave * SPW : -1;
@@ -265,13 +248,16 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
             mx = column_max(mx) * fix;
             const int e2 = mx > 0.f ? __builtin_amdgcn_frexp_expf(mx) : 0;   // mx = f * 2^e2, f in [0.5, 1)
             if (valid) {
+                // fix and the power of two in one factor: scaling by 2^-e2
+                // commutes with each rounding, so this equals applying it last
+                const float sc = ldexpf(fix, -e2);
 #pragma unroll
                 for (int rb = 0; rb < 4; ++rb) {
-                    const f32x4 dv = d[rb] * fix;
-                    const float4 out = make_float4(ldexpf(fmaf(dv[0], f[rb].x, dv[0] * fl[rb].x), -e2),
-                                                   ldexpf(fmaf(dv[1], f[rb].y, dv[1] * fl[rb].y), -e2),
-                                                   ldexpf(fmaf(dv[2], f[rb].z, dv[2] * fl[rb].z), -e2),
-                                                   ldexpf(fmaf(dv[3], f[rb].w, dv[3] * fl[rb].w), -e2));
+                    const f32x4 dv = d[rb] * sc;
+                    const float4 out = make_float4(fmaf(dv[0], f[rb].x, dv[0] * fl[rb].x),
+                                                   fmaf(dv[1], f[rb].y, dv[1] * fl[rb].y),
+                                                   fmaf(dv[2], f[rb].z, dv[2] * fl[rb].z),
+                                                   fmaf(dv[3], f[rb].w, dv[3] * fl[rb].w));
                     *reinterpret_cast<float4*>(&state[seq][rb * 16 + rsub]) = out;
                 }
                 if ((lane >> 4) == 0) estep[par][seq] = e2;

@@ -289,6 +289,9 @@ __device__ __forceinline__ void chunk_scan(const uint8_t* __restrict__ bytes, co
                 stream_body2<kScanPF / 2, VS>(v, v + na * VS, nb, na > nb, vector2, vector);
                 sc.join(sb);
             } else {
+                // ordinary loads on the image too: non-temporal ones measured
+                // slower on a synchronising automaton (faster on one that keeps
+                // composing)
                 stream_blocks<kScanPF, VS>(v, nvec, [&](const uint4 (&a)[kScanPF]) { sc.block(a); }, vector);
             }
         };
