@@ -768,22 +768,29 @@ def multi_leg(args, engines, batch, stream, per_pattern_ms: float, nbytes: int):
         return [a.elapsed_ms(b) for a, b in evs]
 
     if args.multi_sweep:
-        for form in ("k2f", "k2fp", "k2w"):
+        for form in ("k2f", "k2fp", "k2fc", "k2w"):
             multi.set_form(form)
             for group in (1, 2, 3, 4, 5, 6, 7, 8):
                 multi.set_group(group)
                 ms = float(np.median(timed(max(3, args.steps))))
                 log(f"multi sweep: form {form} group {group}: {ms:.4f} ms per 8 patterns, "
                     f"{P * nbytes / (ms * 1e-3) / 1e9:.0f} Gchars/s, {per_pattern_ms / ms:.2f}x per-pattern launches")
-        multi.set_form("k2fp")
-        multi.set_group(4)
+        multi.close()
+        multi = DfaMultiEngine(engines)   # back to the library's default form and group
     ms = timed(max(3, args.steps))
     avg = float(np.mean(ms))
     info = multi.info()
     alg_bytes = nbytes + 16 * n + 2 * P * n
     launches = -(-P // info["group"])
     per_launch = -(-P // launches)
-    res = {"kernel": f"dfa16_multi_{info['form']}_p{per_launch}{'_il' if batch.layout == 'interleaved' else ''}",
+    il_tag = "_il" if batch.layout == "interleaved" else ""
+    if info["form"] == "k2fc":
+        small = sum(1 for e in engines if e.table.n_states <= 4)
+        kname = f"dfa16_multi_k2fc_f{P - small}_b{small}{il_tag}" if launches == 1 else f"dfa16_multi_k2fc_*{il_tag}"
+    else:
+        kname = f"dfa16_multi_{info['form']}_p{per_launch}{il_tag}"
+    res = {"kernel": kname, "form": info["form"],
+           "patterns_of_at_most_4_states": sum(1 for e in engines if e.table.n_states <= 4),
            "patterns_per_launch": per_launch, "launches_per_step": launches,
            "ms_per_step": round(avg, 4), "ms_min": round(min(ms), 4), "ms_max": round(max(ms), 4),
            "gchars_s": round(P * nbytes / (avg * 1e-3) / 1e9, 1),
@@ -792,6 +799,33 @@ def multi_leg(args, engines, batch, stream, per_pattern_ms: float, nbytes: int):
            "hbm_frac": round(alg_bytes / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     log(f"multi-pattern pass ({res['kernel']}): {avg:.4f} ms for {P} patterns = {res['gchars_s']} Gchars/s, "
         f"{res['speedup_vs_per_pattern_launches']}x the per-pattern launches, {res['hbm_gbs']} GB/s of HBM")
+    # the paired form in launches of four (the default before the compact
+    # tables), for comparison; its results are checked against the default's
+    if info["form"] == "k2fc":
+        multi.set_form("k2fp")
+        multi.set_group(4)
+        d_f2, d_a2 = DeviceBuffer(P * n), DeviceBuffer(P * n)
+        cms = []
+        for _ in range(args.warmup):
+            multi.match_batch_device(batch, d_f2.ptr, d_a2.ptr, stream)
+        for _ in range(max(3, args.steps)):
+            a, b = Event(), Event()
+            a.record(stream)
+            multi.match_batch_device(batch, d_f2.ptr, d_a2.ptr, stream)
+            b.record(stream)
+            stream.synchronize()
+            cms.append(a.elapsed_ms(b))
+        cavg = float(np.mean(cms))
+        same = bool(np.array_equal(d_f2.download(np.uint8, P * n, stream), d_f.download(np.uint8, P * n, stream))
+                    and np.array_equal(d_a2.download(np.uint8, P * n, stream), d_a.download(np.uint8, P * n, stream)))
+        res["paired_form"] = {"form": "k2fp", "patterns_per_launch": 4, "ms_per_step": round(cavg, 4),
+                              "gchars_s": round(P * nbytes / (cavg * 1e-3) / 1e9, 1),
+                              "speedup_vs_per_pattern_launches": round(per_pattern_ms / cavg, 2),
+                              "same_results_as_default_form": same}
+        log(f"multi-pattern pass, paired tables in launches of four: {cavg:.4f} ms = "
+            f"{res['paired_form']['gchars_s']} Gchars/s; same results as the compact form: {same}")
+        d_f2.free()
+        d_a2.free()
     multi.close()
     return res, (d_f, d_a)
 

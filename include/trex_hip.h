@@ -207,14 +207,20 @@ typedef struct trex_dfa_multi trex_dfa_multi;
  * the pattern). */
 int trex_dfa_multi_create(trex_dfa* const* patterns, int n_patterns, trex_dfa_multi** out);
 int trex_dfa_multi_destroy(trex_dfa_multi* m);
-/* Table form: TREX_DFA_MULTI_K2F_PAIRED (default: the pair functions of two
+/* Table form: TREX_DFA_MULTI_K2F_PAIRED (the pair functions of two
  * patterns side by side, one 16-byte LDS read for both; rebuilt when the
- * group changes), TREX_DFA_K2_WINDOW_F (8 KB of LDS per pattern) or
- * TREX_DFA_K2_WINDOW (16 KB per pattern). */
+ * group changes), TREX_DFA_MULTI_K2F_COMPACT (patterns of at most 4 states
+ * as one byte per pair entry, eight to a 64-bit word, beside the others'
+ * full functions: fewer table reads per byte pair; rebuilt when the group
+ * changes), TREX_DFA_K2_WINDOW_F (8 KB of LDS per pattern) or
+ * TREX_DFA_K2_WINDOW (16 KB per pattern). TREX_DFA_AUTO (and a new handle):
+ * compact when at least two patterns have at most 4 states, else paired. */
 #define TREX_DFA_MULTI_K2F_PAIRED 7
+#define TREX_DFA_MULTI_K2F_COMPACT 8
 int trex_dfa_multi_set_form(trex_dfa_multi* m, int form);
-/* Patterns advanced per launch at most, 1..8 (default 4); the patterns are
- * split evenly over ceil(n_patterns / group) launches. */
+/* Patterns advanced per launch at most, 1..8 (a new handle: 8 when it
+ * starts in the compact form, else 4); the patterns are split evenly over
+ * ceil(n_patterns / group) launches. */
 int trex_dfa_multi_set_group(trex_dfa_multi* m, int group);
 int trex_dfa_multi_info(const trex_dfa_multi* m, int* n_patterns, int* form, int* group);
 /* Final state and accept bit of every (pattern, string), pattern-major:
@@ -477,7 +483,9 @@ int trex_plan_set_chains(trex_plan* plan, int chains);
 /* Bytes left after each tile of the plan's image (a multiple of 16, at most
  * 65536; 256 by default). Changes trex_plan_image_size and the layout of
  * images built afterwards; an image must be built under the gap it is
- * matched with. A tuning choice with no counterpart in the reference. */
+ * matched with. Waits for the device to finish queued work first (launches
+ * in flight read the old layout). A tuning choice with no counterpart in the
+ * reference. */
 int trex_plan_set_image_gap(trex_plan* plan, int64_t gap);
 /* Error word of the plan's last launch (0 = none; bit 0 = a wait expired),
  * read synchronously after the work queued on `stream`. */

@@ -1,7 +1,7 @@
 """Multi-pattern DFA-16 pass (trex_dfa_multi_*): every (pattern, string)
 result equals the C oracle's table walk and the pattern's own single-pattern
 kernel, bit for bit, over ragged and empty strings, bytes outside the
-patterns' windows, patterns with different windows, every layout, all three table
+patterns' windows, patterns with different windows, every layout, all four table
 forms and every group size."""
 
 import numpy as np
@@ -59,7 +59,7 @@ def test_multi_binding_declared():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("layout", ["csr", "staggered", "interleaved"])
-@pytest.mark.parametrize("form", ["k2f", "k2w", "k2fp"])
+@pytest.mark.parametrize("form", ["k2f", "k2w", "k2fp", "k2fc"])
 def test_gpu_multi_equals_oracle(gpu_lib, layout, form):
     from trex_amd.engine import DfaEngine, DfaMultiEngine
     rng = np.random.default_rng(101 + len(layout) + len(form))
@@ -97,14 +97,67 @@ def test_gpu_multi_every_launch_width(gpu_lib, n_patterns):
     offsets = offsets + 3                                   # no string starts 16-byte aligned
     data = np.concatenate([np.zeros(3, np.uint8), data])
     want_f, want_a = _oracle(tables, data, offsets)
-    for form in ("k2f", "k2w", "k2fp"):
+    small = sum(t.n_states <= 4 for t in tables)
+    for form in ("k2f", "k2w", "k2fp", "k2fc"):
         multi = DfaMultiEngine([DfaEngine(t) for t in tables])
+        # a new handle: compact tables, all patterns in one launch, when two or
+        # more patterns have at most 4 states; else paired in launches of four
+        start = multi.info()
+        assert start["form"] == ("k2fc" if small >= 2 else "k2fp")
+        assert start["group"] == (8 if small >= 2 else 4)
         multi.set_form(form)
-        assert multi.info() == {"n_patterns": n_patterns, "form": form, "group": 4}
+        assert multi.info() == {"n_patterns": n_patterns, "form": form, "group": start["group"]}
         multi.set_group(8)                                  # all in one launch
         f, a = multi.match_arrays(data, offsets, layout="csr")
         assert np.array_equal(f, want_f) and np.array_equal(a, want_a), form
         multi.close()
+
+
+def _sized_patterns(rng, small, large):
+    """`small` tables of at most 4 states and `large` of more, over the three
+    windows (random regexes redrawn until they fall in the wanted class)."""
+    from trex_amd.tables import dfa_byte_table
+    from trex_amd.workloads import random_dfa_pattern
+    bases = list(ALPHABETS)
+    out, want = [], [True] * small + [False] * large
+    i = 0
+    while want:
+        alphabet = ALPHABETS[bases[i % len(bases)]]
+        safe = "".join(c for c in alphabet if c not in "[]\\()|*+?.{}^$-")
+        t = dfa_byte_table(random_dfa_pattern(rng, safe)[1])
+        i += 1
+        if (t.n_states <= 4) in want:
+            want.remove(t.n_states <= 4)
+            out.append(t)
+    order = rng.permutation(len(out))            # classes interleaved in pattern order
+    return [out[k] for k in order]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("small,large", [(8, 0), (0, 8), (5, 3), (3, 9), (11, 2)])
+def test_gpu_multi_compact_mixed_widths(gpu_lib, small, large):
+    """The compact form with every mix of one-byte (at most 4 states) and
+    full patterns per launch: launch order puts full patterns first and the
+    outputs go back to the patterns' own rows; every layout and group size,
+    head bytes, long strings and bytes outside the windows."""
+    from trex_amd.engine import DfaEngine, DfaMultiEngine
+    rng = np.random.default_rng(300 + 16 * small + large)
+    tables = _sized_patterns(rng, small, large)
+    assert sum(t.n_states <= 4 for t in tables) == small
+    data, offsets = _batch(rng, 900, 3000)
+    offsets = offsets + 5
+    data = np.concatenate([np.zeros(5, np.uint8), data])
+    want_f, want_a = _oracle(tables, data, offsets)
+    multi = DfaMultiEngine([DfaEngine(t) for t in tables])
+    multi.set_form("k2fc")
+    for layout in ("csr", "interleaved"):
+        for group in (8, 5, 3, 1):
+            multi.set_group(group)
+            f, a = multi.match_arrays(data, offsets, layout=layout)
+            assert np.array_equal(f, want_f), (layout, group, np.argwhere(f != want_f)[:5])
+            assert np.array_equal(a, want_a), (layout, group)
+    assert multi.info()["form"] == "k2fc"
+    multi.close()
 
 
 @pytest.mark.gpu

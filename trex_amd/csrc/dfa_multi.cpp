@@ -7,9 +7,10 @@
 // A dispatch launches ceil(n_patterns / group) kernels, the patterns split
 // as evenly as possible among them; each reads the batch once and advances
 // its patterns. The kernels are bound by LDS bank conflicts of the table
-// reads; the default paired form reads two patterns' pair functions with one
-// 16-byte read, and the default group of 4 (33 KB of tables) leaves room for
-// four workgroups per CU.
+// reads. The paired form reads two patterns' pair functions with one 16-byte
+// read (default group 4: 33 KB of tables, four workgroups per CU); the
+// compact form also packs patterns of at most 4 states eight to a word and
+// takes all patterns in one launch (the default when two or more fit).
 
 #include <hip/hip_runtime_api.h>
 
@@ -37,6 +38,24 @@ struct MultiParams {
     uint32_t accept[kMultiMax];
 };
 
+// The compact form's argument block (CompactParams in dfa_compact_kernels.hip):
+// the same per launch pattern, plus the pattern's output row.
+struct CompactParams {
+    uint32_t win4[kMultiMax];
+    uint32_t start[kMultiMax];
+    uint32_t dead[kMultiMax];
+    uint32_t accept[kMultiMax];
+    uint32_t slot[kMultiMax];
+};
+
+// One launch of the compact form: NF full and NB byte patterns, its tables
+// at byte offsets into the handle's compact buffers.
+struct CompactLaunch {
+    int nf = 0, nb = 0;
+    size_t words_off = 0, syms_off = 0;
+    CompactParams prm{};
+};
+
 }  // namespace
 
 struct trex_dfa_multi {
@@ -52,6 +71,12 @@ struct trex_dfa_multi {
     uint8_t* d_fp = nullptr;      // paired tables of the current grouping, launch after launch
     std::vector<size_t> fp_off;   // byte offset of each launch's paired tables in d_fp
     int fp_group = -1;            // the group d_fp was built for
+    std::vector<uint64_t> h_f1;   // [n][32] single-symbol functions (host copy)
+    std::vector<int> n_states;
+    uint8_t* d_fc = nullptr;      // compact pair tables of the current grouping
+    uint8_t* d_fcs = nullptr;     // and the launches' single-symbol tables in launch order
+    std::vector<CompactLaunch> fc;
+    int fc_group = -1;            // the group d_fc was built for
 };
 
 namespace {
@@ -94,6 +119,90 @@ int build_paired(trex_dfa_multi* m) {
     return TREX_OK;
 }
 
+// A pattern of at most 4 states as one byte per pair entry: 2 bits per state
+// (states 0..3; a table of fewer states maps its padding states to themselves).
+bool narrow(const trex_dfa_multi* m, int p) { return m->n_states[p] <= 4; }
+
+uint64_t byte_function(uint64_t f) {
+    uint64_t b = 0;
+    for (int s = 0; s < 4; ++s) b |= ((f >> (4 * s)) & 3u) << (2 * s);
+    return b;
+}
+
+// Compact tables for the current grouping: per launch, its full patterns
+// first (one 64-bit word each), then its patterns of at most 4 states (eight
+// bytes to a word); the words of entry x stored pair by pair (NW / 2 tables of
+// 1024 x 16 bytes, then 1024 x 8 bytes for an odd last word), and the
+// single-symbol functions in the same pattern order.
+int build_compact(trex_dfa_multi* m) {
+    std::vector<uint64_t> words, syms;
+    std::vector<CompactLaunch> fc;
+    for (const auto& [p0, P] : launches_of(m->n, m->group)) {
+        std::vector<int> order;
+        for (int k = p0; k < p0 + P; ++k)
+            if (!narrow(m, k)) order.push_back(k);
+        const int nf = (int)order.size();
+        for (int k = p0; k < p0 + P; ++k)
+            if (narrow(m, k)) order.push_back(k);
+        CompactLaunch L;
+        L.nf = nf;
+        L.nb = P - nf;
+        L.words_off = words.size() * 8;
+        L.syms_off = syms.size() * 8;
+        const int nw = nf + (L.nb + 7) / 8;
+        std::vector<uint64_t> entry((size_t)1024 * nw, 0);   // [x][word]
+        for (int x = 0; x < 1024; ++x) {
+            for (int k = 0; k < nf; ++k) entry[(size_t)x * nw + k] = m->h_f2[(size_t)order[k] * 1024 + x];
+            for (int i = 0; i < L.nb; ++i)
+                entry[(size_t)x * nw + nf + i / 8] |=
+                    byte_function(m->h_f2[(size_t)order[nf + i] * 1024 + x]) << (8 * (i % 8));
+        }
+        for (int j = 0; j < nw / 2; ++j)
+            for (int x = 0; x < 1024; ++x) {
+                words.push_back(entry[(size_t)x * nw + 2 * j]);
+                words.push_back(entry[(size_t)x * nw + 2 * j + 1]);
+            }
+        if (nw & 1)
+            for (int x = 0; x < 1024; ++x) words.push_back(entry[(size_t)x * nw + nw - 1]);
+        for (int k = 0; k < P; ++k) {
+            const int q = order[k];
+            for (int c = 0; c < 32; ++c) syms.push_back(m->h_f1[(size_t)q * 32 + c]);
+            L.prm.win4[k] = m->win4[q];
+            L.prm.start[k] = m->start[q];
+            L.prm.dead[k] = m->dead[q];
+            L.prm.accept[k] = m->accept[q];
+            L.prm.slot[k] = (uint32_t)q;
+        }
+        fc.push_back(L);
+    }
+    hipFree(m->d_fc);
+    hipFree(m->d_fcs);
+    m->d_fc = m->d_fcs = nullptr;
+    m->fc_group = -1;
+    TREX_TRY(upload((void**)&m->d_fc, words.data(), words.size() * 8));
+    TREX_TRY(upload((void**)&m->d_fcs, syms.data(), syms.size() * 8));
+    m->fc = std::move(fc);
+    m->fc_group = m->group;
+    return TREX_OK;
+}
+
+// The default form: compact tables, all patterns of a launch at once, when at
+// least two patterns fit one byte (8 seeded 32-symbol patterns, 5 of them of
+// 3 states: 0.41 ms per 1 GiB against 0.69 ms paired in launches of four);
+// otherwise paired tables in launches of four.
+int auto_form(const trex_dfa_multi* m) {
+    int small = 0;
+    for (int p = 0; p < m->n; ++p) small += narrow(m, p) ? 1 : 0;
+    return small >= 2 ? TREX_DFA_MULTI_K2F_COMPACT : TREX_DFA_MULTI_K2F_PAIRED;
+}
+
+// (Re)builds the tables the handle's form needs for its grouping.
+int ensure_tables(trex_dfa_multi* m) {
+    if (m->form == TREX_DFA_MULTI_K2F_PAIRED && m->fp_group != m->group) TREX_TRY(build_paired(m));
+    if (m->form == TREX_DFA_MULTI_K2F_COMPACT && m->fc_group != m->group) TREX_TRY(build_compact(m));
+    return TREX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -105,6 +214,8 @@ int trex_dfa_multi_destroy(trex_dfa_multi* m) {
     hipFree(m->d_t2);
     hipFree(m->d_t1);
     hipFree(m->d_fp);
+    hipFree(m->d_fc);
+    hipFree(m->d_fcs);
     delete m;
     return TREX_OK;
 }
@@ -151,14 +262,20 @@ int trex_dfa_multi_create(trex_dfa* const* patterns, int n_patterns, trex_dfa_mu
             }
         uint32_t mask = 0;
         for (int s = 0; s < 16; ++s) mask |= (acc[s] ? 1u : 0u) << s;
+        m->n_states.push_back(h->n_states);
         m->win4.push_back((uint32_t)base * 0x01010101u);
         m->start.push_back((uint32_t)h->start);
         m->dead.push_back((uint32_t)h->dead);
         m->accept.push_back(mask);
     }
+    m->h_f1 = f1;
     if (!rc) rc = upload((void**)&m->d_f1, f1.data(), f1.size() * sizeof(uint64_t));
     if (!rc) rc = upload((void**)&m->d_t1, t1w.data(), t1w.size());
-    if (!rc) rc = build_paired(m);
+    if (!rc) {
+        m->form = auto_form(m);
+        m->group = m->form == TREX_DFA_MULTI_K2F_COMPACT ? kMultiMax : kMultiGroup;
+        rc = ensure_tables(m);
+    }
     if (rc) { trex_dfa_multi_destroy(m); return rc; }
     *out = m;
     return TREX_OK;
@@ -166,21 +283,23 @@ int trex_dfa_multi_create(trex_dfa* const* patterns, int n_patterns, trex_dfa_mu
 
 int trex_dfa_multi_set_form(trex_dfa_multi* m, int form) {
     if (!m) return fail(TREX_ERR_INVALID, "null handle");
-    if (form == TREX_DFA_AUTO) form = TREX_DFA_MULTI_K2F_PAIRED;
-    if (form != TREX_DFA_K2_WINDOW_F && form != TREX_DFA_K2_WINDOW && form != TREX_DFA_MULTI_K2F_PAIRED)
-        return fail(TREX_ERR_INVALID, "form %d: use TREX_DFA_K2_WINDOW_F, TREX_DFA_K2_WINDOW or "
-                    "TREX_DFA_MULTI_K2F_PAIRED", form);
-    if (form == TREX_DFA_MULTI_K2F_PAIRED && m->fp_group != m->group) TREX_TRY(build_paired(m));
+    if (form == TREX_DFA_AUTO) form = auto_form(m);
+    if (form != TREX_DFA_K2_WINDOW_F && form != TREX_DFA_K2_WINDOW && form != TREX_DFA_MULTI_K2F_PAIRED &&
+        form != TREX_DFA_MULTI_K2F_COMPACT)
+        return fail(TREX_ERR_INVALID, "form %d: use TREX_DFA_K2_WINDOW_F, TREX_DFA_K2_WINDOW, "
+                    "TREX_DFA_MULTI_K2F_PAIRED or TREX_DFA_MULTI_K2F_COMPACT", form);
+    const int prev = m->form;
     m->form = form;
-    return TREX_OK;
+    const int rc = ensure_tables(m);
+    if (rc) m->form = prev;
+    return rc;
 }
 
 int trex_dfa_multi_set_group(trex_dfa_multi* m, int group) {
     if (!m) return fail(TREX_ERR_INVALID, "null handle");
     if (group < 1 || group > kMultiMax) return fail(TREX_ERR_INVALID, "group=%d outside 1..%d", group, kMultiMax);
     m->group = group;
-    if (m->form == TREX_DFA_MULTI_K2F_PAIRED && m->fp_group != m->group) TREX_TRY(build_paired(m));
-    return TREX_OK;
+    return ensure_tables(m);
 }
 
 int trex_dfa_multi_info(const trex_dfa_multi* m, int* n_patterns, int* form, int* group) {
@@ -204,6 +323,19 @@ int multi_dispatch(trex_dfa_multi* m, const uint8_t* d_bytes, const int64_t* d_b
     const int64_t grid64 = (n_strings + kBlock - 1) / kBlock;
     if (grid64 > 0x7fffffff) return fail(TREX_ERR_CAPACITY, "too many strings");
     const unsigned grid = (unsigned)grid64;
+    if (m->form == TREX_DFA_MULTI_K2F_COMPACT) {
+        if (m->fc_group != m->group) return fail(TREX_ERR_INVALID, "compact tables not built");
+        for (const CompactLaunch& L : m->fc) {
+            const uint8_t* words = m->d_fc + L.words_off;
+            const uint8_t* syms = m->d_fcs + L.syms_off;
+            CompactParams prm = L.prm;
+            void* args[] = {&d_bytes, &d_begin, &d_end, &n_strings, &words, &syms, &prm, &d_final_state, &d_accept};
+            char name[48];
+            std::snprintf(name, sizeof(name), "dfa16_multi_k2fc_f%d_b%d%s", L.nf, L.nb, il ? "_il" : "");
+            TREX_TRY(launch(name, grid, kBlock, 0, (hipStream_t)stream, args));
+        }
+        return TREX_OK;
+    }
     const bool paired = m->form == TREX_DFA_MULTI_K2F_PAIRED;
     const bool f = m->form == TREX_DFA_K2_WINDOW_F || paired;
     const size_t pair_bytes = f ? 1024 * 8 : 1024 * 16, sym_bytes = f ? 32 * 8 : 32 * 16;

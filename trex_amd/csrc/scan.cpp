@@ -235,6 +235,10 @@ int trex_plan_set_image_gap(trex_plan* p, int64_t gap) {
     if (gap < 0 || gap > 65536 || gap % 16 != 0)
         return fail(TREX_ERR_INVALID, "image gap %lld: a multiple of 16 in [0, 65536]", (long long)gap);
     std::lock_guard<std::mutex> lock(p->mu);
+    // launches already queued read the tile offsets on the device: let them
+    // finish before the offsets change under them
+    const hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(e, "synchronise before changing the image layout");
     p->image_gap = gap;
     return layout_tiles(p);
 }

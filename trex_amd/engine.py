@@ -229,7 +229,7 @@ def interleaved_image(data: np.ndarray, offsets: np.ndarray) -> tuple[np.ndarray
     return image, np.ascontiguousarray(begin, np.int64), np.ascontiguousarray(end, np.int64)
 
 
-MULTI_FORMS = {"k2f": 5, "k2w": 3, "k2fp": 7}
+MULTI_FORMS = {"k2f": 5, "k2w": 3, "k2fp": 7, "k2fc": 8}
 MULTI_FORM_NAMES = {v: k for k, v in MULTI_FORMS.items()}
 
 
