@@ -12,7 +12,8 @@
 // spread over all 16 bank quads), plus one 8-byte entry when NW is odd.
 //
 // Full pattern k:  s = (w_k >> 4s) & 15
-// Byte pattern i:  s = (w_{NF + i/8} >> (8 (i % 8) + 2s)) & 3
+// Byte pattern i:  s = bits [8 (i % 8) + 2s, +2) of w_{NF + i/8} (one bit-field
+//                  extract from the word's half that holds the byte)
 //
 // Head and tail bytes take the 16-nibble single-symbol functions of every
 // pattern (a state below 4 reads a nibble below 4). The window check and the
@@ -84,11 +85,20 @@ __device__ __forceinline__ void compact_kernel(const uint8_t* __restrict__ bytes
 #pragma unroll
         for (int k = 0; k < NF; ++k) s[k] = (uint32_t)(W[k] >> (s[k] << 2)) & 15u;
 #pragma unroll
-        for (int i = 0; i < NB; ++i)
-            s[NF + i] = (uint32_t)(W[NF + i / 8] >> (8 * (i % 8) + (s[NF + i] << 1))) & 3u;
+        for (int i = 0; i < NB; ++i) {
+            // the byte's dword, then one bit-field extract (offset 8 (i % 4) + 2s)
+            const uint64_t wv = W[NF + i / 8];
+            const uint32_t dw = (i % 8) < 4 ? (uint32_t)wv : (uint32_t)(wv >> 32);
+            s[NF + i] = __builtin_amdgcn_ubfe(dw, 8u * (i % 4) + (s[NF + i] << 1), 2u);
+        }
     };
     auto word = [&](uint32_t x) {
         w.add(x);
+        // the word's table reads go out once the previous word's steps are
+        // done (an empty asm ties the word to a state): one word ahead at
+        // most, or the compiler hoists a whole block's reads and runs out of
+        // registers
+        asm volatile("" : "+v"(x) : "v"(s[0]));
         pstep(pair_lo(x));
         pstep(pair_hi(x));
     };
