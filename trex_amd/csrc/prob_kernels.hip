@@ -66,8 +66,11 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
     // per-symbol rows padded to kRow floats: lanes of one read that name
     // different symbols start 16 bytes apart in the banks instead of on the
     // same ones (a 64-float row is exactly one turn of the 64 banks)
-    __shared__ __attribute__((aligned(16))) float dscale[kMaxSymbols][kRow];      // column factors, high part
-    __shared__ __attribute__((aligned(16))) float dscale_lo[kMaxSymbols][kRow];   // and the FP32 rest
+    // column factors, high part and FP32 rest; row kMaxSymbols is the identity
+    // (1, 0), read when the next step takes no factors, so that read needs
+    // no branch
+    __shared__ __attribute__((aligned(16))) float dscale[kMaxSymbols + 1][kRow];
+    __shared__ __attribute__((aligned(16))) float dscale_lo[kMaxSymbols + 1][kRow];
     __shared__ __attribute__((aligned(16))) float dcq[kMaxSymbols][kRow];         // colsum(round T), high
     __shared__ __attribute__((aligned(16))) float dcq_lo[kMaxSymbols][kRow];      // and rest
     __shared__ uint8_t syms[kSeqPerGroup][kChunk + 4];
@@ -128,12 +131,14 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
             syms[s][k] = v;
         }
     };
-    for (int i = tid; i < kMaxSymbols * kStates; i += kBlock) {
+    for (int i = tid; i < (kMaxSymbols + 1) * kStates; i += kBlock) {
         const bool in = i < n_sym * kStates;
         dscale[i / kStates][i % kStates] = in ? colscale[i] : 1.f;
         dscale_lo[i / kStates][i % kStates] = in ? colscale[n_sym * kStates + i] : 0.f;
-        dcq[i / kStates][i % kStates] = in ? colscale[2 * n_sym * kStates + i] : 0.f;
-        dcq_lo[i / kStates][i % kStates] = in ? colscale[3 * n_sym * kStates + i] : 0.f;
+        if (i < kMaxSymbols * kStates) {
+            dcq[i / kStates][i % kStates] = in ? colscale[2 * n_sym * kStates + i] : 0.f;
+            dcq_lo[i / kStates][i % kStates] = in ? colscale[3 * n_sym * kStates + i] : 0.f;
+        }
     }
     if (tid < 2 * kSeqPerGroup) estep[tid / kSeqPerGroup][tid % kSeqPerGroup] = 0;
     __syncthreads();
@@ -226,13 +231,11 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
             // column factors of the next symbol for this lane's 16 result rows
             // (none when the next step is the last one or the sequence ends)
             float4 f[4], fl[4];
-            const bool use_f = cn < n_sym;
+            const int fr = cn < n_sym ? cn : kMaxSymbols;   // the identity row otherwise
 #pragma unroll
             for (int rb = 0; rb < 4; ++rb) {
-                f[rb] = use_f ? *reinterpret_cast<const float4*>(&dscale[cn][rb * 16 + rsub])
-                              : make_float4(1.f, 1.f, 1.f, 1.f);
-                fl[rb] = use_f ? *reinterpret_cast<const float4*>(&dscale_lo[cn][rb * 16 + rsub])
-                               : make_float4(0.f, 0.f, 0.f, 0.f);
+                f[rb] = *reinterpret_cast<const float4*>(&dscale[fr][rb * 16 + rsub]);
+                fl[rb] = *reinterpret_cast<const float4*>(&dscale_lo[fr][rb * 16 + rsub]);
             }
             float mx = 0.f, mass_out = 0.f;
 #pragma unroll
