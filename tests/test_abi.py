@@ -55,8 +55,20 @@ def test_kernel_module_built_for_gfx950():
                    b"pnfa64_bf16x2", b"dfa16_multi_k2f_p4_il", b"dfa16_multi_k2w_p8", b"nfa64_window6",
                    b"nfa64_window5", b"mix_persistent_slice", b"pnfa64_chunk_f16x2", b"pnfa64_combine",
                    b"dfa16_chunks_window", b"dfa16_chunks_bytes", b"dfa16_chunks_window_img",
-                   b"dfa16_chunks_bytes_img"]:
+                   b"dfa16_chunks_bytes_img", b"dfa16_chunks_window_c2", b"dfa16_k2f_rot_pf8",
+                   b"dfa16_multi_k2fp_p4_il", b"dfa16_multi_k2fc_f3_b5_il", b"dfa16_multi_k2fc_f0_b8",
+                   b"dfa16_multi_k2fc_f8_b0_il", b"chunk_table_count", b"chunk_table_fill"]:
         assert kernel in blob
+
+
+def test_header_is_plain_c():
+    """The boundary header compiles as C99 and as C++ on its own: plain
+    pointers and sizes, no HIP or torch types in the signatures."""
+    for lang, std in (("c", "-std=c99"), ("c++", "-std=c++17")):
+        r = subprocess.run(["gcc" if lang == "c" else "g++", "-x", lang, std, "-fsyntax-only", "-Wall", "-Werror",
+                            "-I", str(ROOT / "include"), "-"], input='#include "trex_hip.h"\nint main(void) { return 0; }\n',
+                           capture_output=True, text=True)
+        assert r.returncode == 0, (lang, r.stderr)
 
 
 def test_library_loads_without_device():
