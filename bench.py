@@ -525,11 +525,18 @@ def scan_leg(args, data, stream) -> dict:
     rf = ra = None
     if not args.no_verify:
         from oracle import cpu as ocpu
-        rf, ra = ocpu.dfa_walk(table.delta, table.accept, table.start, data, offsets)
+        threads = min(ocpu.default_threads(), n_long)
+        t0 = time.perf_counter()
+        rf, ra = ocpu.dfa_walk(table.delta, table.accept, table.start, data, offsets, threads=threads)
+        t_cpu = time.perf_counter() - t0
         same = bool(np.array_equal(d_f.download(np.uint8, n_long), rf)
                     and np.array_equal(d_a.download(np.uint8, n_long), ra) and err == 0)
         out["verified_bit_exact"] = same
-        log(f"look-back scan: bit-exact={same}")
+        out["cpu_baseline"] = {"value": round(data.nbytes / t_cpu / 1e9, 3), "unit": "Gchars/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"the same 16 x 64 MiB through the C oracle byte walk (one thread per "
+                                         f"string, {threads} threads), {t_cpu:.2f} s; host {cpu_model()}"}
+        log(f"look-back scan: bit-exact={same}; C oracle {data.nbytes / t_cpu / 1e9:.2f} Gchars/s on {threads} threads")
         if not same:
             log("PARITY FAILURE (look-back scan)")
 
