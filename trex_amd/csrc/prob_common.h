@@ -64,17 +64,25 @@ __device__ __forceinline__ void split8(const float (&x)[8], typename Lp<BF16>::v
     typedef typename Lp<BF16>::scalar scalar;
     if constexpr (!BF16 && SPLIT) {
         typedef __fp16 h2 __attribute__((ext_vector_type(2)));
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        u32x4 hw, lw;
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
             const h2 t = __builtin_amdgcn_cvt_pkrtz(x[2 * p], x[2 * p + 1]);
             const uint32_t bits = __builtin_bit_cast(uint32_t, t);
-            const _Float16 h0 = __builtin_bit_cast(_Float16, (uint16_t)(bits & 0xffffu));
-            const _Float16 h1 = __builtin_bit_cast(_Float16, (uint16_t)(bits >> 16));
-            hi[2 * p] = h0;
-            hi[2 * p + 1] = h1;
-            lo[2 * p] = (_Float16)(x[2 * p] - (float)h0);
-            lo[2 * p + 1] = (_Float16)(x[2 * p + 1] - (float)h1);
+            // rest = round(x - high) into each half by one mixed-precision FMA
+            // (FP16 source, FP32 sum, one rounding to FP16): the subtraction is
+            // exact, so this is the same value as converting, subtracting and
+            // rounding, in one instruction per element instead of three
+            uint32_t rest;
+            asm("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(rest) : "v"(bits), "v"(x[2 * p]));
+            asm("v_fma_mixhi_f16 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+                : "+v"(rest) : "v"(bits), "v"(x[2 * p + 1]));
+            hw[p] = bits;
+            lw[p] = rest;
         }
+        hi = __builtin_bit_cast(typename Lp<BF16>::vec, hw);
+        lo = __builtin_bit_cast(typename Lp<BF16>::vec, lw);
     } else {
         constexpr unsigned kMask = BF16 ? 0xffff0000u : 0xffffe000u;
 #pragma unroll

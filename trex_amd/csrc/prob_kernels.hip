@@ -18,7 +18,8 @@
 //    MFMAs per tile, about 22 bits of state) or rounded once;
 //  * each step, every wave gathers the sequences whose current symbol it owns
 //    into 16-column tiles (the n-th set bit of a ballot mask -> column n, by
-//    one ds_permute of the ranks) whose
+//    one ds_permute of the ranks, worked out during the step before, as it
+//    reads only the staged symbols) whose
 //    k dimension runs over all of the wave's matrices, each column's state
 //    placed in its own symbol's k block and zeros elsewhere; it runs
 //    v_mfma_f32_16x16x32_{f16,bf16} with FP32 accumulation, rescales each
@@ -57,25 +58,24 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
     typedef typename Lp<BF16>::vec vec;
     constexpr int kBlock = 64 * (kMaxSymbols / SPW);
     constexpr int kTps = kBlock / kSeqPerGroup;   // threads per sequence in the last step
-    constexpr int kStage = kChunk + 1;            // staged symbols per sequence: a chunk plus the next one
+    constexpr int kRing = 2 * kChunk;             // staged symbols per sequence: this chunk and the next
     constexpr uint8_t kDone = 0xfe;               // no rounded step at this position (the sequence has ended)
-    constexpr uint8_t kBad = 0xff;                // symbol outside the alphabet: the state empties
     // One state row per sequence, updated in place: a row is read and then
     // rewritten by the one wave that owns its sequence's symbol at that step.
     __shared__ __attribute__((aligned(16))) float state[kSeqPerGroup][kRow];
     // per-symbol rows padded to kRow floats: lanes of one read that name
     // different symbols start 16 bytes apart in the banks instead of on the
     // same ones (a 64-float row is exactly one turn of the 64 banks)
-    // column factors, high part and FP32 rest; row kMaxSymbols is the identity
-    // (1, 0), read when the next step takes no factors, so that read needs
-    // no branch
-    __shared__ __attribute__((aligned(16))) float dscale[kMaxSymbols + 1][kRow];
-    __shared__ __attribute__((aligned(16))) float dscale_lo[kMaxSymbols + 1][kRow];
+    __shared__ __attribute__((aligned(16))) float dscale[kMaxSymbols][kRow];      // column factors, high
+    __shared__ __attribute__((aligned(16))) float dscale_lo[kMaxSymbols][kRow];   // and FP32 rest
     __shared__ __attribute__((aligned(16))) float dcq[kMaxSymbols][kRow];         // colsum(round T), high
     __shared__ __attribute__((aligned(16))) float dcq_lo[kMaxSymbols][kRow];      // and rest
-    __shared__ uint8_t syms[kSeqPerGroup][kChunk + 4];
-    __shared__ int32_t estep[2][kSeqPerGroup];   // exponent taken out at a step, by step parity
-    __shared__ int32_t expo[kSeqPerGroup];
+    // symbol at position p of sequence s: syms[s][p % kRing] (rows of an odd
+    // number of words: lanes reading one position of different sequences hit
+    // different banks)
+    __shared__ uint8_t syms[kSeqPerGroup][kRing + 4];
+    __shared__ int32_t expo[kSeqPerGroup];    // log2-scale of each sequence's state
+    __shared__ int32_t empty[kSeqPerGroup];   // a symbol outside the alphabet: the state is zero from there on
     __shared__ int64_t sq_beg[kSeqPerGroup];
     __shared__ int32_t sq_len[kSeqPerGroup];
     __shared__ int32_t max_len;
@@ -120,159 +120,197 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
         sq_len[tid] = (int32_t)len;
         atomicMax(&max_len, (int32_t)len);
     }
-    auto stage = [&](int t) {
-        for (int i = tid; i < kSeqPerGroup * kStage; i += kBlock) {
-            const int s = i / kStage, k = i % kStage;
+    // positions p0 .. p0 + kChunk - 1 of every sequence into the ring. A
+    // symbol outside the alphabet empties the state, which then stays zero:
+    // the sequence is marked and its results are -inf whatever it steps
+    // through afterwards (the position itself takes no step).
+    auto stage = [&](int p0) {
+        for (int i = tid; i < kSeqPerGroup * kChunk; i += kBlock) {
+            const int s = i / kChunk, k = i % kChunk;
             uint8_t v = kDone;
-            if (t + k < sq_len[s] - 1) {
-                const uint8_t c = symbols[sq_beg[s] + t + k];
-                v = c < n_sym ? c : kBad;
+            if (p0 + k < sq_len[s] - 1) {
+                const uint8_t c = symbols[sq_beg[s] + p0 + k];
+                if (c < n_sym) v = c;
+                else empty[s] = 1;
             }
-            syms[s][k] = v;
+            syms[s][(p0 + k) % kRing] = v;
         }
     };
-    for (int i = tid; i < (kMaxSymbols + 1) * kStates; i += kBlock) {
+    for (int i = tid; i < kMaxSymbols * kStates; i += kBlock) {
         const bool in = i < n_sym * kStates;
         dscale[i / kStates][i % kStates] = in ? colscale[i] : 1.f;
         dscale_lo[i / kStates][i % kStates] = in ? colscale[n_sym * kStates + i] : 0.f;
-        if (i < kMaxSymbols * kStates) {
-            dcq[i / kStates][i % kStates] = in ? colscale[2 * n_sym * kStates + i] : 0.f;
-            dcq_lo[i / kStates][i % kStates] = in ? colscale[3 * n_sym * kStates + i] : 0.f;
-        }
+        dcq[i / kStates][i % kStates] = in ? colscale[2 * n_sym * kStates + i] : 0.f;
+        dcq_lo[i / kStates][i % kStates] = in ? colscale[3 * n_sym * kStates + i] : 0.f;
     }
-    if (tid < 2 * kSeqPerGroup) estep[tid / kSeqPerGroup][tid % kSeqPerGroup] = 0;
+    if (tid < kSeqPerGroup) {
+        expo[tid] = e0;
+        empty[tid] = 0;
+    }
     __syncthreads();
     stage(0);
-    __syncthreads();
-    // The state kept in LDS is already multiplied by the column factors of the
-    // symbol it meets next (when that step runs on the rounded matrices).
-    for (int i = tid; i < kSeqPerGroup * kStates; i += kBlock) {
-        const int s = i / kStates, k = i % kStates;
-        const int c0 = syms[s][0];
-        state[s][k] = c0 < n_sym ? fmaf(s0[k], dscale[c0][k], s0[k] * dscale_lo[c0][k]) : s0[k];
-    }
-    int expo_home = e0;   // wave 0, lane j: log2-scale of sequence j
+    stage(kChunk);
+    for (int i = tid; i < kSeqPerGroup * kStates; i += kBlock) state[i / kStates][i % kStates] = s0[i % kStates];
     const int steps = max_len - 1;
     __syncthreads();
 
-    for (int t = 0; t < steps; ++t) {
-        if (t % kChunk == 0 && t > 0) {
-            stage(t);
-            __syncthreads();
+    // Which sequence each of this wave's tile columns holds at step t: the
+    // sequences whose symbol the wave owns, in lane order. Each lane pushes
+    // its index and symbol to lane rank-among-the-selected, the others to
+    // lanes past cnt (a permutation, one ds_permute); two lane swaps then
+    // copy columns 0..15 to every row of 16 lanes. It reads only the symbol
+    // ring, so the next step's routing is worked out during this step's
+    // tile, in three parts (route_part).
+    struct Route {
+        int cj;       // lane j: symbol of sequence j at this step
+        int cnt;      // sequences on this wave's symbols
+        int colseq;   // lane n: sequence of column n (bits 0-7) and its symbol (bits 8-15)
+        int seq, c;   // the first tile's column (lane & 15): its sequence and symbol
+    };
+    auto route_part = [&](Route& r, int t, int part) {
+        if (part == 0) r.cj = syms[lane][t % kRing];
+        if (part == 1) {
+            const bool sel = r.cj < n_sym && r.cj / SPW == wave;
+            const uint64_t m = __ballot(sel);
+            r.cnt = __popcll(m);
+            const int below =
+                (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            const int dst = sel ? below : r.cnt + (lane - below);
+            r.colseq = __builtin_amdgcn_ds_permute(dst << 2, lane | (r.cj << 8));
         }
-        const int par = t & 1;
-        const int tc = t % kChunk;
-        if (wave == 0 && t > 0) expo_home += estep[par ^ 1][lane];
-        const int cj = syms[lane][tc];
-        if (wave == 0 && cj >= n_sym) {
-            if (cj == kBad) {   // a symbol outside the alphabet empties the state
-                float4* row = reinterpret_cast<float4*>(&state[lane][0]);
-                for (int k = 0; k < kStates / 4; ++k) row[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (part == 2) {
+            const unsigned v0 = (unsigned)r.colseq;
+            const auto h = __builtin_amdgcn_permlane32_swap(v0, v0, false, false);   // lanes 32..63 <- 0..31
+            const auto q = __builtin_amdgcn_permlane16_swap(h[0], h[0], false, false);   // odd rows <- even rows
+            const int v = (int)q[0];
+            r.seq = v & 0xff;
+            // a lane past the tile's columns reads any row of the symbol tables
+            r.c = (lane & 15) < r.cnt ? (v >> 8) & 0xff : wave * SPW;
+        }
+    };
+
+    // One tile: up to 16 sequences (column n = lane & 15) on any of this
+    // wave's symbols; the k dimension runs over the SPW symbols' matrices
+    // (SPW x 64), and each column's state sits only in its own symbol's k
+    // block. hook(-2) runs at the start, hook(-1) after the operand split,
+    // hook(q) after symbol q's products (issued or skipped): the next step's
+    // routing goes out between them.
+    auto tile = [&](int seq, int c, bool valid, auto&& hook) {
+        const int myq = valid ? c - wave * SPW : -1;
+        hook(-2);
+        vec bhi[2], blo[2];
+        float mass_in = 0.f;   // sum_k colsum(round T[c])_k x_k over this lane's 16 k
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) {
+            const int k0 = kb * 32 + koff;
+            const float4* src = reinterpret_cast<const float4*>(&state[seq][k0]);
+            const float4 x0 = src[0], x1 = src[1];
+            float x[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+            const float4* fh = reinterpret_cast<const float4*>(&dscale[c][k0]);
+            const float4* fo = reinterpret_cast<const float4*>(&dscale_lo[c][k0]);
+            const float4 fa = fh[0], fb = fh[1], oa = fo[0], ob = fo[1];
+            const float f[8] = {fa.x, fa.y, fa.z, fa.w, fb.x, fb.y, fb.z, fb.w};
+            const float fl[8] = {oa.x, oa.y, oa.z, oa.w, ob.x, ob.y, ob.z, ob.w};
+            const float4* qh = reinterpret_cast<const float4*>(&dcq[c][k0]);
+            const float4* ql = reinterpret_cast<const float4*>(&dcq_lo[c][k0]);
+            const float4 qa = qh[0], qb = qh[1], ra = ql[0], rb2 = ql[1];
+            const float q[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+            const float rr[8] = {ra.x, ra.y, ra.z, ra.w, rb2.x, rb2.y, rb2.z, rb2.w};
+            // the symbol's column factors: the rounded matrix then moves the
+            // mass the FP32 one does
+#pragma unroll
+            for (int e = 0; e < 8; ++e) x[e] = fmaf(x[e], f[e], x[e] * fl[e]);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) mass_in = fmaf(x[e], q[e], fmaf(x[e], rr[e], mass_in));
+            split8<BF16, SPLIT>(x, bhi[kb], blo[kb]);
+        }
+        hook(-1);
+        f32x4 d[4];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) d[rb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < SPW; ++q) {
+            const bool mine = myq == q;
+            if (__ballot(mine) == 0) {   // no column of this tile on symbol q
+                hook(q);
+                continue;
             }
-            estep[par][lane] = 0;
-        }
-        // One tile holds up to 16 sequences on any of this wave's symbols: the
-        // k dimension runs over the SPW symbols' matrices (SPW x 64), and each
-        // column's state sits only in its own symbol's k block.
-        const bool sel = cj < n_sym && cj / SPW == wave;
-        const uint64_t m = __ballot(sel);
-        const int cnt = __popcll(m);
-        // column n of the wave's tiles = the n-th selected sequence: each lane
-        // (a sequence) pushes its index to lane rank-among-the-selected, the
-        // others to lanes past cnt (a permutation, one ds_permute)
-        const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-        const int dst = sel ? below : cnt + (lane - below);
-        const int colseq = __builtin_amdgcn_ds_permute(dst << 2, lane);
-        for (int g = 0; g * 16 < cnt; ++g) {
-            const int n = g * 16 + (lane & 15);
-            const bool valid = n < cnt;
-            const int seq = __shfl(colseq, valid ? n : g * 16, 64);
-            const int c = syms[seq][tc];
-            const int cn = syms[seq][tc + 1];
-            const int myq = valid ? c - wave * SPW : -1;
-            vec bhi[2], blo[2];
-            float mass_in = 0.f;   // sum_k colsum(round T[c])_k x_k over this lane's 16 k
+            const vec zero{};
+            vec bh[2], bl[2];
 #pragma unroll
             for (int kb = 0; kb < 2; ++kb) {
-                const float4* src = reinterpret_cast<const float4*>(&state[seq][kb * 32 + koff]);
-                const float4 x0 = src[0], x1 = src[1];
-                const float x[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
-                const float4* qh = reinterpret_cast<const float4*>(&dcq[c][kb * 32 + koff]);
-                const float4* ql = reinterpret_cast<const float4*>(&dcq_lo[c][kb * 32 + koff]);
-                const float4 qa = qh[0], qb = qh[1], ra = ql[0], rb2 = ql[1];
-                const float q[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
-                const float r[8] = {ra.x, ra.y, ra.z, ra.w, rb2.x, rb2.y, rb2.z, rb2.w};
-#pragma unroll
-                for (int e = 0; e < 8; ++e) mass_in = fmaf(x[e], q[e], fmaf(x[e], r[e], mass_in));
-                split8<BF16, SPLIT>(x, bhi[kb], blo[kb]);
+                bh[kb] = mine ? bhi[kb] : zero;
+                if constexpr (SPLIT) bl[kb] = mine ? blo[kb] : zero;
             }
-            f32x4 d[4];
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb) d[rb] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int q = 0; q < SPW; ++q) {
-                const bool mine = myq == q;
-                if (__ballot(mine) == 0) continue;   // no column of this tile on symbol q
-                const vec zero{};
-                vec bh[2], bl[2];
-#pragma unroll
-                for (int kb = 0; kb < 2; ++kb) {
-                    bh[kb] = mine ? bhi[kb] : zero;
-                    if constexpr (SPLIT) bl[kb] = mine ? blo[kb] : zero;
-                }
-#pragma unroll
-                for (int kb = 0; kb < 2; ++kb)
-#pragma unroll
-                    for (int rb = 0; rb < 4; ++rb) {
-                        d[rb] = Lp<BF16>::mfma(A[q][rb][kb], bh[kb], d[rb]);
-                        if constexpr (SPLIT) d[rb] = Lp<BF16>::mfma(A[q][rb][kb], bl[kb], d[rb]);
-                    }
-            }
-            // column factors of the next symbol for this lane's 16 result rows
-            // (none when the next step is the last one or the sequence ends)
-            float4 f[4], fl[4];
-            const int fr = cn < n_sym ? cn : kMaxSymbols;   // the identity row otherwise
-#pragma unroll
-            for (int rb = 0; rb < 4; ++rb) {
-                f[rb] = *reinterpret_cast<const float4*>(&dscale[fr][rb * 16 + rsub]);
-                fl[rb] = *reinterpret_cast<const float4*>(&dscale_lo[fr][rb * 16 + rsub]);
-            }
-            float mx = 0.f, mass_out = 0.f;
-#pragma unroll
-            for (int rb = 0; rb < 4; ++rb) {
-                mx = fmaxf(mx, fmaxf(fmaxf(fabsf(d[rb][0]), fabsf(d[rb][1])), fmaxf(fabsf(d[rb][2]), fabsf(d[rb][3]))));
-                mass_out += (d[rb][0] + d[rb][1]) + (d[rb][2] + d[rb][3]);
-            }
-            // the product's mass back to what its operands carry: the matrix cores'
-            // FP32 accumulation loses a little mass on every step, in one direction
-            mass_in = column_sum(mass_in);
-            mass_out = column_sum(mass_out);
-            const float fix = mass_out > 0.f && mass_in > 0.f ? mass_in / mass_out : 1.f;
-            mx = column_max(mx) * fix;
-            const int e2 = mx > 0.f ? __builtin_amdgcn_frexp_expf(mx) : 0;   // mx = f * 2^e2, f in [0.5, 1)
-            if (valid) {
-                // fix and the power of two in one factor: scaling by 2^-e2
-                // commutes with each rounding, so this equals applying it last
-                const float sc = ldexpf(fix, -e2);
+            for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
                 for (int rb = 0; rb < 4; ++rb) {
-                    const f32x4 dv = d[rb] * sc;
-                    const float4 out = make_float4(fmaf(dv[0], f[rb].x, dv[0] * fl[rb].x),
-                                                   fmaf(dv[1], f[rb].y, dv[1] * fl[rb].y),
-                                                   fmaf(dv[2], f[rb].z, dv[2] * fl[rb].z),
-                                                   fmaf(dv[3], f[rb].w, dv[3] * fl[rb].w));
-                    *reinterpret_cast<float4*>(&state[seq][rb * 16 + rsub]) = out;
+                    d[rb] = Lp<BF16>::mfma(A[q][rb][kb], bh[kb], d[rb]);
+                    if constexpr (SPLIT) d[rb] = Lp<BF16>::mfma(A[q][rb][kb], bl[kb], d[rb]);
                 }
-                if ((lane >> 4) == 0) estep[par][seq] = e2;
+            hook(q);
+        }
+        float mx = 0.f, mass_out = 0.f;
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            mx = fmaxf(mx, fmaxf(fmaxf(fabsf(d[rb][0]), fabsf(d[rb][1])), fmaxf(fabsf(d[rb][2]), fabsf(d[rb][3]))));
+            mass_out += (d[rb][0] + d[rb][1]) + (d[rb][2] + d[rb][3]);
+        }
+        // the product's mass back to what its operands carry: the matrix cores'
+        // FP32 accumulation loses a little mass on every step, in one direction
+        mass_in = column_sum(mass_in);
+        mass_out = column_sum(mass_out);
+        // the quotient by one Newton step on the reciprocal: within half an
+        // ulp and unbiased, as the fix is applied on every step (the
+        // reciprocal alone errs in one direction, a drift over 10^5 steps)
+        float fix = 1.f;
+        if (mass_out > 0.f && mass_in > 0.f) {
+            const float rcp = __builtin_amdgcn_rcpf(mass_out);
+            const float q0 = mass_in * rcp;
+            fix = fmaf(fmaf(-mass_out, q0, mass_in), rcp, q0);
+        }
+        mx = column_max(mx) * fix;
+        const int e2 = mx > 0.f ? __builtin_amdgcn_frexp_expf(mx) : 0;   // mx = f * 2^e2, f in [0.5, 1)
+        if (valid) {
+            // fix and the power of two in one factor: scaling by 2^-e2
+            // commutes with each rounding, so this equals applying it last
+            const float sc = ldexpf(fix, -e2);
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
+                const f32x4 dv = d[rb] * sc;
+                *reinterpret_cast<float4*>(&state[seq][rb * 16 + rsub]) = make_float4(dv[0], dv[1], dv[2], dv[3]);
+            }
+            if ((lane >> 4) == 0) atomicAdd(&expo[seq], e2);   // the step's owner alone
+        }
+    };
+
+    Route r;
+    for (int part = 0; part < 3; ++part) route_part(r, 0, part);
+    for (int t = 0; t < steps; ++t) {
+        // the chunk after this one, into the half of the ring the previous
+        // chunk held (first read at step t + kChunk - 1, barriers apart)
+        if (t % kChunk == 0 && t > 0) stage(t + kChunk);
+        Route next;
+        // the first tile always runs (with no column named it does no
+        // products); the next step's routing goes out between its products
+        tile(r.seq, r.c, (lane & 15) < r.cnt, [&](int point) {
+            if (point < 1) route_part(next, t + 1, point + 2);
+        });
+        if (r.cnt > 16) {   // more tiles: the column map again (not kept through the first tile)
+            Route again;
+            route_part(again, t, 0);
+            route_part(again, t, 1);
+            for (int g = 1; g * 16 < r.cnt; ++g) {
+                const int n = g * 16 + (lane & 15);
+                const bool valid = n < r.cnt;
+                const int v = __shfl(again.colseq, valid ? n : g * 16, 64);
+                tile(v & 0xff, (v >> 8) & 0xff, valid, [](int) {});
             }
         }
         __syncthreads();
+        r = next;
     }
-    if (wave == 0) {
-        if (steps >= 1) expo_home += estep[(steps - 1) & 1][lane];
-        expo[lane] = expo_home;
-    }
-    __syncthreads();
 
     // last step in FP32 from the unrounded T: kTps threads per sequence
     constexpr int kRows = kStates / kTps;
@@ -305,8 +343,8 @@ __device__ void pnfa_forward(const uint16_t* __restrict__ tlp, const float* __re
     }
     if (s < nseq && part == 0) {
         const double scale = (double)expo[s] * 0.69314718055994530942;   // ln 2
-        out_ll[seq0 + s] = log(acc_a) + scale;
-        out_logmass[seq0 + s] = log(acc_m) + scale;
+        out_ll[seq0 + s] = empty[s] ? -INFINITY : log(acc_a) + scale;
+        out_logmass[seq0 + s] = empty[s] ? -INFINITY : log(acc_m) + scale;
     }
 }
 
