@@ -367,6 +367,43 @@ def test_gpu_ragged_sequences(gpu_lib, waves):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("waves", [4, 8])
+def test_gpu_outside_symbols_at_staging_edges(gpu_lib, waves):
+    """Symbols outside the alphabet at the positions where the kernel stages
+    its next 64 symbols (0, 63, 64, 127, 128, ...), at a sequence's last
+    rounded step (length - 2) and at its last symbol (length - 1): those
+    sequences give -inf for both outputs, every other sequence of the launch
+    matches the FP64 oracle."""
+    rng = np.random.default_rng(23)
+    T, start, accept = dirichlet_pnfa(rng, 32)
+    n = 150
+    lengths = rng.integers(200, 330, size=n)
+    offsets = np.zeros(n + 1, np.int64)
+    np.cumsum(lengths, out=offsets[1:])
+    data = rng.integers(0, 32, size=int(offsets[-1])).astype(np.uint8)
+    cases = [0, 1, 62, 63, 64, 65, 127, 128, 129, 191, 192]
+    bad = []
+    for i, pos in enumerate(cases):
+        data[offsets[2 * i] + pos] = 32 + i          # outside the 32-symbol alphabet
+        bad.append(2 * i)
+    for i, back in enumerate((2, 1)):                # last rounded step, last symbol
+        j = 2 * len(cases) + 2 * i
+        data[offsets[j + 1] - back] = 255
+        bad.append(j)
+    eng = _engine(T, start, accept)
+    eng.set_waves(waves)
+    ll, lm = eng.loglik_ragged(data, offsets)
+    bad = np.array(bad)
+    assert np.all(ll[bad] == -np.inf) and np.all(lm[bad] == -np.inf)
+    good = np.setdiff1d(np.arange(n), bad)
+    for i in good:
+        sy = data[offsets[i]:offsets[i + 1]][None, :]
+        ref_ll, ref_lm = oprob.forward_loglik(T, start, accept, sy)
+        assert _rel(ll[i:i + 1], ref_ll).max() <= SPLIT_TOL, i
+        assert abs(lm[i] - ref_lm[0]) <= SPLIT_TOL * abs(ref_ll[0]) + 1e-5, i
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("grid", [1, 3])
 def test_gpu_grid_cap_same_results(gpu_lib, grid):
     """A grid cap splits a call into consecutive launches of at most `grid`
